@@ -1,0 +1,230 @@
+"""Benchmark: elastic FD / RTM hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sh_layered]
+
+One step = one single-shot RTM of the workload: forward modeling with a
+snapshot every isnap steps, then backward propagation of the recorded
+seismogram with the zero-lag image accumulated in the step kernel
+(the reference's rtm.py:935-1000 sequence).  Inputs (medium, wavelets) are
+resident in HBM before the timed region; the state is reset between steps.
+
+value  = useful cell-updates / second (2 * nx * nz * nt per shot: forward and
+         backward steps; nothing is recomputed).
+roofline: the dominant kernel is the backward step with imaging; its average
+         launch time is measured with HIP events on the launch stream over the
+         timed steps; algorithmic bytes per cell per launch are listed in
+         ALGO_BYTES.
+cpu_baseline: the oracle (C/OpenMP restatement of the reference's unfused
+         per-step kernel sequence, float32, both wave systems updated as the
+         reference does) timed on a bounded sample of the same workload.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GCell-updates/s + RTM shots/hour on 1 MI355X; % HBM roofline; vs Taichi CPU"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per cell per launch (float32 words; sponge from the
+# cell index, 0 bytes; source/receiver traffic ignored):
+#   SH forward step with snapshot:   v, syx, syz read+write 24 + myx, myz, ir 12 + snapshot write 4
+#   SH backward step with imaging:   24 + 12 + forward snapshot read 4 + image read+write 8
+#   P-SV forward with snapshot:      u, w, sxx, szz, sxz r+w 40 + lam, mu, mxz, iru, irw 20 + u, w snapshot 8
+#   P-SV backward with imaging:      40 + 20 + u, w snapshot 8 + u, w image r+w 16
+ALGO_BYTES = {
+    ("sh", "forward"): 40, ("sh", "backward"): 48,
+    ("psv", "forward"): 68, ("psv", "backward"): 84,
+}
+
+
+def build_workload(name):
+    from taichirtm_amd import configs
+    if name == "sh_layered":
+        return configs.sh_layered(), "sh"
+    if name == "psv_anomaly":
+        return configs.psv_anomaly(), "psv"
+    raise SystemExit("unknown workload %r" % name)
+
+
+class RTMShot:
+    """Forward + backward propagators of one shot, built once."""
+
+    def __init__(self, cfg, isnap, kernel):
+        import taichirtm_amd as trtm
+        shot = cfg.shots[0]
+        self.kw = cfg.kwargs(shot)
+        wu, wv, ww = cfg.wavelets(shot)
+        self.isnap = isnap
+        self.fw = trtm.ForwardModeling(wavelet_u_field=wu, wavelet_v_field=wv,
+                                       wavelet_w_field=ww, isnap=isnap, kernel=kernel, **self.kw)
+        nr, nt = len(shot.rec), shot.nt
+        zero = np.zeros((nr, nt), np.float32)
+        # observed data: the forward's own seismogram, copied on the device each step
+        obs = [zero if c not in self.fw._seis else self.fw._seis[c].new_zeros((nr, nt))
+               for c in ("u", "v", "w")]
+        self.obs = obs
+        self.nt = nt
+        # one forward pass fills the observed data, so the backward propagator
+        # sees which wave systems carry data when it is built
+        if self.fw.run(save=True):
+            raise RuntimeError("forward modeling became non-finite")
+        self._copy_obs()
+        self.bw = trtm.BackwardModeling(obsdata_u_field=obs[0], obsdata_v_field=obs[1],
+                                        obsdata_w_field=obs[2], kernel=kernel, **self.kw)
+
+    def _copy_obs(self):
+        from taichirtm_amd import _lib
+        for c, t in self.fw._seis.items():
+            o = self.obs[("u", "v", "w").index(c)]
+            _lib.call("trtm_transpose", self.nt, o.shape[0], ctypes_ptr(t), ctypes_ptr(o),
+                      stream())
+
+    def run(self, ev=None):
+        self.fw.reset_state()
+        self.bw.reset_state()
+        if ev:
+            ev[0].record()
+        code = self.fw.run(save=True)
+        if ev:
+            ev[1].record()
+        # forward seismogram -> observed data of the backward pass (device transpose)
+        self._copy_obs()
+        if ev:
+            ev[2].record()
+        code_b = self.bw.run_calc(self.fw.u_save_field, self.fw.v_save_field,
+                                  self.fw.w_save_field, self.fw.isnaps_field, self.fw.num_snaps,
+                                  self.isnap)
+        if ev:
+            ev[3].record()
+        if code or code_b:
+            raise RuntimeError("non-finite wavefield (codes %d, %d)" % (code, code_b))
+
+
+def ctypes_ptr(t):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream():
+    from taichirtm_amd.field import stream_handle
+    return stream_handle()
+
+
+def cpu_baseline(cfg, target_s=12.0):
+    """Oracle forward + backward (with imaging every step) on a truncated
+    record of the same model; rate in cell-updates per second."""
+    import oracle
+    shot = cfg.shots[0]
+    mu, lam = cfg.moduli()
+
+    def timed(nt):
+        m = oracle.Model(mu, lam, cfg.rho, cfg.dx, cfg.dz, cfg.fs, nt, [shot.src], shot.rec,
+                         cfg.fw, isnap=1, sci=100)
+        wu, wv, ww = (w[:, :nt] for w in cfg.wavelets(shot))
+        t0 = time.perf_counter()
+        run = oracle.rtm(m, [wu, wv, ww], isnap=1)
+        dt = time.perf_counter() - t0
+        assert run["fw"]["code"] == 0 and run["bw"]["code"] == 0
+        return dt
+
+    probe = 20
+    t_probe = timed(probe)
+    nt = int(min(cfg.nt, max(probe, probe * target_s / max(t_probe, 1e-3))))
+    t = timed(nt)
+    cells = 2.0 * cfg.nx * cfg.nz * nt
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": cells / t / 1e9, "unit": "GCell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": "oracle float32 forward+backward+image (isnap=1), %dx%d grid, %d of %d "
+                      "steps each way, %.1f s" % (cfg.nx, cfg.nz, nt, cfg.nt, t),
+            "shots_per_hour": 3600.0 / (t * cfg.nt / nt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="sh_layered")
+    ap.add_argument("--kernel", default="fused")
+    ap.add_argument("--isnap", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU; multi-GPU runs are not part of this build "
+              "(shots shard across GPUs with one image all-reduce)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    torch.cuda.set_device(0)
+    cfg, system = build_workload(args.workload)
+    shot = RTMShot(cfg, args.isnap, args.kernel)
+    for _ in range(args.warmup):
+        shot.run()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        shot.run(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fw_ms = [e[0].elapsed_time(e[1]) for e in evs]
+    bw_ms = [e[2].elapsed_time(e[3]) for e in evs]
+
+    nt = cfg.nt
+    cells = cfg.nx * cfg.nz
+    ms_per_step = wall * 1000.0 / args.steps
+    gcells = 2.0 * cells * nt / (ms_per_step / 1000.0) / 1e9
+    bw_launch_us = 1000.0 * float(np.mean(bw_ms)) / nt
+    fw_launch_us = 1000.0 * float(np.mean(fw_ms)) / nt
+    b_bytes = ALGO_BYTES[(system, "backward")] * cells
+    f_bytes = ALGO_BYTES[(system, "forward")] * cells
+    achieved = b_bytes / (bw_launch_us * 1e-6) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(gcells, 3),
+        "unit": "GCell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "%s single-shot RTM (forward + backward + zero-lag image, isnap=%d)"
+                               % (cfg.name, args.isnap),
+                   "grid": [cfg.nx, cfg.nz], "nt": nt, "dx": cfg.dx, "dt": cfg.dt,
+                   "absorbing_frame": cfg.fw, "receivers": len(cfg.shots[0].rec),
+                   "kernel": args.kernel, "system": system},
+        "shots_per_hour": round(3600.0 / (ms_per_step / 1000.0), 1),
+        "phases_ms": {"forward": round(float(np.mean(fw_ms)), 3),
+                      "backward": round(float(np.mean(bw_ms)), 3)},
+        "roofline": {"bound": "hbm", "kernel": "k_fused backward step with imaging",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": b_bytes,
+                     "avg_launch_us": round(bw_launch_us, 3),
+                     "forward_launch_us": round(fw_launch_us, 3),
+                     "forward_achieved": round(f_bytes / (fw_launch_us * 1e-6) / 1e9, 1)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

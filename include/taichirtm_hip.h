@@ -1,0 +1,193 @@
+/*
+ * taichirtm_hip.h — C ABI of the MI355X (gfx950) elastic FD / RTM hot path.
+ *
+ * Drop-in boundary for mkt-kuno/TaichiRTM's data-parallel path: the Taichi
+ * kernels of src/forward_modeling.py and src/backward_modeling.py and the time
+ * loops that sequence them.  The reference exposes that path only as Python
+ * classes over Taichi JIT kernels; the Python package `taichirtm_amd` keeps
+ * those classes (same names, kwargs, return codes) and binds this ABI with
+ * ctypes.  Citations are file:line in the reference's src/.
+ *
+ * Conventions
+ *  - Grid fields are (nx, nz) float32, C order, z (j) contiguous — the
+ *    reference's ti.field layout (forward_modeling.py:158-190).
+ *  - All pointers named in a descriptor are DEVICE pointers unless the field
+ *    name ends in _host.  The library never allocates caller-visible memory;
+ *    scratch comes from the caller after trtm_workspace_bytes().
+ *  - `stream` is a hipStream_t (NULL = default stream).  Calls are
+ *    asynchronous on that stream unless stated.
+ *  - Return values: TRTM_OK (0), or a negative TRTM_ERR_* with a message from
+ *    trtm_last_error() (thread-local).  Blow-up is not an error return: it is
+ *    reported through check_flags with the reference's codes 1-3 (forward) and
+ *    4-6 (backward) (forward_modeling.py:522-529, backward_modeling.py:533-537).
+ *  - Thread safety: no global mutable state; distinct descriptors may be used
+ *    concurrently on distinct streams.
+ */
+#ifndef TAICHIRTM_HIP_H
+#define TAICHIRTM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TRTM_ABI_VERSION 1
+
+#define TRTM_OK 0
+#define TRTM_ERR_ARG (-1)         /* invalid argument or shape */
+#define TRTM_ERR_HIP (-2)         /* HIP runtime error */
+#define TRTM_ERR_UNSUPPORTED (-3) /* request outside what this build supports */
+
+/* wave systems (systems field): the P-SV (u, w, sxx, szz, sxz) and SH (v, syx,
+ * syz) systems are decoupled (forward_modeling.py:330-347,367-384) */
+#define TRTM_PSV 1
+#define TRTM_SH 2
+
+/* step kernel selection (kernel field) */
+#define TRTM_KERNEL_AUTO 0
+#define TRTM_KERNEL_SIMPLE 1 /* one launch per reference kernel group */
+#define TRTM_KERNEL_FUSED 2  /* one LDS-tiled launch per time step */
+
+/* ABI version compiled into the library (== TRTM_ABI_VERSION). */
+int trtm_abi_version(void);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *trtm_last_error(void);
+
+/*
+ * Averaged moduli, densities and reciprocals.
+ * Replaces ForwardModeling._compute_shear_avg / _compute_rho_avg /
+ * _compute_inv_rho (forward_modeling.py:243-307; identical in
+ * backward_modeling.py:246-310).  myx, myz, mxz, rho_u, rho_w are written on
+ * the interior and zeroed on the border; inv_* on the full grid (1/0 = +inf on
+ * the averaged borders, as in the reference).  Expression order follows the
+ * reference operand for operand, so float32 overflow of mu^4 in mxz (for mu
+ * above ~4.3e9) is reproduced.
+ */
+int trtm_prepare_medium(int32_t nx, int32_t nz, const float *mu, const float *rho,
+                        float *myx, float *myz, float *mxz, float *rho_u, float *rho_w,
+                        float *inv_rho, float *inv_rho_u, float *inv_rho_w, void *stream);
+
+/*
+ * Sponge taper table, host side: table_host[k] = expf((-a*a*k)*k) in float32
+ * for k = 0..fw (fw+1 entries).  The step kernels evaluate the Cerjan sponge
+ * of forward_modeling.py:203-241 from the cell index and this table.
+ */
+int trtm_sponge_table(int32_t fw, float a, float *table_host);
+
+/*
+ * Full (nx, nz) sponge coefficient array, for the reference's absorb_coeff
+ * field (forward_modeling.py:203-241): left strip, then right, then bottom,
+ * later writes winning; requires 0 <= fw <= min(nx, nz).  table: device copy
+ * of trtm_sponge_table's output.
+ */
+int trtm_sponge_fill(int32_t nx, int32_t nz, int32_t fw, const float *table, float *coeff,
+                     void *stream);
+
+/*
+ * Injection plan, host side.  The reference adds every source (or receiver)
+ * entry with a parallel `+=` (forward_modeling.py:432-443,
+ * backward_modeling.py:455-460), i.e. atomically.  This library groups entries
+ * by cell so that duplicates are summed in entry order, deterministically.
+ * loc_host: (n, 2) int32 (i, j).  Outputs: order_host[n] (entry indices sorted
+ * by cell, stable), group_start_host[n+1]; *n_groups receives the number of
+ * distinct cells.  Fails if a location is outside [0,nx) x [0,nz).
+ */
+int trtm_plan_injection(int32_t nx, int32_t nz, int32_t n, const int32_t *loc_host,
+                        int32_t *order_host, int32_t *group_start_host, int32_t *n_groups);
+
+/* One propagator: medium, geometry, state and I/O of a forward or backward run. */
+typedef struct trtm_step_desc {
+    int32_t struct_size; /* = sizeof(trtm_step_desc) */
+    int32_t nx, nz, nt;
+    float dt;         /* float32(1/fs) (forward_modeling.py:135,320) */
+    float inv_dx;     /* float32(1/dx) */
+    float inv_dz;     /* float32(1/dz) */
+    float src_factor; /* float32(dt*dx*dz) (forward_modeling.py:430) */
+    int32_t fw;       /* absorbing_frame */
+    int32_t systems;  /* TRTM_PSV | TRTM_SH: systems that can become nonzero */
+    int32_t kernel;   /* TRTM_KERNEL_* */
+    int32_t reserved0;
+    const float *sponge_table; /* fw+1 floats (trtm_sponge_table) */
+    /* medium (trtm_prepare_medium outputs, plus the caller's lam and mu) */
+    const float *lam, *mu, *mxz, *myx, *myz, *inv_rho, *inv_rho_u, *inv_rho_w;
+    const float *surface_matrix; /* NULL: flat free surface */
+    /* state, updated in place */
+    float *u, *v, *w, *sxx, *szz, *sxz, *syx, *syz;
+    /* injection: forward = sources with wavelets (scaled by src_factor/rho);
+     * backward = receivers with observed data (raw).  (n_inj, nt) each,
+     * reference layout; a component pointer may be NULL when its system is
+     * inactive. */
+    int32_t n_inj, n_inj_groups;
+    const int32_t *inj_loc;         /* (n_inj, 2) */
+    const int32_t *inj_order;       /* trtm_plan_injection order */
+    const int32_t *inj_group_start; /* trtm_plan_injection group starts */
+    const float *inj_u, *inj_v, *inj_w;
+    /* recording: forward = receivers (seismogram); backward = sources
+     * (synthetic source, backward_modeling.py:462-470).  Time-major (nt, n_rec);
+     * NULL to skip. */
+    int32_t n_rec, reserved1;
+    const int32_t *rec_loc; /* (n_rec, 2) */
+    float *rec_u, *rec_v, *rec_w;
+    /* finite check every check_interval steps (it % check_interval == 0);
+     * check_flags has nt/check_interval + 1 ints, zeroed by the caller; entry
+     * it/check_interval receives the highest code found at step it. */
+    int32_t check_interval, reserved2;
+    int32_t *check_flags;
+    /* snapshots: forward writes slot it/isnap-1 at it % isnap == 0, it != 0;
+     * backward correlates them into the image.  Snapshot-major
+     * (num_snaps, nx, nz) layout. */
+    int32_t isnap, num_snaps;
+    float *snap_u, *snap_v, *snap_w;
+    const int32_t *isnaps_host; /* backward only: HOST array of num_snaps step indices */
+    float *image_u, *image_v, *image_w;
+    /* scratch of trtm_workspace_bytes() bytes (fused kernel), may be NULL otherwise */
+    void *workspace;
+    int64_t workspace_bytes;
+} trtm_step_desc;
+
+/* Scratch bytes trtm_forward_steps / trtm_backward_steps need for desc. */
+int64_t trtm_workspace_bytes(const trtm_step_desc *desc);
+
+/*
+ * Forward steps it_begin <= it < it_end.  Replaces the body of
+ * ForwardModeling.run (forward_modeling.py:539-572): per step, boundary
+ * condition, velocity update, source injection (wavelet column it), stress
+ * update, sponge, receiver record (row it of rec_*), finite check when
+ * it % check_interval == 0, and, when save != 0, the snapshot at
+ * it % isnap == 0, it != 0.
+ */
+int trtm_forward_steps(const trtm_step_desc *desc, int32_t it_begin, int32_t it_end, int32_t save,
+                       void *stream);
+
+/*
+ * Backward steps it_begin <= it < it_end, t = nt-1-it.  Replaces the body of
+ * BackwardModeling.run_calc (backward_modeling.py:539-576): boundary
+ * condition, backward velocity update, observed-data injection (column t),
+ * backward stress update, sponge, synthetic-source record (row t of rec_*),
+ * zero-lag image accumulation image += snap[s] * state when t > 0,
+ * t % isnap == 0, s = t/isnap - 1 < num_snaps and isnaps_host[s] == t, and the
+ * finite check (codes 4-6) when it % check_interval == 0.
+ */
+int trtm_backward_steps(const trtm_step_desc *desc, int32_t it_begin, int32_t it_end,
+                        void *stream);
+
+/*
+ * Standalone finite check of u, v, w over the full grid: NaN or |x| > 1e30
+ * gives code base+1 / base+2 / base+3, the highest found is atomically
+ * max-ed into *flag (device).  Replaces _check_finite
+ * (forward_modeling.py:455-488 with base 0; backward_modeling.py:417-450
+ * with base 3).
+ */
+int trtm_check_finite(int32_t nx, int32_t nz, const float *u, const float *v, const float *w,
+                      int32_t base, int32_t *flag, void *stream);
+
+/* dst (cols, rows) = transpose of src (rows, cols), float32, device. */
+int trtm_transpose(int32_t rows, int32_t cols, const float *src, float *dst, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TAICHIRTM_HIP_H */

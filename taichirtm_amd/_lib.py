@@ -1,0 +1,104 @@
+"""ctypes binding of libtaichirtm_hip.so (C ABI declared in include/taichirtm_hip.h).
+
+The product path has no CPU fallback: loading fails loudly when the library
+is missing, and device calls need a visible HIP device.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtaichirtm_hip.so")
+
+ABI_VERSION = 1
+OK = 0
+ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3
+PSV, SH = 1, 2
+KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_FUSED = 0, 1, 2
+
+_p = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_f32 = ctypes.c_float
+
+
+class StepDesc(ctypes.Structure):
+    """Mirror of trtm_step_desc."""
+    _fields_ = [
+        ("struct_size", _i32), ("nx", _i32), ("nz", _i32), ("nt", _i32),
+        ("dt", _f32), ("inv_dx", _f32), ("inv_dz", _f32), ("src_factor", _f32),
+        ("fw", _i32), ("systems", _i32), ("kernel", _i32), ("reserved0", _i32),
+        ("sponge_table", _p),
+        ("lam", _p), ("mu", _p), ("mxz", _p), ("myx", _p), ("myz", _p),
+        ("inv_rho", _p), ("inv_rho_u", _p), ("inv_rho_w", _p),
+        ("surface_matrix", _p),
+        ("u", _p), ("v", _p), ("w", _p), ("sxx", _p), ("szz", _p), ("sxz", _p),
+        ("syx", _p), ("syz", _p),
+        ("n_inj", _i32), ("n_inj_groups", _i32),
+        ("inj_loc", _p), ("inj_order", _p), ("inj_group_start", _p),
+        ("inj_u", _p), ("inj_v", _p), ("inj_w", _p),
+        ("n_rec", _i32), ("reserved1", _i32),
+        ("rec_loc", _p), ("rec_u", _p), ("rec_v", _p), ("rec_w", _p),
+        ("check_interval", _i32), ("reserved2", _i32),
+        ("check_flags", _p),
+        ("isnap", _i32), ("num_snaps", _i32),
+        ("snap_u", _p), ("snap_v", _p), ("snap_w", _p),
+        ("isnaps_host", _p),
+        ("image_u", _p), ("image_v", _p), ("image_w", _p),
+        ("workspace", _p), ("workspace_bytes", ctypes.c_int64),
+    ]
+
+
+# name -> (restype, argtypes) for every entry point of include/taichirtm_hip.h
+SIGNATURES = {
+    "trtm_abi_version": (ctypes.c_int, []),
+    "trtm_last_error": (ctypes.c_char_p, []),
+    "trtm_prepare_medium": (ctypes.c_int, [_i32, _i32] + [_p] * 10 + [_p]),
+    "trtm_sponge_table": (ctypes.c_int, [_i32, _f32, _p]),
+    "trtm_sponge_fill": (ctypes.c_int, [_i32, _i32, _i32, _p, _p, _p]),
+    "trtm_plan_injection": (ctypes.c_int, [_i32, _i32, _i32, _p, _p, _p, _p]),
+    "trtm_workspace_bytes": (ctypes.c_int64, [ctypes.POINTER(StepDesc)]),
+    "trtm_forward_steps": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _i32, _i32, _p]),
+    "trtm_backward_steps": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _i32, _p]),
+    "trtm_check_finite": (ctypes.c_int, [_i32, _i32, _p, _p, _p, _i32, _p, _p]),
+    "trtm_transpose": (ctypes.c_int, [_i32, _i32, _p, _p, _p]),
+}
+
+_LIB = None
+
+
+class HIPError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the library (once) and check its ABI version."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise HIPError(
+                "libtaichirtm_hip.so is not built (%s); run `python -m taichirtm_amd.build`"
+                % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if lib.trtm_abi_version() != ABI_VERSION:
+            raise HIPError("libtaichirtm_hip ABI %d, expected %d"
+                           % (lib.trtm_abi_version(), ABI_VERSION))
+        _LIB = lib
+    return _LIB
+
+
+def check(rc, what):
+    if rc < 0:
+        msg = load().trtm_last_error().decode(errors="replace")
+        if rc == ERR_ARG:
+            raise ValueError("%s: %s" % (what, msg))
+        raise HIPError("%s failed (%d): %s" % (what, rc, msg))
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(load(), name)(*args), name)

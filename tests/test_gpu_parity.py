@@ -1,0 +1,202 @@
+"""Parity of the HIP path against the oracle: forward seismograms, final
+wavefields (all 8 fields), images and return codes, bit for bit, for the fused
+and the simple step kernels.  Bar: bitwise equality (both sides evaluate the
+reference's float32 expressions operand by operand, without contraction)."""
+import numpy as np
+import pytest
+
+import oracle
+from taichirtm_amd import configs
+from tests.parity import assert_bitwise, run_hip, run_oracle, same
+
+pytestmark = pytest.mark.gpu
+KERNELS = ["fused", "simple"]
+
+
+def _case(name):
+    print("\n[case]", name, flush=True)
+
+
+def small_psv(nt=300):
+    return configs.psv_anomaly(nx=60, nz=50, nt=nt, fw=10, radius=8, src=(30, 5),
+                               rec_range=(2, 58), f0=40.0)
+
+
+def small_sh(nt=250):
+    return configs.sh_layered(nx=96, nz=70, nt=nt, fw=12, f0=40.0, seed=3)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("isnap", [1, 7])
+def test_psv_rtm_bitwise(kernel, isnap):
+    _case("psv small kernel=%s isnap=%d" % (kernel, isnap))
+    cfg = small_psv()
+    kw = cfg.kwargs(cfg.shots[0])
+    wav = cfg.wavelets(cfg.shots[0])
+    h = run_hip(kw, wav, isnap=isnap, kernel=kernel)
+    o = run_oracle(kw, wav, isnap=isnap)
+    assert_bitwise(h, o)
+    assert np.abs(h["image"][2]).max() > 0
+    assert same(h["fw"].isnaps_field.to_numpy(), o["isnaps"])
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_sh_rtm_bitwise_border_receivers(kernel):
+    """SH with a receiver on every surface cell, including i = 0 and nx-1,
+    which are never stencil-updated but receive injected data backward."""
+    _case("sh small kernel=%s" % kernel)
+    cfg = small_sh()
+    kw = cfg.kwargs(cfg.shots[0])
+    wav = cfg.wavelets(cfg.shots[0])
+    h = run_hip(kw, wav, isnap=3, kernel=kernel)
+    o = run_oracle(kw, wav, isnap=3)
+    assert_bitwise(h, o)
+    assert np.abs(h["state_b"]["v"][0, 1]) > 0  # border cell carries injected data
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_both_systems_duplicates_and_random_obs(kernel):
+    """All three force components at once, two sources on one cell, duplicate
+    receivers, and backward injection of unrelated data."""
+    _case("both systems kernel=%s" % kernel)
+    cfg = small_psv(nt=180)
+    kw = cfg.kwargs(cfg.shots[0])
+    kw["src_loc_field"] = np.array([[30, 5], [30, 5], [12, 40]], np.int32)
+    kw["num_sources"] = 3
+    rec = np.array([[5, 1], [5, 1], [59, 1], [0, 1], [33, 33], [20, 49]], np.int32)
+    kw["recv_loc_field"], kw["num_receivers"] = rec, len(rec)
+    rng = np.random.default_rng(7)
+    w = configs.ricker(40.0, cfg.nt, cfg.dt)
+    wav = (np.stack([w, -0.5 * w, 0.25 * w]), np.stack([0.3 * w, w, w]),
+           np.stack([w, w, -w]))
+    obs = tuple(rng.standard_normal((len(rec), cfg.nt)).astype(np.float32) * 1e-3
+                for _ in range(3))
+    h = run_hip(kw, wav, isnap=2, kernel=kernel, obs=obs)
+    o = run_oracle(kw, wav, isnap=2, obs=obs)
+    assert_bitwise(h, o)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_topography_surface_matrix(kernel):
+    """surface_matrix multiplies syz, sxz, szz every step (forward_modeling.py:412-419)."""
+    _case("topography kernel=%s" % kernel)
+    cfg = small_psv(nt=200)
+    kw = cfg.kwargs(cfg.shots[0])
+    sm = np.ones((cfg.nx, cfg.nz), np.float32)
+    for i in range(cfg.nx):
+        sm[i, : 2 + (i // 10) % 3] = 0.0
+    w = configs.ricker(40.0, cfg.nt, cfg.dt)
+    wav = (np.zeros_like(w)[None], w[None], w[None])
+    h = run_hip(kw, wav, isnap=4, kernel=kernel, surface=sm)
+    o = run_oracle(kw, wav, isnap=4, surface=sm)
+    assert_bitwise(h, o)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_nonfinite_codes(kernel):
+    _case("finite-check codes kernel=%s" % kernel)
+    cfg = small_psv(nt=300)
+    kw = cfg.kwargs(cfg.shots[0])
+    w = configs.ricker(40.0, cfg.nt, cfg.dt)
+    bad = w.copy()
+    bad[5] = np.nan
+    z = np.zeros_like(w)
+    for wav, sci in (((bad[None], z[None], w[None]), 100), ((bad[None], z[None], w[None]), 1)):
+        h = run_hip(kw, wav, sci=sci, kernel=kernel, backward=False)
+        o = run_oracle(kw, wav, sci=sci, backward=False)
+        assert h["code_f"] == o["code_f"] and h["code_f"] != 0
+        assert same(h["seis"][2][:, : o["it_f"] + 1], o["seis"][2][:, : o["it_f"] + 1])
+    obs = [np.zeros((len(kw["recv_loc_field"]), cfg.nt), np.float32) for _ in range(3)]
+    obs[2][0, cfg.nt - 1 - 7] = np.inf
+    h = run_hip(kw, (z[None], z[None], w[None]), sci=1, kernel=kernel, obs=obs)
+    o = run_oracle(kw, (z[None], z[None], w[None]), sci=1, obs=obs)
+    assert h["code_b"] == o["code_b"] == 6
+
+
+def test_mxz_overflow_reproduced():
+    """SI rock moduli overflow mu^4 in float32 (forward_modeling.py:271-276):
+    the reference's shear stress turns NaN; the HIP path reports the same code."""
+    _case("SI-unit overflow")
+    cfg = small_psv(nt=220)
+    mu, lam = configs.moduli(cfg.vp, cfg.vs, cfg.rho * 1000.0)
+    kw = cfg.kwargs(cfg.shots[0])
+    kw.update(mu_field=mu, lam_field=lam, rho_field=(cfg.rho * 1000.0).astype(np.float32))
+    wav = cfg.wavelets(cfg.shots[0])
+    h = run_hip(kw, wav, backward=False)
+    o = run_oracle(kw, wav, backward=False)
+    assert h["code_f"] == o["code_f"]
+
+
+def test_prepare_and_sponge_bitwise():
+    _case("medium preparation and sponge")
+    import taichirtm_amd as trtm
+    rng = np.random.default_rng(1)
+    for nx, nz, fw in ((70, 66, 20), (7, 20, 5), (130, 65, 0)):
+        mu = rng.uniform(1e6, 9e9, (nx, nz)).astype(np.float32)
+        rho = rng.uniform(1.5, 2.6, (nx, nz)).astype(np.float32)
+        lam = mu.copy()
+        fm = trtm.ForwardModeling(nx=nx, nz=nz, dx=5.0, dz=5.0, nt=4, fs=2500.0, mu_field=mu,
+                                  lam_field=lam, rho_field=rho, absorbing_frame=fw,
+                                  src_loc_field=[[1, 1]], recv_loc_field=[[1, 1]],
+                                  wavelet_u_field=np.zeros((1, 4)), wavelet_v_field=np.zeros((1, 4)),
+                                  wavelet_w_field=np.zeros((1, 4)), num_sources=1, num_receivers=1)
+        ref = oracle.prepare(mu, rho)
+        for k, v in ref.items():
+            assert same(getattr(fm, k).to_numpy(), v), k
+        assert same(fm.absorb_coeff.to_numpy(), oracle.sponge(nx, nz, fw))
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_callback_chunks_and_determinism(kernel):
+    """A display callback splits the run into chunks; results must not change,
+    and two runs must agree bit for bit."""
+    _case("callback chunking kernel=%s" % kernel)
+    cfg = small_psv(nt=123)
+    kw = cfg.kwargs(cfg.shots[0])
+    wav = cfg.wavelets(cfg.shots[0])
+    seen = []
+    cb = lambda u, v, w, it, nx, nz, dx, dz: seen.append((it, float(np.abs(w).sum())))
+    a = run_hip(kw, wav, isnap=10, kernel=kernel, callback=cb)
+    b = run_hip(kw, wav, isnap=10, kernel=kernel)
+    o = run_oracle(kw, wav, isnap=10)
+    assert_bitwise(a, o)
+    assert_bitwise(b, o)
+    fw_its = [it for it, _ in seen[:13]]
+    assert fw_its == list(range(0, 123, 10))
+
+
+def test_psv_anomaly_full_size_parity():
+    """Plain P-SV RTM, 200x200, 1500 steps, isnap = 1: HIP against the float32
+    oracle (bitwise), and the float32 scheme's error against float64 stated."""
+    _case("psv_anomaly 200x200x1500")
+    cfg = configs.psv_anomaly()
+    kw = cfg.kwargs(cfg.shots[0])
+    wav = cfg.wavelets(cfg.shots[0])
+    h = run_hip(kw, wav, isnap=1)
+    o = run_oracle(kw, wav, isnap=1)
+    assert_bitwise(h, o)
+    o64 = run_oracle(kw, wav, isnap=1, dtype=np.float64)
+    e_seis = oracle.relative_l2(h["seis"][2], o64["seis"][2])
+    e_img = oracle.relative_l2(h["image"][2], o64["image"][2])
+    print("[fp64] seismogram w rel-L2 %.3e  image w rel-L2 %.3e" % (e_seis, e_img), flush=True)
+    assert e_seis < 1e-4 and e_img < 1e-3
+
+
+def test_sh_layered_reduced_parity_and_full_size_kernels_agree():
+    """SH 1200x600: bitwise against the oracle over 300 steps; at the full
+    4000 steps the fused and simple kernels agree bit for bit."""
+    _case("sh_layered 1200x600")
+    cfg = configs.sh_layered(nt=300)
+    kw = cfg.kwargs(cfg.shots[0])
+    wav = cfg.wavelets(cfg.shots[0])
+    h = run_hip(kw, wav, isnap=10)
+    o = run_oracle(kw, wav, isnap=10)
+    assert_bitwise(h, o)
+    cfg = configs.sh_layered()
+    kw = cfg.kwargs(cfg.shots[0])
+    wav = cfg.wavelets(cfg.shots[0])
+    a = run_hip(kw, wav, isnap=10, kernel="fused")
+    b = run_hip(kw, wav, isnap=10, kernel="simple")
+    for c in range(3):
+        assert same(a["seis"][c], b["seis"][c]) and same(a["image"][c], b["image"][c])
+    assert np.abs(a["image"][1]).max() > 0
