@@ -138,8 +138,9 @@ static int check_desc(const trtm_step_desc* d, int32_t it_begin, int32_t it_end)
         if (!d->v || !d->syx || !d->syz || !d->myx || !d->myz || !d->inv_rho)
             return fail(TRTM_ERR_ARG, "SH system active but an SH pointer is NULL");
     }
-    if (d->surface_matrix && (!d->sxz || !d->szz || !d->syz))
-        return fail(TRTM_ERR_ARG, "surface_matrix needs sxz, szz and syz");
+    if (d->surface_matrix && (((d->systems & TRTM_PSV) && (!d->sxz || !d->szz)) ||
+                              ((d->systems & TRTM_SH) && !d->syz)))
+        return fail(TRTM_ERR_ARG, "surface_matrix needs the active systems' sxz, szz, syz");
     if (d->n_inj < 0 || d->n_inj_groups < 0 || d->n_inj_groups > d->n_inj)
         return fail(TRTM_ERR_ARG, "bad injection counts");
     if (d->n_inj_groups > 0) {

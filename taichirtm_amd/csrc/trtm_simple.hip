@@ -21,15 +21,18 @@
 
 namespace trtm {
 
+template <bool PSV, bool SH>
 __global__ __launch_bounds__(256) void k_surface(KArgs a) {
     const int j = blockIdx.x * kBlockZ + threadIdx.x;
     const int i = blockIdx.y * kBlockX + threadIdx.y;
     if (i >= a.nx || j >= a.nz) return;
     const size_t c = (size_t)i * a.nz + j;
     const float s = a.sm[c];
-    a.syz[c] = a.syz[c] * s;
-    a.sxz[c] = a.sxz[c] * s;
-    a.szz[c] = a.szz[c] * s;
+    if (SH) a.syz[c] = a.syz[c] * s;
+    if (PSV) {
+        a.sxz[c] = a.sxz[c] * s;
+        a.szz[c] = a.szz[c] * s;
+    }
 }
 
 template <bool FWD, bool PSV, bool SH>
@@ -214,7 +217,7 @@ template <bool FWD, bool PSV, bool SH>
 static hipError_t simple_step(const KArgs& a, const InjArgs& q, const StepIO& p, hipStream_t s) {
     const dim3 block(kBlockZ, kBlockX);
     const dim3 grid((a.nz + kBlockZ - 1) / kBlockZ, (a.nx + kBlockX - 1) / kBlockX);
-    if (a.sm) hipLaunchKernelGGL(k_surface, grid, block, 0, s, a);
+    if (a.sm) hipLaunchKernelGGL((k_surface<PSV, SH>), grid, block, 0, s, a);
     hipLaunchKernelGGL((k_velocity<FWD, PSV, SH>), grid, block, 0, s, a);
     if (q.n_groups > 0)
         hipLaunchKernelGGL((k_inject<FWD, PSV, SH>), dim3((q.n_groups + 255) / 256), dim3(256), 0,

@@ -51,7 +51,7 @@ def test_sh_rtm_bitwise_border_receivers(kernel):
     h = run_hip(kw, wav, isnap=3, kernel=kernel)
     o = run_oracle(kw, wav, isnap=3)
     assert_bitwise(h, o)
-    assert np.abs(h["state_b"]["v"][0, 1]) > 0  # border cell carries injected data
+    assert np.abs(h["image"][1]).max() > 0
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -74,6 +74,8 @@ def test_both_systems_duplicates_and_random_obs(kernel):
     h = run_hip(kw, wav, isnap=2, kernel=kernel, obs=obs)
     o = run_oracle(kw, wav, isnap=2, obs=obs)
     assert_bitwise(h, o)
+    # border cells are never stencil-updated but carry injected data backward
+    assert h["state_b"]["u"][0, 1] != 0 and h["state_b"]["v"][59, 1] != 0
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
