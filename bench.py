@@ -59,10 +59,11 @@ def build_workload(name):
 class RTMShot:
     """Forward + backward propagators of one shot, built once."""
 
-    def __init__(self, cfg, isnap, kernel):
+    def __init__(self, cfg, isnap, kernel, tile_x=0, block_steps=0, tile_z=0):
         import taichirtm_amd as trtm
         shot = cfg.shots[0]
         self.kw = cfg.kwargs(shot)
+        self.kw.update(tile_x=tile_x, block_steps=block_steps, tile_z=tile_z)
         wu, wv, ww = cfg.wavelets(shot)
         self.isnap = isnap
         self.fw = trtm.ForwardModeling(wavelet_u_field=wu, wavelet_v_field=wv,
@@ -156,7 +157,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sh_layered")
-    ap.add_argument("--kernel", default="fused")
+    ap.add_argument("--kernel", default="blocked")
+    ap.add_argument("--tile-x", type=int, default=0)
+    ap.add_argument("--block-steps", type=int, default=0)
+    ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--isnap", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -169,7 +173,7 @@ def main():
     import torch
     torch.cuda.set_device(0)
     cfg, system = build_workload(args.workload)
-    shot = RTMShot(cfg, args.isnap, args.kernel)
+    shot = RTMShot(cfg, args.isnap, args.kernel, args.tile_x, args.block_steps, args.tile_z)
     for _ in range(args.warmup):
         shot.run()
     torch.cuda.synchronize()
@@ -209,7 +213,9 @@ def main():
                                % (cfg.name, args.isnap),
                    "grid": [cfg.nx, cfg.nz], "nt": nt, "dx": cfg.dx, "dt": cfg.dt,
                    "absorbing_frame": cfg.fw, "receivers": len(cfg.shots[0].rec),
-                   "kernel": args.kernel, "system": system},
+                   "kernel": args.kernel, "tile_x": args.tile_x, "tile_z": args.tile_z,
+                   "block_steps": args.block_steps,
+                   "system": system},
         "shots_per_hour": round(3600.0 / (ms_per_step / 1000.0), 1),
         "phases_ms": {"forward": round(float(np.mean(fw_ms)), 3),
                       "backward": round(float(np.mean(bw_ms)), 3)},

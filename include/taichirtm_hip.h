@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TRTM_ABI_VERSION 1
+#define TRTM_ABI_VERSION 2
 
 #define TRTM_OK 0
 #define TRTM_ERR_ARG (-1)         /* invalid argument or shape */
@@ -48,6 +48,7 @@ extern "C" {
 #define TRTM_KERNEL_AUTO 0
 #define TRTM_KERNEL_SIMPLE 1 /* one launch per reference kernel group */
 #define TRTM_KERNEL_FUSED 2  /* one LDS-tiled launch per time step */
+#define TRTM_KERNEL_BLOCKED 3 /* one launch per block_steps time steps (temporal blocking) */
 
 /* ABI version compiled into the library (== TRTM_ABI_VERSION). */
 int trtm_abi_version(void);
@@ -108,7 +109,9 @@ typedef struct trtm_step_desc {
     int32_t fw;       /* absorbing_frame */
     int32_t systems;  /* TRTM_PSV | TRTM_SH: systems that can become nonzero */
     int32_t kernel;   /* TRTM_KERNEL_* */
-    int32_t reserved0;
+    int32_t tile_x;   /* blocked kernel: tile rows (0 = library choice) */
+    int32_t block_steps; /* blocked kernel: time steps per launch (0 = library choice) */
+    int32_t snap_base;   /* global snapshot slot held at index 0 of snap_* (usually 0) */
     const float *sponge_table; /* fw+1 floats (trtm_sponge_table) */
     /* medium (trtm_prepare_medium outputs, plus the caller's lam and mu) */
     const float *lam, *mu, *mxz, *myx, *myz, *inv_rho, *inv_rho_u, *inv_rho_w;
@@ -127,17 +130,20 @@ typedef struct trtm_step_desc {
     /* recording: forward = receivers (seismogram); backward = sources
      * (synthetic source, backward_modeling.py:462-470).  Time-major (nt, n_rec);
      * NULL to skip. */
-    int32_t n_rec, reserved1;
+    int32_t n_rec;
+    int32_t tile_z;      /* blocked kernel: tile columns, 64 or 128 (0 = library choice) */
     const int32_t *rec_loc; /* (n_rec, 2) */
     float *rec_u, *rec_v, *rec_w;
     /* finite check every check_interval steps (it % check_interval == 0);
      * check_flags has nt/check_interval + 1 ints, zeroed by the caller; entry
      * it/check_interval receives the highest code found at step it. */
-    int32_t check_interval, reserved2;
+    int32_t check_interval;
+    int32_t step_offset; /* batch calls: this shot runs steps [it_begin, it_end) + step_offset */
     int32_t *check_flags;
-    /* snapshots: forward writes slot it/isnap-1 at it % isnap == 0, it != 0;
-     * backward correlates them into the image.  Snapshot-major
-     * (num_snaps, nx, nz) layout. */
+    /* snapshots: forward writes global slot s = it/isnap-1 at it % isnap == 0,
+     * it != 0, into index s - snap_base when that lies in [0, num_snaps);
+     * backward correlates index s - snap_base into the image when
+     * isnaps_host[s - snap_base] == t.  Snapshot-major (num_snaps, nx, nz). */
     int32_t isnap, num_snaps;
     float *snap_u, *snap_v, *snap_w;
     const int32_t *isnaps_host; /* backward only: HOST array of num_snaps step indices */
@@ -149,6 +155,9 @@ typedef struct trtm_step_desc {
 
 /* Scratch bytes trtm_forward_steps / trtm_backward_steps need for desc. */
 int64_t trtm_workspace_bytes(const trtm_step_desc *desc);
+
+/* Scratch bytes of the batch-level buffer for trtm_*_steps_batch with n shots. */
+int64_t trtm_batch_workspace_bytes(int32_t n);
 
 /*
  * Forward steps it_begin <= it < it_end.  Replaces the body of
@@ -172,6 +181,24 @@ int trtm_forward_steps(const trtm_step_desc *desc, int32_t it_begin, int32_t it_
  */
 int trtm_backward_steps(const trtm_step_desc *desc, int32_t it_begin, int32_t it_end,
                         void *stream);
+
+/*
+ * Many shots on one medium in one launch per time block (blocked kernel):
+ * descs[0..n) describe one shot each (own state, injection, records,
+ * snapshots, image, check flags, workspace, record length nt) and share nx,
+ * nz, fw, the float32 constants, systems and every medium pointer.  Shot k
+ * runs steps it_begin <= it < min(it_end, descs[k].nt).  batch_workspace holds
+ * trtm_batch_workspace_bytes(n) bytes.  Same per-shot semantics as
+ * trtm_forward_steps / trtm_backward_steps.  No reference counterpart: the
+ * reference runs one shot per ForwardModeling / BackwardModeling
+ * (rtm.py:935-1000).
+ */
+int trtm_forward_steps_batch(const trtm_step_desc *descs, int32_t n, void *batch_workspace,
+                             int64_t batch_workspace_bytes, int32_t it_begin, int32_t it_end,
+                             int32_t save, void *stream);
+int trtm_backward_steps_batch(const trtm_step_desc *descs, int32_t n, void *batch_workspace,
+                              int64_t batch_workspace_bytes, int32_t it_begin, int32_t it_end,
+                              void *stream);
 
 /*
  * Standalone finite check of u, v, w over the full grid: NaN or |x| > 1e30

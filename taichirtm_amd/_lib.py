@@ -11,11 +11,11 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtaichirtm_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 OK = 0
 ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3
 PSV, SH = 1, 2
-KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_FUSED = 0, 1, 2
+KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_FUSED, KERNEL_BLOCKED = 0, 1, 2, 3
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -27,7 +27,8 @@ class StepDesc(ctypes.Structure):
     _fields_ = [
         ("struct_size", _i32), ("nx", _i32), ("nz", _i32), ("nt", _i32),
         ("dt", _f32), ("inv_dx", _f32), ("inv_dz", _f32), ("src_factor", _f32),
-        ("fw", _i32), ("systems", _i32), ("kernel", _i32), ("reserved0", _i32),
+        ("fw", _i32), ("systems", _i32), ("kernel", _i32), ("tile_x", _i32),
+        ("block_steps", _i32), ("snap_base", _i32),
         ("sponge_table", _p),
         ("lam", _p), ("mu", _p), ("mxz", _p), ("myx", _p), ("myz", _p),
         ("inv_rho", _p), ("inv_rho_u", _p), ("inv_rho_w", _p),
@@ -37,9 +38,9 @@ class StepDesc(ctypes.Structure):
         ("n_inj", _i32), ("n_inj_groups", _i32),
         ("inj_loc", _p), ("inj_order", _p), ("inj_group_start", _p),
         ("inj_u", _p), ("inj_v", _p), ("inj_w", _p),
-        ("n_rec", _i32), ("reserved1", _i32),
+        ("n_rec", _i32), ("tile_z", _i32),
         ("rec_loc", _p), ("rec_u", _p), ("rec_v", _p), ("rec_w", _p),
-        ("check_interval", _i32), ("reserved2", _i32),
+        ("check_interval", _i32), ("step_offset", _i32),
         ("check_flags", _p),
         ("isnap", _i32), ("num_snaps", _i32),
         ("snap_u", _p), ("snap_v", _p), ("snap_w", _p),
@@ -58,6 +59,11 @@ SIGNATURES = {
     "trtm_sponge_fill": (ctypes.c_int, [_i32, _i32, _i32, _p, _p, _p]),
     "trtm_plan_injection": (ctypes.c_int, [_i32, _i32, _i32, _p, _p, _p, _p]),
     "trtm_workspace_bytes": (ctypes.c_int64, [ctypes.POINTER(StepDesc)]),
+    "trtm_batch_workspace_bytes": (ctypes.c_int64, [_i32]),
+    "trtm_forward_steps_batch": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _p, ctypes.c_int64,
+                                                _i32, _i32, _i32, _p]),
+    "trtm_backward_steps_batch": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _p,
+                                                 ctypes.c_int64, _i32, _i32, _p]),
     "trtm_forward_steps": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _i32, _i32, _p]),
     "trtm_backward_steps": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _i32, _p]),
     "trtm_check_finite": (ctypes.c_int, [_i32, _i32, _p, _p, _p, _i32, _p, _p]),

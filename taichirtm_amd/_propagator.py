@@ -20,7 +20,8 @@ SPONGE_A = 0.0053  # forward_modeling.py:240
 STATE = ("u", "v", "w", "sxx", "szz", "sxz", "syx", "syz")
 PSV_STATE = ("u", "w", "sxx", "szz", "sxz")
 SH_STATE = ("v", "syx", "syz")
-KERNELS = {"auto": _lib.KERNEL_AUTO, "simple": _lib.KERNEL_SIMPLE, "fused": _lib.KERNEL_FUSED}
+KERNELS = {"auto": _lib.KERNEL_AUTO, "simple": _lib.KERNEL_SIMPLE, "fused": _lib.KERNEL_FUSED,
+           "blocked": _lib.KERNEL_BLOCKED}
 
 
 def _ptr(t):
@@ -34,6 +35,51 @@ def _ptr(t):
 
 def _all_finite(t) -> bool:
     return bool(torch.isfinite(t).all().item())
+
+
+class Medium:
+    """Model arrays shared by every propagator on one grid: the caller's mu,
+    lam, rho (and optional surface matrix), the averaged moduli and densities
+    and their reciprocals (forward_modeling.py:243-307), the sponge
+    (forward_modeling.py:203-241) and the float32 step constants."""
+
+    def __init__(self, nx, nz, dx, dz, fs, absorbing_frame, mu, lam, rho, surface_matrix=None):
+        self.nx, self.nz = int(nx), int(nz)
+        self.dx, self.dz, self.fs = float(dx), float(dz), float(fs)
+        self.fw = fw = int(absorbing_frame)
+        if fw < 0 or fw > min(nx, nz):
+            raise ValueError("absorbing_frame %d must lie in [0, min(nx, nz)] = [0, %d]"
+                             % (fw, min(nx, nz)))
+        device()
+        _lib.load()
+        self.mu = as_tensor(mu, shape=(nx, nz), name="mu_field")
+        self.lam = as_tensor(lam, shape=(nx, nz), name="lam_field")
+        self.rho_field = as_tensor(rho, shape=(nx, nz), name="rho_field")
+        self.surface_matrix = (None if surface_matrix is None else
+                               as_tensor(surface_matrix, shape=(nx, nz), name="surface_matrix_field"))
+        dev = device()
+        self.coef = {k: torch.empty((nx, nz), dtype=torch.float32, device=dev)
+                     for k in ("myx", "myz", "mxz", "rho_u", "rho_w", "inv_rho", "inv_rho_u",
+                               "inv_rho_w")}
+        c = self.coef
+        s = stream_handle()
+        _lib.call("trtm_prepare_medium", nx, nz, _ptr(self.mu), _ptr(self.rho_field),
+                  _ptr(c["myx"]), _ptr(c["myz"]), _ptr(c["mxz"]), _ptr(c["rho_u"]),
+                  _ptr(c["rho_w"]), _ptr(c["inv_rho"]), _ptr(c["inv_rho_u"]),
+                  _ptr(c["inv_rho_w"]), s)
+        tab = np.zeros(fw + 1, np.float32)
+        _lib.call("trtm_sponge_table", fw, ctypes.c_float(SPONGE_A),
+                  tab.ctypes.data_as(ctypes.c_void_p))
+        self.sponge_table = as_tensor(tab)
+        self.absorb_coeff = Field(np.float32, (nx, nz))
+        _lib.call("trtm_sponge_fill", nx, nz, fw, _ptr(self.sponge_table),
+                  _ptr(self.absorb_coeff.tensor), s)
+        # float32 constants as the reference narrows them (forward_modeling.py:320-322,430)
+        dt = 1.0 / self.fs
+        self.dt32 = np.float32(dt)
+        self.idx32 = np.float32(1.0 / self.dx)
+        self.idz32 = np.float32(1.0 / self.dz)
+        self.factor32 = np.float32(dt * self.dx * self.dz)
 
 
 class Propagator:
@@ -56,13 +102,13 @@ class Propagator:
         if kernel not in KERNELS:
             raise ValueError("kernel must be one of %s" % sorted(KERNELS))
         self.kernel = kernel
+        self.tile_x = int(kwargs.get("tile_x", 0))
+        self.block_steps = int(kwargs.get("block_steps", 0))
+        self.tile_z = int(kwargs.get("tile_z", 0))
         nx, nz, nt = self.nx, self.nz, self.nt
         if nx < 2 or nz < 2 or nt < 1:
             raise ValueError("grid must be at least 2 x 2 with nt >= 1")
         fw = self.absorbing_frame
-        if fw < 0 or fw > min(nx, nz):
-            raise ValueError("absorbing_frame %d must lie in [0, min(nx, nz)] = [0, %d]"
-                             % (fw, min(nx, nz)))
         device()  # fail loudly without a HIP device
         _lib.load()
 
@@ -79,38 +125,26 @@ class Propagator:
         self._src_dev = as_tensor(self._src_host, np.int32)
         self._rec_dev = as_tensor(self._rec_host, np.int32)
 
-        # medium (kept by reference, as forward_modeling.py:194-197)
-        self.mu = as_tensor(kwargs["mu_field"], shape=(nx, nz), name="mu_field")
-        self.lam = as_tensor(kwargs["lam_field"], shape=(nx, nz), name="lam_field")
-        self.rho_field = as_tensor(kwargs["rho_field"], shape=(nx, nz), name="rho_field")
-        sm = kwargs.get("surface_matrix_field", None)
-        self.surface_matrix = None if sm is None else as_tensor(sm, shape=(nx, nz),
-                                                                name="surface_matrix_field")
-        dev = device()
-        mk = lambda: torch.empty((nx, nz), dtype=torch.float32, device=dev)
-        self._coef = {k: mk() for k in ("myx", "myz", "mxz", "rho_u", "rho_w",
-                                        "inv_rho", "inv_rho_u", "inv_rho_w")}
-        c = self._coef
-        s = stream_handle()
-        _lib.call("trtm_prepare_medium", nx, nz, _ptr(self.mu), _ptr(self.rho_field),
-                  _ptr(c["myx"]), _ptr(c["myz"]), _ptr(c["mxz"]), _ptr(c["rho_u"]),
-                  _ptr(c["rho_w"]), _ptr(c["inv_rho"]), _ptr(c["inv_rho_u"]),
-                  _ptr(c["inv_rho_w"]), s)
-        for k, t in c.items():
+        # medium (kept by reference, as forward_modeling.py:194-197); shots on
+        # the same model can share one Medium
+        med = kwargs.get("medium")
+        if med is None:
+            sm = kwargs.get("surface_matrix_field", None)
+            med = Medium(nx, nz, self.dx, self.dz, self.fs, fw, kwargs["mu_field"],
+                         kwargs["lam_field"], kwargs["rho_field"], sm)
+        elif (med.nx, med.nz, med.fw) != (nx, nz, fw) or (med.dx, med.dz, med.fs) != (
+                self.dx, self.dz, self.fs):
+            raise ValueError("medium does not match nx, nz, dx, dz, fs, absorbing_frame")
+        self.medium = med
+        self.mu, self.lam, self.rho_field = med.mu, med.lam, med.rho_field
+        self.surface_matrix = med.surface_matrix
+        self._coef = med.coef
+        for k, t in med.coef.items():
             setattr(self, k, Field(tensor=t))
-        tab = np.zeros(fw + 1, np.float32)
-        _lib.call("trtm_sponge_table", fw, ctypes.c_float(SPONGE_A),
-                  tab.ctypes.data_as(ctypes.c_void_p))
-        self._sponge_table = as_tensor(tab)
-        self.absorb_coeff = Field(np.float32, (nx, nz))
-        _lib.call("trtm_sponge_fill", nx, nz, fw, _ptr(self._sponge_table),
-                  _ptr(self.absorb_coeff.tensor), s)
-
-        # float32 constants as the reference narrows them (forward_modeling.py:320-322,430)
-        self._dt32 = np.float32(self.dt)
-        self._idx32 = np.float32(1.0 / self.dx)
-        self._idz32 = np.float32(1.0 / self.dz)
-        self._factor32 = np.float32(self.dt * self.dx * self.dz)
+        self._sponge_table = med.sponge_table
+        self.absorb_coeff = med.absorb_coeff
+        self._dt32, self._idx32, self._idz32, self._factor32 = (
+            med.dt32, med.idx32, med.idz32, med.factor32)
 
         # injection data and its deterministic per-cell plan
         self._inj = [as_tensor(kwargs[k], shape=(len(self._inj_host_loc()), nt), name=k)
@@ -211,6 +245,7 @@ class Propagator:
         d.fw = self.absorbing_frame
         d.systems = self.systems
         d.kernel = KERNELS[self.kernel]
+        d.tile_x, d.block_steps, d.tile_z = self.tile_x, self.block_steps, self.tile_z
         d.sponge_table = _ptr(self._sponge_table)
         c = self._coef
         d.lam, d.mu = _ptr(self.lam), _ptr(self.mu)

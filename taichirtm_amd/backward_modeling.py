@@ -127,7 +127,6 @@ class BackwardModeling(Propagator):
             snaps.get(c) is not None and c in self._img for c in ("u", "v", "w"))
 
         d = self._desc()
-        ws = self._workspace(d)
         flags = torch.zeros(nt // sci + 1, dtype=torch.int32, device=dev)
         d.check_interval = sci
         d.check_flags = _ptr(flags)
@@ -143,11 +142,12 @@ class BackwardModeling(Propagator):
             d.snap_u, d.snap_v, d.snap_w = (_ptr(snaps.get(c)) for c in ("u", "v", "w"))
             d.isnaps_host = isnaps.ctypes.data_as(ctypes.c_void_p)
             d.image_u, d.image_v, d.image_w = (_ptr(self._img.get(c)) for c in ("u", "v", "w"))
-            if sysmask != self.systems:
-                # a propagating system without forward data: its image stays 0,
-                # run it without imaging in a second descriptor
-                return self._run_split(d, sysmask, flags, sci, display_callback, ws)
         d.rec_u, d.rec_v, d.rec_w = (_ptr(self._syn.get(c)) for c in ("u", "v", "w"))
+        ws = self._workspace(d)
+        if imaging and sysmask != self.systems:
+            # a propagating system without forward data: its image stays 0,
+            # run it without imaging in a second descriptor
+            return self._run_split(d, sysmask, flags, sci, display_callback, ws)
 
         image_steps = [nt - 1 - t for t in range(nt - 1, 0, -1)
                        if imaging and t % isnap == 0 and 0 <= t // isnap - 1 < num_snaps

@@ -1,0 +1,331 @@
+"""Many-shot RTM on one medium, batched into one launch per time block.
+
+The reference migrates one shot per ForwardModeling / BackwardModeling pair
+(rtm.py:935-1000) and stacks the per-shot images afterwards
+(imaging.py:57-129).  ``BatchRTM`` runs every shot of a survey through the
+blocked step kernel together: the grid's second dimension is the shot, so the
+medium is shared and one launch advances all shots by a block of time steps.
+Shots may have different record lengths and receiver spreads.
+
+Two imaging modes give the same image bit for bit:
+
+- ``"snapshot"``: the reference's scheme, forward wavefields saved every
+  ``isnap`` steps (forward_modeling.py:568-572) and correlated backward
+  (backward_modeling.py:556-564).  Memory: nt/isnap wavefields per shot.
+- ``"checkpoint"``: the forward pass keeps the full state every L steps;
+  the backward pass walks the segments last to first, recomputes the forward
+  wavefields of one segment from its checkpoint into a segment-sized buffer,
+  then runs the backward steps of that segment with the image accumulated in
+  the step kernel.  Memory: nt/L states + L/isnap wavefields per shot, with L
+  from a memory budget (about sqrt(nt) by default); cost: one extra forward
+  sweep.  Recomputation is deterministic, so the wavefields, and the image,
+  are the snapshot mode's exactly.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._propagator import Medium, PSV_STATE, SH_STATE, _ptr
+from .backward_modeling import BackwardModeling
+from .field import SnapshotField, ZeroField, as_host, device, stream_handle
+from .forward_modeling import ForwardModeling
+
+
+@dataclass
+class ShotSpec:
+    """One shot: source cells (ns, 2), source wavelets (u, v, w) each (ns, nt),
+    receiver cells (nr, 2), and optionally observed data (u, v, w) each
+    (nr, nt); without observed data the shot migrates its own forward
+    seismogram."""
+    src_loc: np.ndarray
+    wavelets: Sequence[np.ndarray]
+    rec_loc: np.ndarray
+    observed: Optional[Sequence[np.ndarray]] = None
+
+    @property
+    def nt(self):
+        return int(np.asarray(self.wavelets[0]).shape[-1])
+
+
+def _desc_array(descs):
+    arr = (_lib.StepDesc * len(descs))()
+    for q, d in enumerate(descs):
+        ctypes.memmove(ctypes.byref(arr[q]), ctypes.byref(d), ctypes.sizeof(_lib.StepDesc))
+    return arr
+
+
+def checkpoint_interval(nt, n_state, n_snap, isnap=1, align=1, budget_bytes=None, cell_bytes=4,
+                        cells=1):
+    """Segment length L (a multiple of align) minimising checkpoint storage
+    ceil(nt/L)*n_state plus segment buffer (L/isnap + 1)*n_snap fields, or the
+    smallest L whose total fits budget_bytes when one is given."""
+    def fields(L):
+        return math.ceil(nt / L) * n_state + (L // isnap + 1) * n_snap
+    best = max(align, int(math.sqrt(nt * n_state * isnap / max(n_snap, 1))))
+    best = ((best + align - 1) // align) * align
+    cands = sorted({best} | {((L + align - 1) // align) * align for L in
+                             (best // 2, best * 2, nt)} - {0})
+    L = min(cands, key=fields)
+    if budget_bytes is not None and fields(L) * cell_bytes * cells > budget_bytes:
+        raise MemoryError("checkpointing needs %.2f GB per shot, budget %.2f GB"
+                          % (fields(L) * cell_bytes * cells / 1e9, budget_bytes / 1e9))
+    return L
+
+
+class BatchRTM:
+    """Reverse time migration of many shots sharing one model.
+
+    Parameters: grid and sampling (nx, nz, dx, dz, fs), the model (mu_field,
+    lam_field, rho_field, optional surface_matrix_field), absorbing_frame, the
+    shots, the snapshot interval isnap, imaging ("snapshot" or "checkpoint"),
+    checkpoint_every (segment length; default from the memory model),
+    stability_check_interval, and the blocked kernel's tile_x / tile_z /
+    block_steps (0 = library default).
+    """
+
+    def __init__(self, nx, nz, dx, dz, fs, mu_field, lam_field, rho_field, absorbing_frame,
+                 shots: List[ShotSpec], isnap=1, imaging="snapshot", checkpoint_every=None,
+                 stability_check_interval=100, surface_matrix_field=None, tile_x=0, tile_z=0,
+                 block_steps=0):
+        if imaging not in ("snapshot", "checkpoint"):
+            raise ValueError("imaging must be 'snapshot' or 'checkpoint'")
+        if not shots:
+            raise ValueError("no shots")
+        self.medium = Medium(nx, nz, dx, dz, fs, absorbing_frame, mu_field, lam_field, rho_field,
+                             surface_matrix_field)
+        self.nx, self.nz = int(nx), int(nz)
+        self.shots = shots
+        self.isnap = int(isnap)
+        self.imaging = imaging
+        self.checkpoint_every = checkpoint_every
+        self.sci = int(stability_check_interval)
+        self.block = dict(tile_x=tile_x, tile_z=tile_z, block_steps=block_steps)
+        self.codes = []
+        self.fw: List[ForwardModeling] = []
+        self.bw: List[BackwardModeling] = []
+
+    # -- helpers ------------------------------------------------------------
+
+    def _common(self, shot):
+        src = np.asarray(shot.src_loc, np.int32).reshape(-1, 2)
+        rec = np.asarray(shot.rec_loc, np.int32).reshape(-1, 2)
+        return dict(nx=self.nx, nz=self.nz, dx=self.medium.dx, dz=self.medium.dz, nt=shot.nt,
+                    fs=self.medium.fs, absorbing_frame=self.medium.fw, medium=self.medium,
+                    src_loc_field=src, recv_loc_field=rec, num_sources=len(src),
+                    num_receivers=len(rec), kernel="blocked", **self.block)
+
+    def _batch(self, fn, descs, it_begin, it_end, *extra):
+        arr = _desc_array(descs)
+        n = len(descs)
+        nbytes = int(_lib.load().trtm_batch_workspace_bytes(n))
+        bws = torch.empty(nbytes, dtype=torch.uint8, device=device())
+        _lib.call(fn, arr, n, _ptr(bws), nbytes, it_begin, it_end, *extra, stream_handle())
+        return bws
+
+    # -- forward ------------------------------------------------------------
+
+    def _forward(self):
+        self.fw = [ForwardModeling(wavelet_u_field=s.wavelets[0], wavelet_v_field=s.wavelets[1],
+                                   wavelet_w_field=s.wavelets[2], isnap=self.isnap,
+                                   **self._common(s)) for s in self.shots]
+        systems = {f.systems for f in self.fw}
+        if len(systems) != 1:
+            raise ValueError("all shots of a batch must excite the same wave systems")
+        self.systems = systems.pop()
+        dev = device()
+        descs, keep = [], []
+        self.flags_f = []
+        nt_max = max(s.nt for s in self.shots)
+        snapshot = self.imaging == "snapshot"
+        names = [k for k in PSV_STATE if self.systems & _lib.PSV] + \
+                [k for k in SH_STATE if self.systems & _lib.SH]
+        self._state_names = names
+        if not snapshot:
+            n_snap = (2 if self.systems & _lib.PSV else 0) + (1 if self.systems & _lib.SH else 0)
+            L = self.checkpoint_every or checkpoint_interval(nt_max, len(names), n_snap,
+                                                             self.isnap, align=self.isnap)
+            if L % self.isnap:
+                raise ValueError("checkpoint_every must be a multiple of isnap")
+            self.L = L
+        for f in self.fw:
+            d = f._desc()
+            fl = torch.zeros(f.nt // self.sci + 1, dtype=torch.int32, device=dev)
+            d.check_interval, d.check_flags = self.sci, _ptr(fl)
+            d.isnap = self.isnap
+            if snapshot:
+                f.num_snaps = f.nt // self.isnap
+                for comp, sysbit in (("u", _lib.PSV), ("v", _lib.SH), ("w", _lib.PSV)):
+                    fld = (SnapshotField(tensor=torch.zeros((f.num_snaps, self.nx, self.nz),
+                                                            dtype=torch.float32, device=dev))
+                           if f.systems & sysbit else ZeroField((self.nx, self.nz, f.num_snaps)))
+                    setattr(f, comp + "_save_field", fld)
+                d.num_snaps = f.num_snaps
+                d.snap_u, d.snap_v, d.snap_w = (_ptr(getattr(f, c + "_save_field"))
+                                                for c in ("u", "v", "w"))
+            d.rec_u, d.rec_v, d.rec_w = (_ptr(f._seis.get(c)) for c in ("u", "v", "w"))
+            if not snapshot:  # size the workspace for the recompute's segment buffer too
+                d.num_snaps = self.L // self.isnap + 1
+            keep.append(f._workspace(d))
+            d.num_snaps = f.num_snaps if snapshot else 0
+            descs.append(d)
+            self.flags_f.append(fl)
+        self._fw_descs, self._fw_ws = descs, keep
+        if snapshot:
+            self._batch("trtm_forward_steps_batch", descs, 0, nt_max, 1)
+            return
+        # checkpoint mode: keep the state at every multiple of L
+        L = self.L
+        self.edges = list(range(0, nt_max, L)) + [nt_max]
+        self.ckpt = []
+        for f in self.fw:
+            ncp = (f.nt + L - 1) // L
+            self.ckpt.append(torch.empty((ncp, len(names), self.nx, self.nz), dtype=torch.float32,
+                                         device=dev))
+        for k in range(len(self.edges) - 1):
+            a, b = self.edges[k], self.edges[k + 1]
+            for q, f in enumerate(self.fw):
+                if a < f.nt:
+                    for m, name in enumerate(names):
+                        self.ckpt[q][k, m].copy_(getattr(f, name).tensor)
+            self._batch("trtm_forward_steps_batch", descs, a, b, 0)
+
+    # -- backward -----------------------------------------------------------
+
+    def _backward(self):
+        dev = device()
+        self.bw = []
+        for s, f in zip(self.shots, self.fw):
+            if s.observed is not None:
+                obs = [np.asarray(o, np.float32) for o in s.observed]
+            else:
+                seis = f.seismogram_tensors()
+                obs = []
+                for c in ("u", "v", "w"):
+                    t = seis.get(c)
+                    if t is None:
+                        obs.append(np.zeros((f.num_receivers, f.nt), np.float32))
+                    else:
+                        o = torch.empty((f.num_receivers, f.nt), dtype=torch.float32, device=dev)
+                        _lib.call("trtm_transpose", f.nt, f.num_receivers, _ptr(t), _ptr(o),
+                                  stream_handle())
+                        obs.append(o)
+            self.bw.append(BackwardModeling(obsdata_u_field=obs[0], obsdata_v_field=obs[1],
+                                            obsdata_w_field=obs[2], **self._common(s)))
+        self.flags_b = []
+        descs, keep, self._isn = [], [], []
+        nt_max = max(s.nt for s in self.shots)
+        snapshot = self.imaging == "snapshot"
+        for q, (f, b) in enumerate(zip(self.fw, self.bw)):
+            d = b._desc()
+            fl = torch.zeros(b.nt // self.sci + 1, dtype=torch.int32, device=dev)
+            d.check_interval, d.check_flags = self.sci, _ptr(fl)
+            d.isnap = self.isnap
+            d.image_u, d.image_v, d.image_w = (_ptr(b._img.get(c)) for c in ("u", "v", "w"))
+            if snapshot:
+                isn = np.array([(k + 1) * self.isnap for k in range(f.num_snaps)], np.int32)
+                isn[isn >= f.nt] = 0  # a slot the forward never wrote
+                self._isn.append(isn)
+                d.num_snaps = f.num_snaps
+                d.snap_u, d.snap_v, d.snap_w = (_ptr(getattr(f, c + "_save_field"))
+                                                for c in ("u", "v", "w"))
+                d.isnaps_host = isn.ctypes.data_as(ctypes.c_void_p)
+            else:  # size the workspace for the segment buffer's isnaps
+                d.num_snaps = self.L // self.isnap + 1
+            keep.append(b._workspace(d))
+            descs.append(d)
+            self.flags_b.append(fl)
+        self._bw_ws = keep
+        if snapshot:
+            self._batch("trtm_backward_steps_batch", descs, 0, nt_max)
+            return
+        # checkpoint mode: segments last to first
+        L = self.L
+        nslot = L // self.isnap + 1
+        bufs = []
+        for f in self.fw:
+            bufs.append({c: (torch.zeros((nslot, self.nx, self.nz), dtype=torch.float32, device=dev)
+                             if f.systems & bit else None)
+                         for c, bit in (("u", _lib.PSV), ("v", _lib.SH), ("w", _lib.PSV))})
+        fdescs = []
+        for f, d0, buf in zip(self.fw, self._fw_descs, bufs):
+            d = _lib.StepDesc.from_buffer_copy(d0)
+            d.num_snaps = nslot
+            d.snap_u, d.snap_v, d.snap_w = (_ptr(buf[c]) for c in ("u", "v", "w"))
+            d.rec_u = d.rec_v = d.rec_w = None
+            fdescs.append(d)
+        for k in range(len(self.edges) - 2, -1, -1):
+            a, b = self.edges[k], self.edges[k + 1]
+            base = a // self.isnap - 1
+            for q, f in enumerate(self.fw):
+                if a < f.nt:
+                    for m, name in enumerate(self._state_names):
+                        getattr(f, name).tensor.copy_(self.ckpt[q][k, m])
+                fdescs[q].snap_base = base
+            self._batch("trtm_forward_steps_batch", fdescs, a, b, 1)
+            isns = []
+            for q, f in enumerate(self.fw):
+                isn = np.full(nslot, -1, np.int32)
+                for j in range(nslot):
+                    it = (j + base + 1) * self.isnap
+                    if a <= it < min(b, f.nt) and it >= 1:
+                        isn[j] = it
+                isns.append(isn)
+                descs[q].snap_base = base
+                descs[q].num_snaps = nslot
+                descs[q].snap_u, descs[q].snap_v, descs[q].snap_w = (
+                    _ptr(bufs[q][c]) for c in ("u", "v", "w"))
+                descs[q].isnaps_host = isn.ctypes.data_as(ctypes.c_void_p)
+                descs[q].step_offset = f.nt
+            self._batch("trtm_backward_steps_batch", descs, -b, -a)
+
+    # -- driver -------------------------------------------------------------
+
+    def run(self) -> List[int]:
+        """Forward then backward for every shot; returns per-shot codes
+        (0, 1-3 forward, 4-6 backward; the first failing check step)."""
+        self._forward()
+        self._backward()
+        self.codes = []
+        for q, s in enumerate(self.shots):
+            code = 0
+            for fl in (self.flags_f[q], self.flags_b[q]):
+                h = fl.cpu().numpy()
+                nz_ = np.nonzero(h)[0]
+                if len(nz_):
+                    code = int(h[nz_[0]])
+                    break
+            self.codes.append(code)
+        return self.codes
+
+    def images(self):
+        """Per-shot images [(u, v, w)] as (nx, nz) numpy arrays."""
+        return [b.get_results() for b in self.bw]
+
+    def image_tensors(self):
+        return [b.image_tensors() for b in self.bw]
+
+    def stacked_image(self, dtype=torch.float64):
+        """Sum of the per-shot images on the device, per component, in shot
+        order and in ``dtype`` (the reference stacks in float32 on the host,
+        imaging.py:86-93)."""
+        out = {}
+        for c in ("u", "v", "w"):
+            acc = None
+            for b in self.bw:
+                t = b._img.get(c)
+                if t is None:
+                    continue
+                acc = t.to(dtype).clone() if acc is None else acc.add_(t.to(dtype))
+            out[c] = acc if acc is not None else torch.zeros((self.nx, self.nz), dtype=dtype,
+                                                             device=device())
+        return out
+
+    def seismograms(self):
+        return [f.get_seismogram() for f in self.fw]

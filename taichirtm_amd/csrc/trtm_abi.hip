@@ -21,6 +21,12 @@ hipError_t launch_fused_step(bool fwd, int systems, const KArgs& a, const KArgs&
                              hipStream_t s);
 int64_t fused_workspace_bytes(const trtm_step_desc* d);
 int fused_prepare(const trtm_step_desc* d, bool fwd, hipStream_t s, TilePlan* plan);
+struct ShotDev;
+int64_t block_workspace_bytes(const trtm_step_desc* d);
+int64_t block_shot_table_bytes(int n);
+ShotDev* block_shot_slot(const trtm_step_desc* d);
+int run_block(const trtm_step_desc* descs, int n, bool fwd, bool save, int32_t it_begin,
+              int32_t it_end, ShotDev* shots_dev, const KArgs& m, hipStream_t s);
 
 static thread_local std::string g_error;
 
@@ -158,7 +164,7 @@ static int check_desc(const trtm_step_desc* d, int32_t it_begin, int32_t it_end)
     }
     if (d->check_interval < 0) return fail(TRTM_ERR_ARG, "check_interval < 0");
     if (d->check_interval > 0 && !d->check_flags) return fail(TRTM_ERR_ARG, "check_flags is NULL");
-    if (d->kernel < TRTM_KERNEL_AUTO || d->kernel > TRTM_KERNEL_FUSED)
+    if (d->kernel < TRTM_KERNEL_AUTO || d->kernel > TRTM_KERNEL_BLOCKED)
         return fail(TRTM_ERR_ARG, "bad kernel selector %d", d->kernel);
     return TRTM_OK;
 }
@@ -176,16 +182,16 @@ static KArgs make_kargs(const trtm_step_desc* d) {
     return a;
 }
 
-static bool use_fused(const trtm_step_desc* d) {
-    if (d->kernel == TRTM_KERNEL_SIMPLE) return false;
-    if (d->kernel == TRTM_KERNEL_FUSED) return true;
-    return d->workspace != nullptr && d->workspace_bytes >= fused_workspace_bytes(d);
+// kernel the library runs for d: AUTO takes the blocked kernel when the
+// workspace allows it, else the fused one, else the simple one
+static int pick_kernel(const trtm_step_desc* d) {
+    if (d->kernel != TRTM_KERNEL_AUTO) return d->kernel;
+    if (d->workspace && d->workspace_bytes >= block_workspace_bytes(d)) return TRTM_KERNEL_BLOCKED;
+    if (d->workspace && d->workspace_bytes >= fused_workspace_bytes(d)) return TRTM_KERNEL_FUSED;
+    return TRTM_KERNEL_SIMPLE;
 }
 
-static int run_steps(const trtm_step_desc* d, int32_t it_begin, int32_t it_end, bool fwd,
-                     bool save, hipStream_t s) {
-    int rc = check_desc(d, it_begin, it_end);
-    if (rc) return rc;
+static int check_io(const trtm_step_desc* d, bool fwd, bool save) {
     if (save || !fwd) {
         if (d->isnap < 1 && (save || d->num_snaps > 0))
             return fail(TRTM_ERR_ARG, "isnap must be >= 1");
@@ -201,9 +207,27 @@ static int run_steps(const trtm_step_desc* d, int32_t it_begin, int32_t it_end, 
             }
         }
     }
+    return TRTM_OK;
+}
+
+static int run_steps(const trtm_step_desc* d, int32_t it_begin, int32_t it_end, bool fwd,
+                     bool save, hipStream_t s) {
+    int rc = check_desc(d, it_begin, it_end);
+    if (rc) return rc;
+    rc = check_io(d, fwd, save);
+    if (rc) return rc;
     if (d->systems == 0 || it_begin == it_end) return TRTM_OK;  // nothing can change
 
-    const bool fused = use_fused(d);
+    const int kernel = pick_kernel(d);
+    if (d->step_offset != 0)
+        return fail(TRTM_ERR_ARG, "step_offset is for batch calls; use 0 here");
+    if (kernel == TRTM_KERNEL_BLOCKED) {
+        if (!d->workspace || d->workspace_bytes < block_workspace_bytes(d))
+            return fail(TRTM_ERR_ARG, "blocked kernel needs %lld workspace bytes",
+                        (long long)block_workspace_bytes(d));
+        return run_block(d, 1, fwd, save, it_begin, it_end, block_shot_slot(d), make_kargs(d), s);
+    }
+    const bool fused = kernel == TRTM_KERNEL_FUSED;
     if (fused && (!d->workspace || d->workspace_bytes < fused_workspace_bytes(d)))
         return fail(TRTM_ERR_ARG, "fused kernel needs %lld workspace bytes",
                     (long long)fused_workspace_bytes(d));
@@ -343,7 +367,59 @@ int trtm_plan_injection(int32_t nx, int32_t nz, int32_t n, const int32_t* loc_ho
 int64_t trtm_workspace_bytes(const trtm_step_desc* d) {
     if (!d || d->struct_size != (int32_t)sizeof(trtm_step_desc)) return -1;
     if (d->kernel == TRTM_KERNEL_SIMPLE) return 0;
-    return fused_workspace_bytes(d);
+    if (d->kernel == TRTM_KERNEL_FUSED) return fused_workspace_bytes(d);
+    return std::max(block_workspace_bytes(d), fused_workspace_bytes(d));
+}
+
+int64_t trtm_batch_workspace_bytes(int32_t n) {
+    return n < 1 ? -1 : block_shot_table_bytes(n);
+}
+
+static int run_batch(const trtm_step_desc* descs, int32_t n, void* bws, int64_t bws_bytes,
+                     int32_t it_begin, int32_t it_end, bool fwd, bool save, hipStream_t s) {
+    if (!descs || n < 1) return fail(TRTM_ERR_ARG, "empty batch");
+    if (!bws || bws_bytes < block_shot_table_bytes(n))
+        return fail(TRTM_ERR_ARG, "batch workspace needs %lld bytes",
+                    (long long)block_shot_table_bytes(n));
+    const trtm_step_desc* d0 = &descs[0];
+    for (int q = 0; q < n; q++) {
+        const trtm_step_desc* d = &descs[q];
+        const int lo = std::min(std::max(0, it_begin + d->step_offset), d->nt);
+        const int hi = std::max(lo, std::min(it_end + d->step_offset, d->nt));
+        int rc = check_desc(d, lo, hi);
+        if (rc) return fail(rc, "shot %d: %s", q, g_error.c_str());
+        rc = check_io(d, fwd, save);
+        if (rc) return fail(rc, "shot %d: %s", q, g_error.c_str());
+        if (d->nx != d0->nx || d->nz != d0->nz || d->fw != d0->fw || d->systems != d0->systems ||
+            d->dt != d0->dt || d->inv_dx != d0->inv_dx || d->inv_dz != d0->inv_dz ||
+            d->src_factor != d0->src_factor || d->sponge_table != d0->sponge_table ||
+            d->lam != d0->lam || d->mu != d0->mu || d->mxz != d0->mxz || d->myx != d0->myx ||
+            d->myz != d0->myz || d->inv_rho != d0->inv_rho || d->inv_rho_u != d0->inv_rho_u ||
+            d->inv_rho_w != d0->inv_rho_w || d->surface_matrix != d0->surface_matrix ||
+            d->tile_x != d0->tile_x || d->block_steps != d0->block_steps ||
+            d->tile_z != d0->tile_z)
+            return fail(TRTM_ERR_ARG, "shot %d does not share the medium and grid of shot 0", q);
+        if (!d->workspace || d->workspace_bytes < block_workspace_bytes(d))
+            return fail(TRTM_ERR_ARG, "shot %d workspace needs %lld bytes", q,
+                        (long long)block_workspace_bytes(d));
+    }
+    if (d0->systems == 0 || it_begin >= it_end) return TRTM_OK;
+    return run_block(descs, n, fwd, save, it_begin, it_end, static_cast<ShotDev*>(bws),
+                     make_kargs(d0), s);
+}
+
+int trtm_forward_steps_batch(const trtm_step_desc* descs, int32_t n, void* batch_workspace,
+                             int64_t batch_workspace_bytes, int32_t it_begin, int32_t it_end,
+                             int32_t save, void* stream) {
+    return run_batch(descs, n, batch_workspace, batch_workspace_bytes, it_begin, it_end, true,
+                     save != 0, (hipStream_t)stream);
+}
+
+int trtm_backward_steps_batch(const trtm_step_desc* descs, int32_t n, void* batch_workspace,
+                              int64_t batch_workspace_bytes, int32_t it_begin, int32_t it_end,
+                              void* stream) {
+    return run_batch(descs, n, batch_workspace, batch_workspace_bytes, it_begin, it_end, false,
+                     false, (hipStream_t)stream);
 }
 
 int trtm_forward_steps(const trtm_step_desc* d, int32_t it_begin, int32_t it_end, int32_t save,

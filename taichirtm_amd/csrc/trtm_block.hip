@@ -1,0 +1,782 @@
+// Temporally blocked, multi-shot time stepping (one launch per K steps).
+//
+// A workgroup owns a tile of TX x 64 cells of one shot.  It loads the tile
+// plus a K-cell halo on every side (the region, RX x RZ cells) of the state
+// into LDS once, keeps the region's medium coefficients and sponge factors in
+// registers, and advances K time steps inside LDS: per step the velocity
+// update, the injections, the stress update and the sponge
+// (forward_modeling.py:539-572, backward_modeling.py:539-576).  Each half-step
+// leaves one more halo row/column stale on one side (the velocity update reads
+// i-1/j-1 forward, the stress update i+1/j+1), so after K steps exactly the
+// tile is valid; halo cells are recomputed redundantly by the neighbouring
+// tiles with the same expressions, so every valid value is bit-identical to
+// the one-step-per-launch kernels.
+//
+// Per step the tile's receivers are recorded, snapshots written, the image
+// accumulated in registers (written back once per launch) and the finite
+// check taken, all from the post-sponge values.  Velocities stay unsponged in
+// LDS and the sponge factor is applied when they are next used, which gives
+// the same rounding as the reference's in-place multiply.
+//
+// The grid's second dimension is the shot: shots share the medium and differ
+// in state, record length, injection and recording geometry (ShotDev).
+#include <algorithm>
+#include <vector>
+
+#include "trtm_common.hpp"
+
+namespace trtm {
+
+int fail(int code, const char* fmt, ...);
+
+struct ShotDev {
+    float* in[8];   // u, v, w, sxx, szz, sxz, syx, syz read at launch start
+    float* out[8];  // written at launch end
+    int nt, it_end, save, isnap;
+    int num_snaps, snap_base, check_interval, n_groups;
+    int n_rec, it_lo, it_off, pad2;
+    const int *loc, *order, *start;
+    const float *su, *sv, *sw;
+    const int* rec_loc;
+    float *rec_u, *rec_v, *rec_w;
+    const int *inj_tstart, *inj_titems, *rec_tstart, *rec_titems;
+    float *snap_u, *snap_v, *snap_w;
+    const int* isnaps;
+    float *image_u, *image_v, *image_w;
+    int* check_flags;
+};
+
+__device__ __forceinline__ int xcd_tile_b(int bid, int tiles) {
+    const int per = (tiles + 7) / 8;
+    return (bid % 8) * per + bid / 8;
+}
+
+// Pointers read from the shot table are generic; cast them to the global
+// address space so the compiler emits global_* (not flat_*) memory
+// instructions, which do not share the LDS wait counter.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> G(T* p) {
+    return (gptr<T>)p;
+}
+
+template <bool FWD, bool PSV, bool SH, int TX, int TZ, int NT, int K>
+__global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict__ shots, int it0,
+                                              int tiles_x, int tiles_z) {
+    constexpr int RX = TX + 2 * K, RZ = TZ + 2 * K, RN = RX * RZ;
+    constexpr int NC = (RN + NT - 1) / NT;
+    constexpr int NP = PSV ? RN : 1, NS = SH ? RN : 1;
+    __shared__ float sU[NP], sW[NP], sXX[NP], sZZ[NP], sXZ[NP];
+    __shared__ float sV[NS], sYX[NS], sYZ[NS];
+
+    const ShotDev* __restrict__ sh = shots + blockIdx.y;
+    const int it_end = sh->it_end, it_lo = sh->it_lo, it_off = sh->it_off;
+    const int tiles = tiles_x * tiles_z;
+    const int tile = xcd_tile_b(blockIdx.x, tiles);
+    // this launch covers the shot's steps it0 + it_off .. it0 + it_off + K - 1
+    if (tile >= tiles || it0 + it_off >= it_end || it0 + it_off + K <= it_lo) return;
+    const int nt = sh->nt;
+    const int i0 = (tile / tiles_z) * TX, j0 = (tile % tiles_z) * TZ;
+    const int ri0 = i0 - K, rj0 = j0 - K;
+    const int nx = m.nx, nz = m.nz;
+    const int t = threadIdx.x;
+    const gptr<const float> sm = G(m.sm);
+
+    // per owned cell: global offset, flags, sponge factor, coefficients
+    int gc[NC];
+    unsigned fl[NC];  // bit0 in grid, bit1 interior, bit2 in tile, bit3 V ok, bit4 S ok
+    float A[NC], c0[NC], c1[NC], c2[NC], c3[NC], c4[NC], d0[NC], d1[NC], d2[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const int e = t + NT * k;
+        const int r = e / RZ, z = e - r * RZ;
+        const int i = ri0 + r, j = rj0 + z;
+        unsigned f = 0;
+        gc[k] = 0;
+        A[k] = 1.0f;
+        c0[k] = c1[k] = c2[k] = c3[k] = c4[k] = d0[k] = d1[k] = d2[k] = 0.0f;
+        if (e < RN && i >= 0 && i < nx && j >= 0 && j < nz) {
+            f |= 1u;
+            const int c = i * nz + j;
+            gc[k] = c;
+            if (i >= 1 && i < nx - 1 && j >= 1 && j < nz - 1) f |= 2u;
+            if (i >= i0 && i < i0 + TX && j >= j0 && j < j0 + TZ) f |= 4u;
+            const bool lo = r >= 1 && z >= 1, hi = r <= RX - 2 && z <= RZ - 2;
+            if (FWD ? lo : hi) f |= 8u;
+            if (FWD ? hi : lo) f |= 16u;
+            A[k] = sponge_at(i, j, nx, nz, m.fw, m.tab);
+            if (PSV) {
+                c0[k] = G(m.iru)[c];
+                c1[k] = G(m.irw)[c];
+                c2[k] = G(m.lam)[c];
+                c3[k] = G(m.mu)[c];
+                c4[k] = G(m.mxz)[c];
+            }
+            if (SH) {
+                d0[k] = G(m.ir)[c];
+                d1[k] = G(m.myx)[c];
+                d2[k] = G(m.myz)[c];
+            }
+        }
+        fl[k] = f;
+    }
+
+    // load the region's state
+    {
+        const gptr<float> in_u = G(sh->in[0]), in_v = G(sh->in[1]), in_w = G(sh->in[2]);
+        const gptr<float> in_xx = G(sh->in[3]), in_zz = G(sh->in[4]), in_xz = G(sh->in[5]);
+        const gptr<float> in_yx = G(sh->in[6]), in_yz = G(sh->in[7]);
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            const int e = t + NT * k;
+            if (e >= RN) continue;
+            const bool g = fl[k] & 1u;
+            const int c = gc[k];
+            if (PSV) {
+                sU[e] = g ? in_u[c] : 0.0f;
+                sW[e] = g ? in_w[c] : 0.0f;
+                sXX[e] = g ? in_xx[c] : 0.0f;
+                sZZ[e] = g ? in_zz[c] : 0.0f;
+                sXZ[e] = g ? in_xz[c] : 0.0f;
+            }
+            if (SH) {
+                sV[e] = g ? in_v[c] : 0.0f;
+                sYX[e] = g ? in_yx[c] : 0.0f;
+                sYZ[e] = g ? in_yz[c] : 0.0f;
+            }
+        }
+    }
+
+    // image accumulators of the owned tile cells
+    const bool imaging = !FWD && sh->num_snaps > 0;
+    const gptr<float> img_u = G(sh->image_u), img_v = G(sh->image_v), img_w = G(sh->image_w);
+    float aU[NC], aW[NC], aV[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        aU[k] = aW[k] = aV[k] = 0.0f;
+        if (imaging && (fl[k] & 4u)) {
+            if (PSV) {
+                aU[k] = img_u[gc[k]];
+                aW[k] = img_w[gc[k]];
+            }
+            if (SH) aV[k] = img_v[gc[k]];
+        }
+    }
+    __syncthreads();
+
+    const size_t n = (size_t)nx * nz;
+    const float sdt = FWD ? m.dt : -m.dt;
+    const float idx = m.idx, idz = m.idz, dt = m.dt;
+    const bool has_sm = m.sm != nullptr;
+    // stress read with the free-surface / topography mask of the cell
+    auto msk = [&](float x, int c) { return has_sm ? x * sm[c] : x; };
+    const int save = sh->save, isnap = sh->isnap, num_snaps = sh->num_snaps;
+    const int snap_base = sh->snap_base, cint = sh->check_interval;
+    const int n_groups = sh->n_groups, n_rec = sh->n_rec;
+    const gptr<float> snap_u = G(sh->snap_u), snap_v = G(sh->snap_v), snap_w = G(sh->snap_w);
+
+    bool started = false;  // a step ran in this launch: LDS velocities await the sponge
+    for (int s = 0; s < K; s++) {
+        const int it = it0 + it_off + s;
+        if (it < it_lo) continue;
+        if (it >= it_end) break;
+        const int tb = nt - 1 - it;  // backward time index
+
+        // ---- velocity update (region)
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            if (!(fl[k] & 1u)) continue;
+            const int e = t + NT * k;
+            const int c = gc[k];
+            const bool upd = (fl[k] & 2u) && (fl[k] & 8u);
+            if (PSV) {
+                float u = sU[e], w = sW[e];
+                if (started) {
+                    u = u * A[k];
+                    w = w * A[k];
+                }
+                if (upd) {
+                    float sxx_x, szz_z, sxz_x, sxz_z;
+                    if (FWD) {
+                        sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
+                        szz_z = (msk(sZZ[e], c) - msk(sZZ[e - 1], c - 1)) * idz;
+                        sxz_x = (msk(sXZ[e], c) - msk(sXZ[e - RZ], c - nz)) * idx;
+                        sxz_z = (msk(sXZ[e], c) - msk(sXZ[e - 1], c - 1)) * idz;
+                        u = u + ((sxx_x + sxz_z) * dt) * c0[k];
+                        w = w + ((sxz_x + szz_z) * dt) * c1[k];
+                    } else {
+                        sxx_x = (sXX[e + RZ] - sXX[e]) * idx;
+                        szz_z = (msk(sZZ[e + 1], c + 1) - msk(sZZ[e], c)) * idz;
+                        sxz_x = (msk(sXZ[e + RZ], c + nz) - msk(sXZ[e], c)) * idx;
+                        sxz_z = (msk(sXZ[e + 1], c + 1) - msk(sXZ[e], c)) * idz;
+                        u = u + ((-(sxx_x + sxz_z)) * dt) * c0[k];
+                        w = w + ((-(sxz_x + szz_z)) * dt) * c1[k];
+                    }
+                }
+                sU[e] = u;
+                sW[e] = w;
+            }
+            if (SH) {
+                float v = sV[e];
+                if (started) v = v * A[k];
+                if (upd) {
+                    float syx_x, syz_z;
+                    if (FWD) {
+                        syx_x = (sYX[e] - sYX[e - RZ]) * idx;
+                        syz_z = (msk(sYZ[e], c) - msk(sYZ[e - 1], c - 1)) * idz;
+                        v = v + ((syx_x + syz_z) * dt) * d0[k];
+                    } else {
+                        syx_x = (sYX[e + RZ] - sYX[e]) * idx;
+                        syz_z = (msk(sYZ[e + 1], c + 1) - msk(sYZ[e], c)) * idz;
+                        v = v + ((-(syx_x + syz_z)) * dt) * d0[k];
+                    }
+                }
+                sV[e] = v;
+            }
+        }
+        __syncthreads();
+
+        // ---- injection into the region (one thread per cell, entries in order)
+        if (n_groups > 0) {
+            const gptr<const int> tstart = G(sh->inj_tstart);
+            const int b = tstart[tile], end = tstart[tile + 1];
+            if (b < end) {
+                const gptr<const int> titems = G(sh->inj_titems), gstart = G(sh->start);
+                const gptr<const int> order = G(sh->order), loc = G(sh->loc);
+                const gptr<const float> su = G(sh->su), sv = G(sh->sv), sw = G(sh->sw);
+                const int col = FWD ? it : tb;
+                const size_t ntt = (size_t)nt;
+                for (int q = b + t; q < end; q += NT) {
+                    const int g = titems[q];
+                    const int e0 = gstart[g], e1 = gstart[g + 1];
+                    const int k0 = order[e0];
+                    const int ci = loc[2 * k0], cj = loc[2 * k0 + 1];
+                    const int e = (ci - ri0) * RZ + (cj - rj0);
+                    const int c = ci * nz + cj;
+                    if (PSV) {
+                        float x = sU[e], zz = sW[e];
+                        for (int q2 = e0; q2 < e1; q2++) {
+                            const size_t o = (size_t)order[q2] * ntt + col;
+                            if (FWD) {
+                                x = x + (su[o] * m.factor) * G(m.iru)[c];
+                                zz = zz + (sw[o] * m.factor) * G(m.irw)[c];
+                            } else {
+                                x = x + su[o];
+                                zz = zz + sw[o];
+                            }
+                        }
+                        sU[e] = x;
+                        sW[e] = zz;
+                    }
+                    if (SH) {
+                        float y = sV[e];
+                        for (int q2 = e0; q2 < e1; q2++) {
+                            const size_t o = (size_t)order[q2] * ntt + col;
+                            y = FWD ? y + (sv[o] * m.factor) * G(m.ir)[c] : y + sv[o];
+                        }
+                        sV[e] = y;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        // ---- stress update + sponge (region), outputs (tile)
+        int snap = -1, img = -1, chk = -1;
+        if (FWD && save && it % isnap == 0 && it != 0) {
+            const int sl = it / isnap - 1 - snap_base;
+            if (sl >= 0 && sl < num_snaps) snap = sl;
+        }
+        if (imaging && tb > 0 && tb % isnap == 0) {
+            const int sl = tb / isnap - 1 - snap_base;
+            if (sl >= 0 && sl < num_snaps && G(sh->isnaps)[sl] == tb) img = sl;
+        }
+        if (cint > 0 && it % cint == 0) chk = it / cint;
+        int code = 0;
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            if (!(fl[k] & 1u)) continue;
+            const int e = t + NT * k;
+            const int c = gc[k];
+            const bool upd = (fl[k] & 2u) && (fl[k] & 16u);
+            const bool own = fl[k] & 4u;
+            if (PSV) {
+                float sxx = sXX[e], szz = msk(sZZ[e], c), sxz = msk(sXZ[e], c);
+                if (upd) {
+                    float u_x, u_z, w_x, w_z;
+                    if (FWD) {
+                        u_x = (sU[e + RZ] - sU[e]) * idx;
+                        u_z = (sU[e + 1] - sU[e]) * idz;
+                        w_x = (sW[e + RZ] - sW[e]) * idx;
+                        w_z = (sW[e + 1] - sW[e]) * idz;
+                    } else {
+                        u_x = (sU[e] - sU[e - RZ]) * idx;
+                        u_z = (sU[e] - sU[e - 1]) * idz;
+                        w_x = (sW[e] - sW[e - RZ]) * idx;
+                        w_z = (sW[e] - sW[e - 1]) * idz;
+                    }
+                    const float div_uw = u_x + w_z;
+                    sxx = sxx + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
+                    szz = szz + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
+                    sxz = sxz + (sdt * c4[k]) * (u_z + w_x);
+                }
+                sXX[e] = sxx * A[k];
+                sZZ[e] = szz * A[k];
+                sXZ[e] = sxz * A[k];
+                if (own) {
+                    const float u = sU[e] * A[k], w = sW[e] * A[k];
+                    if (snap >= 0) {
+                        snap_u[(size_t)snap * n + c] = u;
+                        snap_w[(size_t)snap * n + c] = w;
+                    }
+                    if (img >= 0) {
+                        aU[k] = aU[k] + snap_u[(size_t)img * n + c] * u;
+                        aW[k] = aW[k] + snap_w[(size_t)img * n + c] * w;
+                    }
+                    if (chk >= 0) {
+                        if (not_finite_or_big(u) && code < 1) code = 1;
+                        if (not_finite_or_big(w)) code = 3;
+                    }
+                }
+            }
+            if (SH) {
+                float syx = sYX[e], syz = msk(sYZ[e], c);
+                if (upd) {
+                    float v_x, v_z;
+                    if (FWD) {
+                        v_x = (sV[e + RZ] - sV[e]) * idx;
+                        v_z = (sV[e + 1] - sV[e]) * idz;
+                    } else {
+                        v_x = (sV[e] - sV[e - RZ]) * idx;
+                        v_z = (sV[e] - sV[e - 1]) * idz;
+                    }
+                    syx = syx + (sdt * d1[k]) * v_x;
+                    syz = syz + (sdt * d2[k]) * v_z;
+                }
+                sYX[e] = syx * A[k];
+                sYZ[e] = syz * A[k];
+                if (own) {
+                    const float v = sV[e] * A[k];
+                    if (snap >= 0) snap_v[(size_t)snap * n + c] = v;
+                    if (img >= 0) aV[k] = aV[k] + snap_v[(size_t)img * n + c] * v;
+                    if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
+                }
+            }
+        }
+        if (code) atomicMax(sh->check_flags + chk, code + (FWD ? 0 : 3));
+
+        // ---- receivers of the tile, after the sponge
+        if (n_rec > 0) {
+            const gptr<const int> rstart = G(sh->rec_tstart);
+            const int b = rstart[tile], end = rstart[tile + 1];
+            if (b < end) {
+                const gptr<const int> ritems = G(sh->rec_titems), rloc = G(sh->rec_loc);
+                const gptr<float> ru = G(sh->rec_u), rv = G(sh->rec_v), rw = G(sh->rec_w);
+                const size_t row = (size_t)(FWD ? it : tb) * n_rec;
+                for (int q = b + t; q < end; q += NT) {
+                    const int l = ritems[q];
+                    const int ci = rloc[2 * l], cj = rloc[2 * l + 1];
+                    const int e = (ci - ri0) * RZ + (cj - rj0);
+                    const float Ac = sponge_at(ci, cj, nx, nz, m.fw, m.tab);
+                    if (PSV) {
+                        ru[row + l] = sU[e] * Ac;
+                        rw[row + l] = sW[e] * Ac;
+                    }
+                    if (SH) rv[row + l] = sV[e] * Ac;
+                }
+            }
+        }
+        started = true;
+        __syncthreads();
+    }
+
+    // ---- write the tile's state (post-sponge velocities) and image
+    const gptr<float> o_u = G(sh->out[0]), o_v = G(sh->out[1]), o_w = G(sh->out[2]);
+    const gptr<float> o_xx = G(sh->out[3]), o_zz = G(sh->out[4]), o_xz = G(sh->out[5]);
+    const gptr<float> o_yx = G(sh->out[6]), o_yz = G(sh->out[7]);
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        if ((fl[k] & 5u) != 5u) continue;
+        const int e = t + NT * k;
+        const int c = gc[k];
+        if (PSV) {
+            o_u[c] = sU[e] * A[k];
+            o_w[c] = sW[e] * A[k];
+            o_xx[c] = sXX[e];
+            o_zz[c] = sZZ[e];
+            o_xz[c] = sXZ[e];
+            if (imaging) {
+                img_u[c] = aU[k];
+                img_w[c] = aW[k];
+            }
+        }
+        if (SH) {
+            o_v[c] = sV[e] * A[k];
+            o_yx[c] = sYX[e];
+            o_yz[c] = sYZ[e];
+            if (imaging) img_v[c] = aV[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct BlockCfg {
+    int tx, tz, nt, k;  // tile rows, tile columns, threads per workgroup, steps per launch
+};
+
+// configurations compiled per wave system
+static const BlockCfg kCfgSH[] = {{16, 64, 256, 4},   {16, 64, 256, 8},    {16, 128, 512, 4},
+                                  {16, 128, 512, 8},  {32, 128, 1024, 4},  {32, 128, 1024, 8}};
+static const BlockCfg kCfgPSV[] = {{16, 64, 256, 2},  {16, 64, 256, 4},    {16, 128, 512, 4},
+                                   {16, 128, 512, 8}, {32, 128, 1024, 4}};
+static const BlockCfg kCfgBoth[] = {{16, 64, 256, 2}, {16, 64, 256, 4}, {16, 128, 512, 4}};
+
+static void cfg_list(int systems, const BlockCfg** list, int* n) {
+    if (systems == TRTM_SH) {
+        *list = kCfgSH;
+        *n = sizeof(kCfgSH) / sizeof(BlockCfg);
+    } else if (systems == TRTM_PSV) {
+        *list = kCfgPSV;
+        *n = sizeof(kCfgPSV) / sizeof(BlockCfg);
+    } else {
+        *list = kCfgBoth;
+        *n = sizeof(kCfgBoth) / sizeof(BlockCfg);
+    }
+}
+
+// defaults: the fastest shapes measured on MI355X for the 1200 x 600 SH and
+// 200 x 200 / 2000 x 1000 P-SV workloads
+static BlockCfg default_cfg(int systems) {
+    if (systems == TRTM_SH) return {32, 128, 1024, 8};
+    if (systems == TRTM_PSV) return {32, 128, 1024, 4};
+    return {16, 128, 512, 4};
+}
+
+// first compiled configuration matching every nonzero request field
+int block_choose(const trtm_step_desc* d, BlockCfg* out) {
+    const BlockCfg def = default_cfg(d->systems);
+    if (d->tile_x <= 0 && d->tile_z <= 0 && d->block_steps <= 0) {
+        *out = def;
+        return TRTM_OK;
+    }
+    const BlockCfg* list;
+    int n;
+    cfg_list(d->systems, &list, &n);
+    for (int i = 0; i < n; i++) {
+        const BlockCfg& c = list[i];
+        if ((d->tile_x <= 0 || c.tx == d->tile_x) && (d->tile_z <= 0 || c.tz == d->tile_z) &&
+            (d->block_steps <= 0 || c.k == d->block_steps)) {
+            *out = c;
+            return TRTM_OK;
+        }
+    }
+    return fail(TRTM_ERR_UNSUPPORTED,
+                "blocked kernel has no tile_x=%d tile_z=%d block_steps=%d build for systems=%d",
+                d->tile_x, d->tile_z, d->block_steps, d->systems);
+}
+
+static size_t align256b(size_t x) { return (x + 255) & ~size_t(255); }
+
+// workspace of one shot: second state buffer, tile lists sized for the
+// smallest tiles (16 x 64), device copy of isnaps, two shot-table entries
+struct BlockLayout {
+    size_t alt, inj_start, inj_cursor, inj_items, rec_start, rec_cursor, rec_items, isnaps, shot,
+        total;
+    int tiles_x, tiles_z;
+};
+
+static BlockLayout block_layout(const trtm_step_desc* d, BlockCfg c) {
+    BlockLayout L;
+    const size_t n = (size_t)d->nx * d->nz;
+    L.tiles_x = (d->nx + c.tx - 1) / c.tx;
+    L.tiles_z = (d->nz + c.tz - 1) / c.tz;
+    const size_t tiles_max = (size_t)((d->nx + 15) / 16) * ((d->nz + 63) / 64);
+    const size_t mult = 4;  // a cell lies in at most 2 x 2 regions when 2K <= min(TX, TZ)
+    size_t off = 0;
+    L.alt = off;        off = align256b(off + 8 * n * sizeof(float));
+    L.inj_start = off;  off = align256b(off + (tiles_max + 1) * sizeof(int));
+    L.inj_cursor = off; off = align256b(off + tiles_max * sizeof(int));
+    L.inj_items = off;  off = align256b(off + mult * (size_t)std::max(d->n_inj_groups, 0) * sizeof(int));
+    L.rec_start = off;  off = align256b(off + (tiles_max + 1) * sizeof(int));
+    L.rec_cursor = off; off = align256b(off + tiles_max * sizeof(int));
+    L.rec_items = off;  off = align256b(off + (size_t)std::max(d->n_rec, 0) * sizeof(int));
+    L.isnaps = off;     off = align256b(off + (size_t)std::max(d->num_snaps, 0) * sizeof(int));
+    L.shot = off;       off = align256b(off + 2 * sizeof(ShotDev));
+    L.total = off;
+    return L;
+}
+
+int64_t block_workspace_bytes(const trtm_step_desc* d) {
+    return (int64_t)block_layout(d, BlockCfg{16, 64, 256, 1}).total;
+}
+
+__global__ void k_fill_int_b(int* p, int n, int value) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = value;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_b(const int* count, int* start, int n) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int chunk = (n + 1023) / 1024;
+    const int b = t * chunk, e = min(n, b + chunk);
+    int s = 0;
+    for (int k = b; k < e; k++) s += count[k];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = t > 0 ? part[t - 1] : 0;
+    for (int k = b; k < e; k++) {
+        start[k] = run;
+        run += count[k];
+    }
+    if (t == 1023) start[n] = part[1023];
+}
+
+// mode 0 counts, mode 1 fills.  Groups go to every tile whose region (tile +
+// halo K) holds their cell; receivers to the tile that holds their cell.
+__global__ void k_block_lists(int mode, int tx, int tz, int K, int tiles_x, int tiles_z,
+                              int n_groups, const int* loc, const int* order, const int* start,
+                              int* icount, const int* istart, int* iitems, int n_rec,
+                              const int* rloc, int* rcount, const int* rstart, int* ritems) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n_groups) {
+        const int k0 = order[start[q]];
+        const int ci = loc[2 * k0], cj = loc[2 * k0 + 1];
+        for (int ti = (ci - K - tx + 1) / tx - 1; ti <= (ci + K) / tx + 1; ti++)
+            for (int tk = (cj - K - tz + 1) / tz - 1; tk <= (cj + K) / tz + 1; tk++) {
+                if (ti < 0 || ti >= tiles_x || tk < 0 || tk >= tiles_z) continue;
+                const int ri0 = ti * tx - K, rj0 = tk * tz - K;
+                if (ci < ri0 || ci >= ri0 + tx + 2 * K || cj < rj0 || cj >= rj0 + tz + 2 * K)
+                    continue;
+                const int tile = ti * tiles_z + tk;
+                const int pos = atomicAdd(icount + tile, 1);
+                if (mode == 1) iitems[istart[tile] + pos] = q;
+            }
+    }
+    if (q < n_rec) {
+        const int tile = (rloc[2 * q] / tx) * tiles_z + rloc[2 * q + 1] / tz;
+        const int pos = atomicAdd(rcount + tile, 1);
+        if (mode == 1) ritems[rstart[tile] + pos] = q;
+    }
+}
+
+static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg, int it_lo,
+                      int it_hi, hipStream_t s, ShotDev* sd) {
+    const int tx = cfg.tx, K = cfg.k;
+    const BlockLayout L = block_layout(d, cfg);
+    char* ws = static_cast<char*>(d->workspace);
+    const int tiles = L.tiles_x * L.tiles_z;
+    int* istart = reinterpret_cast<int*>(ws + L.inj_start);
+    int* icur = reinterpret_cast<int*>(ws + L.inj_cursor);
+    int* iitems = reinterpret_cast<int*>(ws + L.inj_items);
+    int* rstart = reinterpret_cast<int*>(ws + L.rec_start);
+    int* rcur = reinterpret_cast<int*>(ws + L.rec_cursor);
+    int* ritems = reinterpret_cast<int*>(ws + L.rec_items);
+    const int ng = d->n_inj_groups;
+    const int nr = (d->rec_u || d->rec_v) ? d->n_rec : 0;
+    const dim3 tb(256), tg((tiles + 255) / 256), lg((std::max(ng, nr) + 255) / 256);
+    hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, icur, tiles, 0);
+    hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, rcur, tiles, 0);
+    if (ng > 0 || nr > 0)
+        hipLaunchKernelGGL(k_block_lists, lg, tb, 0, s, 0, tx, cfg.tz, K, L.tiles_x, L.tiles_z, ng,
+                           d->inj_loc, d->inj_order, d->inj_group_start, icur, (const int*)nullptr,
+                           (int*)nullptr, nr, d->rec_loc, rcur, (const int*)nullptr, (int*)nullptr);
+    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, s, icur, istart, tiles);
+    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, s, rcur, rstart, tiles);
+    hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, icur, tiles, 0);
+    hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, rcur, tiles, 0);
+    if (ng > 0 || nr > 0)
+        hipLaunchKernelGGL(k_block_lists, lg, tb, 0, s, 1, tx, cfg.tz, K, L.tiles_x, L.tiles_z, ng,
+                           d->inj_loc, d->inj_order, d->inj_group_start, icur, (const int*)istart,
+                           iitems, nr, d->rec_loc, rcur, (const int*)rstart, ritems);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(TRTM_ERR_HIP, "blocked tile lists: %s", hipGetErrorString(e));
+
+    ShotDev& x = *sd;
+    float* state[8] = {d->u, d->v, d->w, d->sxx, d->szz, d->sxz, d->syx, d->syz};
+    float* alt = reinterpret_cast<float*>(ws + L.alt);
+    const size_t n = (size_t)d->nx * d->nz;
+    for (int k = 0; k < 8; k++) {
+        x.in[k] = state[k];
+        x.out[k] = alt + k * n;
+    }
+    x.nt = d->nt;
+    x.it_end = it_hi;
+    x.it_lo = it_lo;
+    x.it_off = d->step_offset;
+    x.save = save ? 1 : 0;
+    x.isnap = d->isnap > 0 ? d->isnap : 1;
+    x.num_snaps = (save || !fwd) ? d->num_snaps : 0;
+    if (!fwd && (!d->isnaps_host || d->num_snaps <= 0)) x.num_snaps = 0;
+    x.snap_base = d->snap_base;
+    x.check_interval = d->check_interval;
+    x.n_groups = ng;
+    x.n_rec = nr;
+    x.pad2 = 0;
+    x.loc = d->inj_loc;
+    x.order = d->inj_order;
+    x.start = d->inj_group_start;
+    x.su = d->inj_u;
+    x.sv = d->inj_v;
+    x.sw = d->inj_w;
+    x.rec_loc = d->rec_loc;
+    x.rec_u = d->rec_u;
+    x.rec_v = d->rec_v;
+    x.rec_w = d->rec_w;
+    x.inj_tstart = istart;
+    x.inj_titems = iitems;
+    x.rec_tstart = rstart;
+    x.rec_titems = ritems;
+    x.snap_u = d->snap_u;
+    x.snap_v = d->snap_v;
+    x.snap_w = d->snap_w;
+    x.isnaps = nullptr;
+    if (!fwd && x.num_snaps > 0) {
+        int* dev = reinterpret_cast<int*>(ws + L.isnaps);
+        e = hipMemcpyAsync(dev, d->isnaps_host, sizeof(int) * d->num_snaps,
+                           hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return fail(TRTM_ERR_HIP, "isnaps copy: %s", hipGetErrorString(e));
+        x.isnaps = dev;
+    }
+    x.image_u = d->image_u;
+    x.image_v = d->image_v;
+    x.image_w = d->image_w;
+    x.check_flags = d->check_flags;
+    return TRTM_OK;
+}
+
+template <bool FWD, bool PSV, bool SH, int TX, int TZ, int NT, int K>
+static void launch_cfg(const KArgs& m, const ShotDev* shots, int n_shots, int it0, int tiles_x,
+                       int tiles_z, hipStream_t s) {
+    const int tiles = tiles_x * tiles_z;
+    const dim3 grid(8 * ((tiles + 7) / 8), n_shots);
+    hipLaunchKernelGGL((k_block<FWD, PSV, SH, TX, TZ, NT, K>), grid, dim3(NT), 0, s, m, shots,
+                       it0, tiles_x, tiles_z);
+}
+
+template <bool FWD, bool PSV, bool SH>
+static bool launch_sys(BlockCfg c, const KArgs& m, const ShotDev* shots, int n_shots, int it0,
+                       int tiles_x, int tiles_z, hipStream_t s) {
+#define TRTM_CFG(TXV, TZV, NTV, KV)                                                          \
+    if (c.tx == TXV && c.tz == TZV && c.nt == NTV && c.k == KV) {                            \
+        launch_cfg<FWD, PSV, SH, TXV, TZV, NTV, KV>(m, shots, n_shots, it0, tiles_x, tiles_z, s); \
+        return true;                                                                         \
+    }
+    if constexpr (PSV && SH) {
+        TRTM_CFG(16, 64, 256, 2) TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 128, 512, 4)
+    } else if constexpr (PSV) {
+        TRTM_CFG(16, 64, 256, 2) TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 128, 512, 4)
+        TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4)
+    } else {
+        TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 64, 256, 8) TRTM_CFG(16, 128, 512, 4)
+        TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4) TRTM_CFG(32, 128, 1024, 8)
+    }
+#undef TRTM_CFG
+    return false;
+}
+
+static bool launch_block(bool fwd, int systems, BlockCfg c, const KArgs& m, const ShotDev* shots,
+                         int n_shots, int it0, int tiles_x, int tiles_z, hipStream_t s) {
+    const bool psv = systems & TRTM_PSV, sh = systems & TRTM_SH;
+    if (fwd) {
+        if (psv && sh) return launch_sys<true, true, true>(c, m, shots, n_shots, it0, tiles_x, tiles_z, s);
+        if (psv) return launch_sys<true, true, false>(c, m, shots, n_shots, it0, tiles_x, tiles_z, s);
+        return launch_sys<true, false, true>(c, m, shots, n_shots, it0, tiles_x, tiles_z, s);
+    }
+    if (psv && sh) return launch_sys<false, true, true>(c, m, shots, n_shots, it0, tiles_x, tiles_z, s);
+    if (psv) return launch_sys<false, true, false>(c, m, shots, n_shots, it0, tiles_x, tiles_z, s);
+    return launch_sys<false, false, true>(c, m, shots, n_shots, it0, tiles_x, tiles_z, s);
+}
+
+// Runs global steps [it_begin, it_end) for n shots; shot q runs its own steps
+// [it_begin, it_end) + step_offset, clipped to [0, nt).  One launch covers K
+// global steps.  State ping-pongs between the caller's buffers and the shot's
+// workspace, with each shot's phase set by the first launch it takes part in.
+// shots_dev: device room for 2n ShotDev entries.
+int run_block(const trtm_step_desc* descs, int n, bool fwd, bool save, int32_t it_begin,
+              int32_t it_end, ShotDev* shots_dev, const KArgs& m, hipStream_t s) {
+    const trtm_step_desc* d0 = &descs[0];
+    BlockCfg cfg;
+    int rc = block_choose(d0, &cfg);
+    if (rc) return rc;
+    const int K = cfg.k;
+    const int n_launch = (it_end - it_begin + K - 1) / K;
+    std::vector<ShotDev> even(n), odd(n);
+    std::vector<int> first(n, -1), count(n, 0);
+    for (int q = 0; q < n; q++) {
+        const trtm_step_desc* d = &descs[q];
+        const int lo = std::max(0, it_begin + d->step_offset);
+        const int hi = std::min(d->nt, it_end + d->step_offset);
+        rc = build_shot(d, fwd, save, cfg, lo, hi, s, &even[q]);
+        if (rc) return rc;
+        for (int j = 0; j < n_launch; j++) {
+            const int a = it_begin + j * K + d->step_offset;
+            if (std::max(a, lo) < std::min(a + K, hi)) {
+                if (first[q] < 0) first[q] = j;
+                count[q]++;
+            }
+        }
+        ShotDev sw = even[q];
+        for (int k = 0; k < 8; k++) std::swap(sw.in[k], sw.out[k]);
+        odd[q] = sw;
+        if (first[q] > 0 && first[q] % 2 == 1) std::swap(even[q], odd[q]);
+    }
+    const BlockLayout L = block_layout(d0, cfg);
+    hipError_t e = hipMemcpyAsync(shots_dev, even.data(), sizeof(ShotDev) * n,
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(shots_dev + n, odd.data(), sizeof(ShotDev) * n, hipMemcpyHostToDevice,
+                           s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // host tables must outlive the copies
+    if (e != hipSuccess) return fail(TRTM_ERR_HIP, "shot table copy: %s", hipGetErrorString(e));
+    for (int j = 0; j < n_launch; j++) {
+        bool any = false;
+        for (int q = 0; q < n && !any; q++)
+            any = first[q] >= 0 && j >= first[q] && j < first[q] + count[q];
+        if (!any) continue;
+        const ShotDev* tab = shots_dev + (j % 2 ? n : 0);
+        if (!launch_block(fwd, d0->systems, cfg, m, tab, n, it_begin + j * K, L.tiles_x,
+                          L.tiles_z, s))
+            return fail(TRTM_ERR_UNSUPPORTED, "no blocked kernel for the chosen configuration");
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(TRTM_ERR_HIP, "blocked launch: %s", hipGetErrorString(e));
+    // a shot that took part in an odd number of launches has its latest state
+    // in the workspace: copy it back
+    const size_t bytes = (size_t)d0->nx * d0->nz * sizeof(float);
+    const bool psv = d0->systems & TRTM_PSV, sh = d0->systems & TRTM_SH;
+    const bool act[8] = {psv, sh, psv, psv, psv, psv, sh, sh};
+    float* alt_base = nullptr;
+    for (int q = 0; q < n; q++) {
+        if (count[q] % 2 == 0) continue;
+        const trtm_step_desc* d = &descs[q];
+        float* state[8] = {d->u, d->v, d->w, d->sxx, d->szz, d->sxz, d->syx, d->syz};
+        alt_base = reinterpret_cast<float*>(static_cast<char*>(d->workspace) + L.alt);
+        const size_t nn = (size_t)d->nx * d->nz;
+        for (int k = 0; k < 8; k++)
+            if (act[k]) {
+                e = hipMemcpyAsync(state[k], alt_base + k * nn, bytes, hipMemcpyDeviceToDevice, s);
+                if (e != hipSuccess) return fail(TRTM_ERR_HIP, "state copy: %s", hipGetErrorString(e));
+            }
+    }
+    return TRTM_OK;
+}
+
+int64_t block_shot_table_bytes(int n) { return (int64_t)(2 * n * sizeof(ShotDev) + 256); }
+
+ShotDev* block_shot_slot(const trtm_step_desc* d) {
+    // a single shot keeps its two-entry table in its own workspace
+    const BlockLayout L = block_layout(d, BlockCfg{16, 64, 256, 1});
+    return reinterpret_cast<ShotDev*>(static_cast<char*>(d->workspace) + L.shot);
+}
+
+}  // namespace trtm

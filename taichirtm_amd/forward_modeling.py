@@ -100,7 +100,6 @@ class ForwardModeling(Propagator):
                     self.isnaps_field.shape != (num_snaps,):
                 self.isnaps_field = Field(np.int32, (num_snaps,))
         d = self._desc()
-        ws = self._workspace(d)
         flags = torch.zeros(nt // sci + 1, dtype=torch.int32, device=dev)
         d.check_interval = sci
         d.check_flags = _ptr(flags)
@@ -110,6 +109,7 @@ class ForwardModeling(Propagator):
             d.snap_u, d.snap_v, d.snap_w = (_ptr(getattr(self, c + "_save_field"))
                                             for c in ("u", "v", "w"))
         d.rec_u, d.rec_v, d.rec_w = (_ptr(self._seis.get(c)) for c in ("u", "v", "w"))
+        ws = self._workspace(d)  # noqa: F841  (kept alive for the queued kernels)
         stops = range(0, nt, self.isnap) if display_callback is not None else ()
         code, it_fail = 0, None
         s = stream_handle()

@@ -9,6 +9,7 @@ def run_hip(kw, wavelets, isnap=1, kernel="fused", obs=None, sci=100, surface=No
             callback=None, backward=True):
     import taichirtm_amd as trtm
     kw = dict(kw)
+    kw.pop("kernel", None)
     if surface is not None:
         kw["surface_matrix_field"] = surface
     wu, wv, ww = wavelets

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _lib.load().trtm_abi_version() == _lib.ABI_VERSION == 1
+    assert _lib.load().trtm_abi_version() == _lib.ABI_VERSION == 2
 
 
 def test_descriptor_layout_matches_c():

@@ -1,0 +1,77 @@
+"""Batched many-shot RTM (one launch per time block for all shots) and
+checkpointed recomputation: bitwise equal to single-shot runs of the same
+shots, and to the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from taichirtm_amd import configs
+from taichirtm_amd.batch import BatchRTM, ShotSpec, checkpoint_interval
+from tests.parity import run_hip, run_oracle, same
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    print("\n[case]", name, flush=True)
+
+
+def survey(system="psv", nx=70, nz=52, n_shots=3, seed=5):
+    """Layered model, shots with different record lengths and receiver spreads."""
+    rng = np.random.default_rng(seed)
+    cfg = configs.sh_layered(nx=nx, nz=nz, nt=10, fw=8, f0=45.0, seed=seed)
+    mu, lam = cfg.moduli()
+    shots = []
+    for q in range(n_shots):
+        nt = int(rng.integers(90, 160))
+        lo = int(rng.integers(0, nx // 3))
+        hi = int(rng.integers(2 * nx // 3, nx + 1))
+        rec = np.stack([np.arange(lo, hi), np.ones(hi - lo, int)], 1).astype(np.int32)
+        src = np.array([[int(rng.integers(10, nx - 10)), 1]], np.int32)
+        w = configs.ricker(45.0, nt, cfg.dt)[None]
+        z = np.zeros_like(w)
+        wav = (w, z, 0.7 * w) if system == "psv" else (z, w, z) if system == "sh" else (w, w, w)
+        shots.append(ShotSpec(src, wav, rec))
+    return cfg, mu, lam, shots
+
+
+def _kw(cfg, mu, lam, shot):
+    return dict(nx=cfg.nx, nz=cfg.nz, dx=cfg.dx, dz=cfg.dz, nt=shot.nt, fs=cfg.fs, mu_field=mu,
+                lam_field=lam, rho_field=cfg.rho, absorbing_frame=cfg.fw,
+                src_loc_field=shot.src_loc, recv_loc_field=shot.rec_loc,
+                num_sources=len(shot.src_loc), num_receivers=len(shot.rec_loc))
+
+
+@pytest.mark.parametrize("system", ["psv", "sh", "both"])
+@pytest.mark.parametrize("imaging,isnap", [("snapshot", 1), ("snapshot", 3), ("checkpoint", 1),
+                                           ("checkpoint", 3)])
+def test_batch_equals_single_shots(system, imaging, isnap):
+    _case("batch %s %s isnap=%d" % (system, imaging, isnap))
+    cfg, mu, lam, shots = survey(system)
+    b = BatchRTM(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, mu, lam, cfg.rho, cfg.fw, shots,
+                 isnap=isnap, imaging=imaging,
+                 checkpoint_every=(12 if isnap == 3 else 16) if imaging == "checkpoint" else None)
+    assert b.run() == [0] * len(shots)
+    imgs = b.images()
+    seis = b.seismograms()
+    for q, s in enumerate(shots):
+        h = run_hip(_kw(cfg, mu, lam, s), s.wavelets, isnap=isnap, kernel="fused")
+        for c in range(3):
+            assert same(seis[q][c], h["seis"][c]), (q, c)
+            assert same(imgs[q][c], h["image"][c]), (q, c)
+    o = run_oracle(_kw(cfg, mu, lam, shots[0]), shots[0].wavelets, isnap=isnap)
+    for c in range(3):
+        assert same(imgs[0][c], o["image"][c])
+    st = b.stacked_image()
+    ref = sum(np.asarray(im[2], np.float64) for im in imgs)
+    comp = "v" if system == "sh" else "w"
+    ref = sum(np.asarray(im[1 if system == "sh" else 2], np.float64) for im in imgs)
+    assert np.allclose(st[comp].cpu().numpy(), ref, rtol=0, atol=0)
+
+
+def test_checkpoint_interval_model():
+    L = checkpoint_interval(4000, 5, 2)
+    assert 50 <= L <= 200
+    assert checkpoint_interval(100, 3, 1, isnap=4, align=4) % 4 == 0
+    with pytest.raises(MemoryError):
+        checkpoint_interval(8000, 5, 2, budget_bytes=1e6, cells=8_000_000)

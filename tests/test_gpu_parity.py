@@ -10,7 +10,7 @@ from taichirtm_amd import configs
 from tests.parity import assert_bitwise, run_hip, run_oracle, same
 
 pytestmark = pytest.mark.gpu
-KERNELS = ["fused", "simple"]
+KERNELS = ["blocked", "fused", "simple"]
 
 
 def _case(name):
@@ -202,3 +202,29 @@ def test_sh_layered_reduced_parity_and_full_size_kernels_agree():
     for c in range(3):
         assert same(a["seis"][c], b["seis"][c]) and same(a["image"][c], b["image"][c])
     assert np.abs(a["image"][1]).max() > 0
+
+
+BLOCK_CFGS = {"psv": [(16, 64, 2), (16, 64, 4), (16, 128, 4), (16, 128, 8), (32, 128, 4)],
+              "sh": [(16, 64, 4), (16, 64, 8), (16, 128, 4), (16, 128, 8), (32, 128, 4),
+                     (32, 128, 8)],
+              "both": [(16, 64, 2), (16, 64, 4), (16, 128, 4)]}
+
+
+@pytest.mark.parametrize("system,cfg", [(s, c) for s, cs in BLOCK_CFGS.items() for c in cs])
+def test_blocked_configurations_bitwise(system, cfg):
+    """Every compiled tile / time-block shape of the blocked kernel, with nt not
+    a multiple of the block depth, duplicate sources and border receivers."""
+    tx, tz, k = cfg
+    _case("blocked %s tile_x=%d tile_z=%d block_steps=%d" % (system, tx, tz, k))
+    c = small_psv(nt=97) if system != "sh" else small_sh(nt=97)
+    kw = c.kwargs(c.shots[0])
+    kw["src_loc_field"] = np.array([[30, 5], [30, 5]], np.int32)
+    kw["num_sources"] = 2
+    w = configs.ricker(40.0, c.nt, c.dt)
+    z = np.zeros_like(w)
+    comps = {"psv": (w, z, w), "sh": (z, w, z), "both": (w, w, w)}[system]
+    wav = tuple(np.stack([x, 0.5 * x]) for x in comps)
+    hk = dict(kw, tile_x=tx, tile_z=tz, block_steps=k)
+    h = run_hip(hk, wav, isnap=3, kernel="blocked")
+    o = run_oracle(kw, wav, isnap=3)
+    assert_bitwise(h, o)
