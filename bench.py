@@ -47,6 +47,12 @@ ALGO_BYTES = {
 }
 
 
+# measured HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, copy-calibrated)
+MEASURED_TRAFFIC = {
+    ("sh_layered_1200x600x4000", "blocked", (32, 128, 1024, 8)): 57.4e6,
+}
+
+
 def build_workload(name):
     from taichirtm_amd import configs
     if name == "sh_layered":
@@ -191,11 +197,20 @@ def main():
     cells = cfg.nx * cfg.nz
     ms_per_step = wall * 1000.0 / args.steps
     gcells = 2.0 * cells * nt / (ms_per_step / 1000.0) / 1e9
-    bw_launch_us = 1000.0 * float(np.mean(bw_ms)) / nt
-    fw_launch_us = 1000.0 * float(np.mean(fw_ms)) / nt
-    b_bytes = ALGO_BYTES[(system, "backward")] * cells
-    f_bytes = ALGO_BYTES[(system, "forward")] * cells
+    # time steps per launch of the step kernel (1 for the fused and simple kernels)
+    k = shot.bw.block_config()[3] if args.kernel in ("blocked", "auto") else 1
+    launches = (nt + k - 1) // k
+    bw_launch_us = 1000.0 * float(np.mean(bw_ms)) / launches
+    fw_launch_us = 1000.0 * float(np.mean(fw_ms)) / launches
+    b_bytes = ALGO_BYTES[(system, "backward")] * cells * k
+    f_bytes = ALGO_BYTES[(system, "forward")] * cells * k
     achieved = b_bytes / (bw_launch_us * 1e-6) / 1e9
+    kname = ("k_block backward, %d steps per launch, image accumulated in registers" % k
+             if k > 1 else "k_fused backward step with imaging")
+    # HBM bytes per launch of the default blocked SH configuration, from the
+    # FETCH_SIZE / WRITE_SIZE passes recorded under profiles/
+    traffic = MEASURED_TRAFFIC.get((cfg.name, args.kernel, shot.bw.block_config()
+                                    if args.kernel in ("blocked", "auto") else None))
     out = {
         "metric": METRIC,
         "value": round(gcells, 3),
@@ -214,15 +229,15 @@ def main():
                    "grid": [cfg.nx, cfg.nz], "nt": nt, "dx": cfg.dx, "dt": cfg.dt,
                    "absorbing_frame": cfg.fw, "receivers": len(cfg.shots[0].rec),
                    "kernel": args.kernel, "tile_x": args.tile_x, "tile_z": args.tile_z,
-                   "block_steps": args.block_steps,
-                   "system": system},
+                   "block_steps": args.block_steps, "system": system},
         "shots_per_hour": round(3600.0 / (ms_per_step / 1000.0), 1),
         "phases_ms": {"forward": round(float(np.mean(fw_ms)), 3),
                       "backward": round(float(np.mean(bw_ms)), 3)},
-        "roofline": {"bound": "hbm", "kernel": "k_fused backward step with imaging",
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": b_bytes,
+                     "steps_per_launch": k,
                      "avg_launch_us": round(bw_launch_us, 3),
                      "forward_launch_us": round(fw_launch_us, 3),
                      "forward_achieved": round(f_bytes / (fw_launch_us * 1e-6) / 1e9, 1)},

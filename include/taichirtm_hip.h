@@ -156,6 +156,13 @@ typedef struct trtm_step_desc {
 /* Scratch bytes trtm_forward_steps / trtm_backward_steps need for desc. */
 int64_t trtm_workspace_bytes(const trtm_step_desc *desc);
 
+/*
+ * Shape the blocked kernel uses for desc (its systems, tile_x, tile_z and
+ * block_steps requests): out[0] tile rows, out[1] tile columns, out[2]
+ * threads per workgroup, out[3] time steps per launch.
+ */
+int trtm_block_config(const trtm_step_desc *desc, int32_t *out);
+
 /* Scratch bytes of the batch-level buffer for trtm_*_steps_batch with n shots. */
 int64_t trtm_batch_workspace_bytes(int32_t n);
 

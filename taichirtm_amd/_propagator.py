@@ -265,6 +265,14 @@ class Propagator:
         d.rec_loc = _ptr(self._rec_dev_loc()) if len(rl) else None
         return d
 
+    def block_config(self):
+        """(tile rows, tile columns, threads, steps per launch) of the blocked
+        kernel for this propagator's wave systems and requests."""
+        d = self._desc()
+        out = (ctypes.c_int32 * 4)()
+        _lib.call("trtm_block_config", d, out)
+        return tuple(out)
+
     def _workspace(self, d):
         if self.kernel == "simple":
             return None

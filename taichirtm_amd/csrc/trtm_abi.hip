@@ -24,6 +24,7 @@ int fused_prepare(const trtm_step_desc* d, bool fwd, hipStream_t s, TilePlan* pl
 struct ShotDev;
 int64_t block_workspace_bytes(const trtm_step_desc* d);
 int64_t block_shot_table_bytes(int n);
+int block_config_query(const trtm_step_desc* d, int32_t* out);
 ShotDev* block_shot_slot(const trtm_step_desc* d);
 int run_block(const trtm_step_desc* descs, int n, bool fwd, bool save, int32_t it_begin,
               int32_t it_end, ShotDev* shots_dev, const KArgs& m, hipStream_t s);
@@ -369,6 +370,12 @@ int64_t trtm_workspace_bytes(const trtm_step_desc* d) {
     if (d->kernel == TRTM_KERNEL_SIMPLE) return 0;
     if (d->kernel == TRTM_KERNEL_FUSED) return fused_workspace_bytes(d);
     return std::max(block_workspace_bytes(d), fused_workspace_bytes(d));
+}
+
+int trtm_block_config(const trtm_step_desc* d, int32_t* out) {
+    if (!d || !out || d->struct_size != (int32_t)sizeof(trtm_step_desc))
+        return fail(TRTM_ERR_ARG, "bad arguments");
+    return block_config_query(d, out);
 }
 
 int64_t trtm_batch_workspace_bytes(int32_t n) {

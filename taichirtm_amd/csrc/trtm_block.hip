@@ -771,6 +771,17 @@ int run_block(const trtm_step_desc* descs, int n, bool fwd, bool save, int32_t i
     return TRTM_OK;
 }
 
+int block_config_query(const trtm_step_desc* d, int32_t* out) {
+    BlockCfg c;
+    const int rc = block_choose(d, &c);
+    if (rc) return rc;
+    out[0] = c.tx;
+    out[1] = c.tz;
+    out[2] = c.nt;
+    out[3] = c.k;
+    return TRTM_OK;
+}
+
 int64_t block_shot_table_bytes(int n) { return (int64_t)(2 * n * sizeof(ShotDev) + 256); }
 
 ShotDev* block_shot_slot(const trtm_step_desc* d) {
