@@ -207,6 +207,75 @@ int trtm_backward_steps_batch(const trtm_step_desc *descs, int32_t n, void *batc
                               int64_t batch_workspace_bytes, int32_t it_begin, int32_t it_end,
                               void *stream);
 
+/* ------------------------------------------------------------------------
+ * FWI gradient: exact discrete adjoint of the forward step.
+ * No reference counterpart (the reference's backward propagator is not the
+ * adjoint of its forward step: it swaps the difference directions and injects
+ * data unscaled, backward_modeling.py:331-345,458-460).  REAL is float
+ * (real_bytes = 4) or double (real_bytes = 8, for gradient checks).
+ * ------------------------------------------------------------------------ */
+
+/* Medium preparation of trtm_prepare_medium in double precision. */
+int trtm_prepare_medium_f64(int32_t nx, int32_t nz, const double *mu, const double *rho,
+                            double *myx, double *myz, double *mxz, double *rho_u, double *rho_w,
+                            double *inv_rho, double *inv_rho_u, double *inv_rho_w, void *stream);
+
+typedef struct trtm_fwi_desc {
+    int32_t struct_size; /* = sizeof(trtm_fwi_desc) */
+    int32_t real_bytes;  /* 4 or 8: element size of every REAL array below */
+    int32_t nx, nz, nt, fw;
+    int32_t systems;     /* TRTM_PSV | TRTM_SH */
+    int32_t n_src, n_src_groups, n_rec;
+    double dt, inv_dx, inv_dz, src_factor; /* narrowed to REAL by the kernels */
+    const void *sponge_table;              /* REAL[fw+1] */
+    const void *lam, *mu, *mxz, *myx, *myz, *inv_rho, *inv_rho_u, *inv_rho_w; /* REAL (nx, nz) */
+    void *u, *v, *w, *sxx, *szz, *sxz, *syx, *syz;          /* forward state */
+    void *pu, *pv, *pw, *psxx, *pszz, *psxz, *psyx, *psyz;  /* adjoint state */
+    const int32_t *src_loc, *src_order, *src_group_start;   /* trtm_plan_injection plan */
+    const void *wav_u, *wav_v, *wav_w;                      /* REAL (n_src, nt) */
+    const int32_t *rec_loc, *rec_order, *rec_group_start;   /* receivers and their plan */
+    int32_t n_rec_groups, hist_base;
+    void *rec_u, *rec_v, *rec_w;       /* forward records, REAL time-major (nt, n_rec) */
+    const void *res_u, *res_v, *res_w; /* data adjoint dJ/d(record), REAL (nt, n_rec) */
+    /* history of steps [hist_base, hist_base + hist_len): for each step the
+     * pre-step stresses and the post-update velocities, REAL
+     * (hist_len, n_hist_fields, nx, nz), n_hist_fields = 5 (P-SV) + 3 (SH) for the
+     * active systems in the order sxx, szz, sxz, u, w, syx, syz, v */
+    int32_t hist_len, reserved;
+    void *hist;
+    /* gradient accumulators in coefficient space, REAL (nx, nz), += */
+    void *g_lam, *g_mu, *g_mxz, *g_myx, *g_myz, *g_ir, *g_iru, *g_irw;
+} trtm_fwi_desc;
+
+/*
+ * Forward steps [it_begin, it_end) of the forward scheme (same expressions as
+ * trtm_forward_steps) in REAL, recording into rec_* and, when hist != NULL,
+ * the history of every step inside [hist_base, hist_base + hist_len).
+ */
+int trtm_fwi_forward(const trtm_fwi_desc *d, int32_t it_begin, int32_t it_end, void *stream);
+
+/*
+ * Adjoint steps n = it_end-1 down to it_begin (history must hold them): add
+ * res_*[n] at the receivers to the adjoint state, apply the transposes of the
+ * sponge, stress update, source term and velocity update of forward step n,
+ * and accumulate the gradient with respect to lam, mu, mxz, myx, myz, inv_rho,
+ * inv_rho_u, inv_rho_w.  Flat free surface only.
+ */
+int trtm_fwi_adjoint(const trtm_fwi_desc *d, int32_t it_begin, int32_t it_end, void *stream);
+
+/*
+ * Coefficient-space gradient to (lam, mu, rho) through the harmonic and
+ * arithmetic averages (forward_modeling.py:243-307), then to (vp, vs, rho)
+ * with mu = rho vs^2, lam = rho (vp^2 - 2 vs^2) (rtm.py:901-902).  Double
+ * precision inputs (nx, nz) and outputs.
+ */
+int trtm_fwi_chain_gradient(int32_t nx, int32_t nz, const double *mu, const double *rho,
+                            const double *vp, const double *vs, const double *g_lam,
+                            const double *g_mu, const double *g_mxz, const double *g_myx,
+                            const double *g_myz, const double *g_ir, const double *g_iru,
+                            const double *g_irw, double *g_vp, double *g_vs, double *g_rho,
+                            void *stream);
+
 /*
  * Standalone finite check of u, v, w over the full grid: NaN or |x| > 1e30
  * gives code base+1 / base+2 / base+3, the highest found is atomically

@@ -50,6 +50,36 @@ class StepDesc(ctypes.Structure):
     ]
 
 
+_f64 = ctypes.c_double
+
+
+class FwiDesc(ctypes.Structure):
+    """Mirror of trtm_fwi_desc."""
+    _fields_ = [
+        ("struct_size", _i32), ("real_bytes", _i32), ("nx", _i32), ("nz", _i32), ("nt", _i32),
+        ("fw", _i32), ("systems", _i32), ("n_src", _i32), ("n_src_groups", _i32),
+        ("n_rec", _i32),
+        ("dt", _f64), ("inv_dx", _f64), ("inv_dz", _f64), ("src_factor", _f64),
+        ("sponge_table", _p),
+        ("lam", _p), ("mu", _p), ("mxz", _p), ("myx", _p), ("myz", _p), ("inv_rho", _p),
+        ("inv_rho_u", _p), ("inv_rho_w", _p),
+        ("u", _p), ("v", _p), ("w", _p), ("sxx", _p), ("szz", _p), ("sxz", _p), ("syx", _p),
+        ("syz", _p),
+        ("pu", _p), ("pv", _p), ("pw", _p), ("psxx", _p), ("pszz", _p), ("psxz", _p),
+        ("psyx", _p), ("psyz", _p),
+        ("src_loc", _p), ("src_order", _p), ("src_group_start", _p),
+        ("wav_u", _p), ("wav_v", _p), ("wav_w", _p),
+        ("rec_loc", _p), ("rec_order", _p), ("rec_group_start", _p),
+        ("n_rec_groups", _i32), ("hist_base", _i32),
+        ("rec_u", _p), ("rec_v", _p), ("rec_w", _p),
+        ("res_u", _p), ("res_v", _p), ("res_w", _p),
+        ("hist_len", _i32), ("reserved", _i32),
+        ("hist", _p),
+        ("g_lam", _p), ("g_mu", _p), ("g_mxz", _p), ("g_myx", _p), ("g_myz", _p), ("g_ir", _p),
+        ("g_iru", _p), ("g_irw", _p),
+    ]
+
+
 # name -> (restype, argtypes) for every entry point of include/taichirtm_hip.h
 SIGNATURES = {
     "trtm_abi_version": (ctypes.c_int, []),
@@ -68,6 +98,10 @@ SIGNATURES = {
     "trtm_forward_steps": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _i32, _i32, _p]),
     "trtm_backward_steps": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _i32, _p]),
     "trtm_check_finite": (ctypes.c_int, [_i32, _i32, _p, _p, _p, _i32, _p, _p]),
+    "trtm_prepare_medium_f64": (ctypes.c_int, [_i32, _i32] + [_p] * 10 + [_p]),
+    "trtm_fwi_forward": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _p]),
+    "trtm_fwi_adjoint": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _p]),
+    "trtm_fwi_chain_gradient": (ctypes.c_int, [_i32, _i32] + [_p] * 15 + [_p]),
     "trtm_transpose": (ctypes.c_int, [_i32, _i32, _p, _p, _p]),
 }
 

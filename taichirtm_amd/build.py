@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtaichirtm_hip.so")
-SOURCES = ["trtm_abi.hip", "trtm_simple.hip", "trtm_fused.hip", "trtm_block.hip"]
+SOURCES = ["trtm_abi.hip", "trtm_simple.hip", "trtm_fused.hip", "trtm_block.hip",
+           "trtm_adjoint.hip"]
 ARCH = os.environ.get("TRTM_OFFLOAD_ARCH", "gfx950")
 
 

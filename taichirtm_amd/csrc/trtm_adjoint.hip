@@ -1,0 +1,511 @@
+// Exact discrete adjoint of the forward step, for FWI gradients.
+//
+// Forward step n (forward_modeling.py:539-551), as linear maps of the state x
+// for a fixed medium m:  V (velocity update, interior), S_n (source term),
+// S (stress update, interior), P (sponge), then the record y_n = R x_{n+1}.
+// With J = sum_n j(y_n) and p = dJ/dx_{n+1}, one adjoint step is
+//     p += R^T dj/dy_n;  p = P^T p;  p += S^T p  (gradients of lam, mu, mxz,
+//     myx, myz);  source gradient (inv_rho_*);  p += V^T p  (gradients of
+//     inv_rho, inv_rho_u, inv_rho_w)
+// after which p = dJ/dx_n.  The coefficient gradients need the stresses the
+// velocity update read and the velocities the stress update read; the forward
+// pass records them as a per-step history (recomputed per checkpoint segment).
+//
+// The transposes are written in gather form: every output cell sums the
+// contributions of its own and neighbouring cells, with the interior masks of
+// the forward updates, so each kernel writes only its own cell.
+#include <algorithm>
+#include <stdarg.h>
+
+#include "trtm_real.hpp"
+
+namespace trtm {
+
+int fail(int code, const char* fmt, ...);
+
+template <class R>
+struct AA {
+    int nx, nz, nt, fw, n_rec, n_rec_groups, n_src_groups;
+    R dt, idx, idz, factor;
+    const R* tab;
+    const R *lam, *mu, *mxz, *myx, *myz, *ir, *iru, *irw;
+    R *pu, *pv, *pw, *psxx, *pszz, *psxz, *psyx, *psyz;
+    const int *rec_loc, *rec_order, *rec_start;
+    const R *res_u, *res_v, *res_w;
+    const int *src_loc, *src_order, *src_start;
+    const R *wav_u, *wav_v, *wav_w;
+    R *g_lam, *g_mu, *g_mxz, *g_myx, *g_myz, *g_ir, *g_iru, *g_irw;
+    // history plane offsets (in cells) of sxx, szz, sxz, u, w, syx, syz, v
+    long long h_sxx, h_szz, h_sxz, h_u, h_w, h_syx, h_syz, h_v;
+};
+
+// data adjoint into the adjoint state at the receivers, one thread per cell
+template <class R, bool PSV, bool SH>
+__global__ void k_adj_inject(AA<R> a, int n) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.n_rec_groups) return;
+    const int e0 = a.rec_start[g], e1 = a.rec_start[g + 1];
+    const int k0 = a.rec_order[e0];
+    const size_t c = (size_t)a.rec_loc[2 * k0] * a.nz + a.rec_loc[2 * k0 + 1];
+    const size_t row = (size_t)n * a.n_rec;
+    if (PSV) {
+        R x = a.pu[c], z = a.pw[c];
+        for (int e = e0; e < e1; e++) {
+            const int l = a.rec_order[e];
+            x += a.res_u[row + l];
+            z += a.res_w[row + l];
+        }
+        a.pu[c] = x;
+        a.pw[c] = z;
+    }
+    if (SH) {
+        R y = a.pv[c];
+        for (int e = e0; e < e1; e++) y += a.res_v[row + a.rec_order[e]];
+        a.pv[c] = y;
+    }
+}
+
+// sponge transpose + stress-update transpose: new velocity adjoint, and the
+// gradients of lam, mu, mxz (P-SV) / myx, myz (SH)
+template <class R, bool PSV, bool SH>
+__global__ __launch_bounds__(256) void k_adj_stress(AA<R> a, const R* __restrict__ h) {
+    const int j = blockIdx.x * kBlockZ + threadIdx.x;
+    const int i = blockIdx.y * kBlockX + threadIdx.y;
+    const int nx = a.nx, nz = a.nz;
+    if (i >= nx || j >= nz) return;
+    const size_t c = (size_t)i * nz + j;
+    const R A = sponge_at_t<R>(i, j, nx, nz, a.fw, a.tab);
+    auto inner = [&](int ii, int jj) { return ii >= 1 && ii < nx - 1 && jj >= 1 && jj < nz - 1; };
+    auto sp = [&](const R* f, int ii, int jj) {
+        return sponge_at_t<R>(ii, jj, nx, nz, a.fw, a.tab) * f[(size_t)ii * nz + jj];
+    };
+    const R dt = a.dt;
+    if (PSV) {
+        auto qx = [&](int ii, int jj) -> R {
+            if (!inner(ii, jj)) return R(0);
+            const size_t k = (size_t)ii * nz + jj;
+            const R axx = dt * sp(a.psxx, ii, jj), azz = dt * sp(a.pszz, ii, jj);
+            return (a.lam[k] + R(2) * a.mu[k]) * axx + a.lam[k] * azz;
+        };
+        auto qz = [&](int ii, int jj) -> R {
+            if (!inner(ii, jj)) return R(0);
+            const size_t k = (size_t)ii * nz + jj;
+            const R axx = dt * sp(a.psxx, ii, jj), azz = dt * sp(a.pszz, ii, jj);
+            return a.lam[k] * axx + (a.lam[k] + R(2) * a.mu[k]) * azz;
+        };
+        auto ss = [&](int ii, int jj) -> R {
+            if (!inner(ii, jj)) return R(0);
+            return a.mxz[(size_t)ii * nz + jj] * dt * sp(a.psxz, ii, jj);
+        };
+        const R pu = A * a.pu[c] + a.idx * (qx(i - 1, j) - qx(i, j)) + a.idz * (ss(i, j - 1) - ss(i, j));
+        const R pw = A * a.pw[c] + a.idx * (ss(i - 1, j) - ss(i, j)) + a.idz * (qz(i, j - 1) - qz(i, j));
+        a.pu[c] = pu;
+        a.pw[c] = pw;
+        if (inner(i, j)) {
+            const R axx = dt * A * a.psxx[c], azz = dt * A * a.pszz[c], axz = dt * A * a.psxz[c];
+            const R* hu = h + a.h_u;
+            const R* hw = h + a.h_w;
+            const R ux = (hu[c + nz] - hu[c]) * a.idx, uz = (hu[c + 1] - hu[c]) * a.idz;
+            const R wx = (hw[c + nz] - hw[c]) * a.idx, wz = (hw[c + 1] - hw[c]) * a.idz;
+            a.g_lam[c] += (axx + azz) * (ux + wz);
+            a.g_mu[c] += R(2) * (axx * ux + azz * wz);
+            a.g_mxz[c] += axz * (uz + wx);
+        }
+    }
+    if (SH) {
+        auto qy = [&](int ii, int jj) -> R {
+            if (!inner(ii, jj)) return R(0);
+            return a.myx[(size_t)ii * nz + jj] * dt * sp(a.psyx, ii, jj);
+        };
+        auto qv = [&](int ii, int jj) -> R {
+            if (!inner(ii, jj)) return R(0);
+            return a.myz[(size_t)ii * nz + jj] * dt * sp(a.psyz, ii, jj);
+        };
+        a.pv[c] = A * a.pv[c] + a.idx * (qy(i - 1, j) - qy(i, j)) + a.idz * (qv(i, j - 1) - qv(i, j));
+        if (inner(i, j)) {
+            const R* hv = h + a.h_v;
+            const R vx = (hv[c + nz] - hv[c]) * a.idx, vz = (hv[c + 1] - hv[c]) * a.idz;
+            a.g_myx[c] += dt * A * a.psyx[c] * vx;
+            a.g_myz[c] += dt * A * a.psyz[c] * vz;
+        }
+    }
+}
+
+// source-term gradient: u += (wavelet * factor) * inv_rho_u at the source cell
+template <class R, bool PSV, bool SH>
+__global__ void k_adj_source(AA<R> a, int n) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.n_src_groups) return;
+    const int e0 = a.src_start[g], e1 = a.src_start[g + 1];
+    const int k0 = a.src_order[e0];
+    const size_t c = (size_t)a.src_loc[2 * k0] * a.nz + a.src_loc[2 * k0 + 1];
+    for (int e = e0; e < e1; e++) {
+        const size_t o = (size_t)a.src_order[e] * a.nt + n;
+        if (PSV) {
+            a.g_iru[c] += a.pu[c] * a.wav_u[o] * a.factor;
+            a.g_irw[c] += a.pw[c] * a.wav_w[o] * a.factor;
+        }
+        if (SH) a.g_ir[c] += a.pv[c] * a.wav_v[o] * a.factor;
+    }
+}
+
+// sponge transpose on the stresses + velocity-update transpose: new stress
+// adjoint, and the gradients of inv_rho_u, inv_rho_w (P-SV) / inv_rho (SH)
+template <class R, bool PSV, bool SH>
+__global__ __launch_bounds__(256) void k_adj_velocity(AA<R> a, const R* __restrict__ h) {
+    const int j = blockIdx.x * kBlockZ + threadIdx.x;
+    const int i = blockIdx.y * kBlockX + threadIdx.y;
+    const int nx = a.nx, nz = a.nz;
+    if (i >= nx || j >= nz) return;
+    const size_t c = (size_t)i * nz + j;
+    const R A = sponge_at_t<R>(i, j, nx, nz, a.fw, a.tab);
+    auto inner = [&](int ii, int jj) { return ii >= 1 && ii < nx - 1 && jj >= 1 && jj < nz - 1; };
+    const R dt = a.dt;
+    if (PSV) {
+        auto bu = [&](int ii, int jj) -> R {
+            if (!inner(ii, jj)) return R(0);
+            const size_t k = (size_t)ii * nz + jj;
+            return dt * a.iru[k] * a.pu[k];
+        };
+        auto bw = [&](int ii, int jj) -> R {
+            if (!inner(ii, jj)) return R(0);
+            const size_t k = (size_t)ii * nz + jj;
+            return dt * a.irw[k] * a.pw[k];
+        };
+        const R bu0 = bu(i, j), bw0 = bw(i, j);
+        a.psxx[c] = A * a.psxx[c] + a.idx * (bu0 - bu(i + 1, j));
+        a.psxz[c] = A * a.psxz[c] + a.idz * (bu0 - bu(i, j + 1)) + a.idx * (bw0 - bw(i + 1, j));
+        a.pszz[c] = A * a.pszz[c] + a.idz * (bw0 - bw(i, j + 1));
+        if (inner(i, j)) {
+            const R* hxx = h + a.h_sxx;
+            const R* hzz = h + a.h_szz;
+            const R* hxz = h + a.h_sxz;
+            const R sxx_x = (hxx[c] - hxx[c - nz]) * a.idx, sxz_z = (hxz[c] - hxz[c - 1]) * a.idz;
+            const R sxz_x = (hxz[c] - hxz[c - nz]) * a.idx, szz_z = (hzz[c] - hzz[c - 1]) * a.idz;
+            a.g_iru[c] += a.pu[c] * dt * (sxx_x + sxz_z);
+            a.g_irw[c] += a.pw[c] * dt * (sxz_x + szz_z);
+        }
+    }
+    if (SH) {
+        auto bv = [&](int ii, int jj) -> R {
+            if (!inner(ii, jj)) return R(0);
+            const size_t k = (size_t)ii * nz + jj;
+            return dt * a.ir[k] * a.pv[k];
+        };
+        const R bv0 = bv(i, j);
+        a.psyx[c] = A * a.psyx[c] + a.idx * (bv0 - bv(i + 1, j));
+        a.psyz[c] = A * a.psyz[c] + a.idz * (bv0 - bv(i, j + 1));
+        if (inner(i, j)) {
+            const R* hyx = h + a.h_syx;
+            const R* hyz = h + a.h_syz;
+            const R syx_x = (hyx[c] - hyx[c - nz]) * a.idx, syz_z = (hyz[c] - hyz[c - 1]) * a.idz;
+            a.g_ir[c] += a.pv[c] * dt * (syx_x + syz_z);
+        }
+    }
+}
+
+template <class R, bool PSV, bool SH>
+static hipError_t adjoint_step(const AA<R>& a, const R* h, int n, hipStream_t s) {
+    const dim3 block(kBlockZ, kBlockX);
+    const dim3 grid((a.nz + kBlockZ - 1) / kBlockZ, (a.nx + kBlockX - 1) / kBlockX);
+    if (a.n_rec_groups > 0)
+        hipLaunchKernelGGL((k_adj_inject<R, PSV, SH>), dim3((a.n_rec_groups + 255) / 256),
+                           dim3(256), 0, s, a, n);
+    hipLaunchKernelGGL((k_adj_stress<R, PSV, SH>), grid, block, 0, s, a, h);
+    if (a.n_src_groups > 0)
+        hipLaunchKernelGGL((k_adj_source<R, PSV, SH>), dim3((a.n_src_groups + 255) / 256),
+                           dim3(256), 0, s, a, n);
+    hipLaunchKernelGGL((k_adj_velocity<R, PSV, SH>), grid, block, 0, s, a, h);
+    return hipGetLastError();
+}
+
+// coefficient space -> (lam, mu, rho) -> (vp, vs, rho), double precision
+__global__ __launch_bounds__(256) void k_chain(int nx, int nz, const double* mu, const double* rho,
+                                               const double* vp, const double* vs,
+                                               const double* g_lam, const double* g_mu,
+                                               const double* g_mxz, const double* g_myx,
+                                               const double* g_myz, const double* g_ir,
+                                               const double* g_iru, const double* g_irw,
+                                               double* o_vp, double* o_vs, double* o_rho) {
+    const int j = blockIdx.x * kBlockZ + threadIdx.x;
+    const int i = blockIdx.y * kBlockX + threadIdx.y;
+    if (i >= nx || j >= nz) return;
+    auto at = [&](const double* f, int ii, int jj) { return f[(size_t)ii * nz + jj]; };
+    auto inner = [&](int ii, int jj) { return ii >= 1 && ii < nx - 1 && jj >= 1 && jj < nz - 1; };
+    const double m0 = at(mu, i, j), r0 = at(rho, i, j);
+    double gmu = at(g_mu, i, j);
+    // mxz(i', j') = 4 / sum 1/mu over the 2x2 block at (i', j'): d/dmu = mxz^2 / (4 mu^2)
+    for (int di = -1; di <= 0; di++)
+        for (int dj = -1; dj <= 0; dj++) {
+            const int ii = i + di, jj = j + dj;
+            if (!inner(ii, jj)) continue;
+            const double s = 1.0 / at(mu, ii, jj) + 1.0 / at(mu, ii + 1, jj) +
+                             1.0 / at(mu, ii, jj + 1) + 1.0 / at(mu, ii + 1, jj + 1);
+            const double H = 4.0 / s;
+            gmu += at(g_mxz, ii, jj) * H * H / (4.0 * m0 * m0);
+        }
+    // myx(i', j) = 2 / (1/mu(i') + 1/mu(i'+1)), myz(i, j') likewise in z: d/dmu = H^2 / (2 mu^2)
+    for (int di = -1; di <= 0; di++) {
+        const int ii = i + di;
+        if (!inner(ii, j)) continue;
+        const double H = 2.0 / (1.0 / at(mu, ii, j) + 1.0 / at(mu, ii + 1, j));
+        gmu += at(g_myx, ii, j) * H * H / (2.0 * m0 * m0);
+    }
+    for (int dj = -1; dj <= 0; dj++) {
+        const int jj = j + dj;
+        if (!inner(i, jj)) continue;
+        const double H = 2.0 / (1.0 / at(mu, i, jj) + 1.0 / at(mu, i, jj + 1));
+        gmu += at(g_myz, i, jj) * H * H / (2.0 * m0 * m0);
+    }
+    // inv_rho = 1/rho; inv_rho_u(i', j) = 2 / (rho(i') + rho(i'+1)); inv_rho_w likewise in z
+    double grho = -at(g_ir, i, j) / (r0 * r0);
+    for (int di = -1; di <= 0; di++) {
+        const int ii = i + di;
+        if (!inner(ii, j)) continue;
+        const double q = 2.0 / (at(rho, ii, j) + at(rho, ii + 1, j));
+        grho += -0.5 * at(g_iru, ii, j) * q * q;
+    }
+    for (int dj = -1; dj <= 0; dj++) {
+        const int jj = j + dj;
+        if (!inner(i, jj)) continue;
+        const double q = 2.0 / (at(rho, i, jj) + at(rho, i, jj + 1));
+        grho += -0.5 * at(g_irw, i, jj) * q * q;
+    }
+    // mu = rho vs^2, lam = rho (vp^2 - 2 vs^2)
+    const double p = at(vp, i, j), sv = at(vs, i, j), gl = at(g_lam, i, j);
+    const size_t c = (size_t)i * nz + j;
+    o_vp[c] = gl * 2.0 * r0 * p;
+    o_vs[c] = gmu * 2.0 * r0 * sv - gl * 4.0 * r0 * sv;
+    o_rho[c] = grho + gmu * sv * sv + gl * (p * p - 2.0 * sv * sv);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_prepare_t(int nx, int nz, const R* __restrict__ mu,
+                                                   const R* __restrict__ rho, R* myx, R* myz,
+                                                   R* mxz, R* rho_u, R* rho_w, R* inv_rho,
+                                                   R* inv_rho_u, R* inv_rho_w) {
+    const int j = blockIdx.x * kBlockZ + threadIdx.x;
+    const int i = blockIdx.y * kBlockX + threadIdx.y;
+    if (i >= nx || j >= nz) return;
+    const size_t c = (size_t)i * nz + j;
+    R hx = 0, hz = 0, hxz = 0, ru = 0, rw = 0;
+    if (i >= 1 && i < nx - 1 && j >= 1 && j < nz - 1) {
+        const R a = mu[c], b = mu[c + nz], cc = mu[c + 1], d = mu[c + nz + 1];
+        hx = R(2) * a * b / (a + b);
+        hz = R(2) * a * cc / (a + cc);
+        const R product = a * b * cc * d;
+        const R sum_inv_prod = b * cc * d + a * cc * d + a * b * d + a * b * cc;
+        hxz = R(4) * product / sum_inv_prod;
+        const R r = rho[c];
+        ru = R(0.5) * (r + rho[c + nz]);
+        rw = R(0.5) * (r + rho[c + 1]);
+    }
+    myx[c] = hx;
+    myz[c] = hz;
+    mxz[c] = hxz;
+    rho_u[c] = ru;
+    rho_w[c] = rw;
+    inv_rho[c] = R(1) / rho[c];
+    inv_rho_u[c] = R(1) / ru;
+    inv_rho_w[c] = R(1) / rw;
+}
+
+// ---------------------------------------------------------------------------
+
+static int check_fwi(const trtm_fwi_desc* d, int it_begin, int it_end) {
+    if (!d) return fail(TRTM_ERR_ARG, "descriptor is NULL");
+    if (d->struct_size != (int32_t)sizeof(trtm_fwi_desc))
+        return fail(TRTM_ERR_ARG, "struct_size %d != %d (ABI mismatch)", d->struct_size,
+                    (int)sizeof(trtm_fwi_desc));
+    if (d->real_bytes != 4 && d->real_bytes != 8) return fail(TRTM_ERR_ARG, "real_bytes must be 4 or 8");
+    if (d->nx < 2 || d->nz < 2 || d->nt < 1 || d->fw < 0 || d->fw > d->nx || d->fw > d->nz)
+        return fail(TRTM_ERR_ARG, "bad grid");
+    if (it_begin < 0 || it_end > d->nt || it_begin > it_end)
+        return fail(TRTM_ERR_ARG, "step range [%d, %d) outside [0, %d]", it_begin, it_end, d->nt);
+    if (d->systems <= 0 || (d->systems & ~(TRTM_PSV | TRTM_SH)))
+        return fail(TRTM_ERR_ARG, "bad systems %d", d->systems);
+    if (!d->sponge_table || !d->lam || !d->mu || !d->mxz || !d->inv_rho_u || !d->inv_rho_w ||
+        !d->myx || !d->myz || !d->inv_rho)
+        return fail(TRTM_ERR_ARG, "medium pointer is NULL");
+    return TRTM_OK;
+}
+
+static int hist_fields(int systems) {
+    return ((systems & TRTM_PSV) ? 5 : 0) + ((systems & TRTM_SH) ? 3 : 0);
+}
+
+template <class R>
+static KA<R> fwi_kargs(const trtm_fwi_desc* d) {
+    KA<R> a;
+    a.nx = d->nx; a.nz = d->nz; a.nt = d->nt; a.fw = d->fw;
+    a.dt = (R)d->dt; a.idx = (R)d->inv_dx; a.idz = (R)d->inv_dz; a.factor = (R)d->src_factor;
+    a.tab = (const R*)d->sponge_table;
+    a.lam = (const R*)d->lam; a.mu = (const R*)d->mu; a.mxz = (const R*)d->mxz;
+    a.myx = (const R*)d->myx; a.myz = (const R*)d->myz; a.ir = (const R*)d->inv_rho;
+    a.iru = (const R*)d->inv_rho_u; a.irw = (const R*)d->inv_rho_w;
+    a.sm = nullptr;
+    a.u = (R*)d->u; a.v = (R*)d->v; a.w = (R*)d->w;
+    a.sxx = (R*)d->sxx; a.szz = (R*)d->szz; a.sxz = (R*)d->sxz;
+    a.syx = (R*)d->syx; a.syz = (R*)d->syz;
+    return a;
+}
+
+template <class R>
+static int fwi_forward_t(const trtm_fwi_desc* d, int it_begin, int it_end, hipStream_t s) {
+    KA<R> a = fwi_kargs<R>(d);
+    IA<R> q;
+    q.n_groups = d->n_src_groups; q.loc = d->src_loc; q.order = d->src_order;
+    q.start = d->src_group_start;
+    q.su = (const R*)d->wav_u; q.sv = (const R*)d->wav_v; q.sw = (const R*)d->wav_w;
+    SIO<R> p;
+    p.snap_slot = p.image_slot = p.check_slot = -1;
+    p.check_base = 0;
+    p.snap_u = p.snap_v = p.snap_w = p.image_u = p.image_v = p.image_w = nullptr;
+    p.check_flags = nullptr;
+    p.n_rec = (d->rec_u || d->rec_v) ? d->n_rec : 0;
+    p.rec_loc = d->rec_loc;
+    p.rec_u = (R*)d->rec_u; p.rec_v = (R*)d->rec_v; p.rec_w = (R*)d->rec_w;
+    const bool psv = d->systems & TRTM_PSV, sh = d->systems & TRTM_SH;
+    const size_t n = (size_t)d->nx * d->nz;
+    const int nf = hist_fields(d->systems);
+    for (int it = it_begin; it < it_end; it++) {
+        q.col = it;
+        p.rec_row = it;
+        R* hs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        R* hv[3] = {nullptr, nullptr, nullptr};
+        const int slot = it - d->hist_base;
+        if (d->hist && slot >= 0 && slot < d->hist_len) {
+            R* base = (R*)d->hist + (size_t)slot * nf * n;
+            int f = 0;
+            if (psv) {
+                hs[0] = base + (f++) * n;  // sxx
+                hs[1] = base + (f++) * n;  // szz
+                hs[2] = base + (f++) * n;  // sxz
+                hv[0] = base + (f++) * n;  // u
+                hv[1] = base + (f++) * n;  // w
+            }
+            if (sh) {
+                hs[3] = base + (f++) * n;  // syx
+                hs[4] = base + (f++) * n;  // syz
+                hv[2] = base + (f++) * n;  // v
+            }
+        }
+        hipError_t e = launch_hist_step<R>(d->systems, a, q, p, hs, hv, s);
+        if (e != hipSuccess) return fail(TRTM_ERR_HIP, "fwi forward step %d: %s", it, hipGetErrorString(e));
+    }
+    return TRTM_OK;
+}
+
+template <class R>
+static int fwi_adjoint_t(const trtm_fwi_desc* d, int it_begin, int it_end, hipStream_t s) {
+    AA<R> a;
+    a.nx = d->nx; a.nz = d->nz; a.nt = d->nt; a.fw = d->fw;
+    a.n_rec = d->n_rec; a.n_rec_groups = d->n_rec_groups; a.n_src_groups = d->n_src_groups;
+    a.dt = (R)d->dt; a.idx = (R)d->inv_dx; a.idz = (R)d->inv_dz; a.factor = (R)d->src_factor;
+    a.tab = (const R*)d->sponge_table;
+    a.lam = (const R*)d->lam; a.mu = (const R*)d->mu; a.mxz = (const R*)d->mxz;
+    a.myx = (const R*)d->myx; a.myz = (const R*)d->myz; a.ir = (const R*)d->inv_rho;
+    a.iru = (const R*)d->inv_rho_u; a.irw = (const R*)d->inv_rho_w;
+    a.pu = (R*)d->pu; a.pv = (R*)d->pv; a.pw = (R*)d->pw;
+    a.psxx = (R*)d->psxx; a.pszz = (R*)d->pszz; a.psxz = (R*)d->psxz;
+    a.psyx = (R*)d->psyx; a.psyz = (R*)d->psyz;
+    a.rec_loc = d->rec_loc; a.rec_order = d->rec_order; a.rec_start = d->rec_group_start;
+    a.res_u = (const R*)d->res_u; a.res_v = (const R*)d->res_v; a.res_w = (const R*)d->res_w;
+    a.src_loc = d->src_loc; a.src_order = d->src_order; a.src_start = d->src_group_start;
+    a.wav_u = (const R*)d->wav_u; a.wav_v = (const R*)d->wav_v; a.wav_w = (const R*)d->wav_w;
+    a.g_lam = (R*)d->g_lam; a.g_mu = (R*)d->g_mu; a.g_mxz = (R*)d->g_mxz;
+    a.g_myx = (R*)d->g_myx; a.g_myz = (R*)d->g_myz; a.g_ir = (R*)d->g_ir;
+    a.g_iru = (R*)d->g_iru; a.g_irw = (R*)d->g_irw;
+    const bool psv = d->systems & TRTM_PSV, sh = d->systems & TRTM_SH;
+    const long long n = (long long)d->nx * d->nz;
+    int f = 0;
+    a.h_sxx = a.h_szz = a.h_sxz = a.h_u = a.h_w = a.h_syx = a.h_syz = a.h_v = 0;
+    if (psv) {
+        a.h_sxx = (f++) * n; a.h_szz = (f++) * n; a.h_sxz = (f++) * n;
+        a.h_u = (f++) * n; a.h_w = (f++) * n;
+    }
+    if (sh) {
+        a.h_syx = (f++) * n; a.h_syz = (f++) * n; a.h_v = (f++) * n;
+    }
+    const int nf = f;
+    for (int it = it_end - 1; it >= it_begin; it--) {
+        const int slot = it - d->hist_base;
+        if (slot < 0 || slot >= d->hist_len)
+            return fail(TRTM_ERR_ARG, "adjoint step %d has no history (base %d, length %d)", it,
+                        d->hist_base, d->hist_len);
+        const R* h = (const R*)d->hist + (size_t)slot * nf * n;
+        hipError_t e;
+        if (psv && sh) e = adjoint_step<R, true, true>(a, h, it, s);
+        else if (psv) e = adjoint_step<R, true, false>(a, h, it, s);
+        else e = adjoint_step<R, false, true>(a, h, it, s);
+        if (e != hipSuccess) return fail(TRTM_ERR_HIP, "adjoint step %d: %s", it, hipGetErrorString(e));
+    }
+    return TRTM_OK;
+}
+
+}  // namespace trtm
+
+using namespace trtm;
+
+extern "C" {
+
+int trtm_prepare_medium_f64(int32_t nx, int32_t nz, const double* mu, const double* rho,
+                            double* myx, double* myz, double* mxz, double* rho_u, double* rho_w,
+                            double* inv_rho, double* inv_rho_u, double* inv_rho_w, void* stream) {
+    if (nx < 1 || nz < 1 || !mu || !rho || !myx || !myz || !mxz || !rho_u || !rho_w || !inv_rho ||
+        !inv_rho_u || !inv_rho_w)
+        return fail(TRTM_ERR_ARG, "bad arguments");
+    hipLaunchKernelGGL(k_prepare_t<double>,
+                       dim3((nz + kBlockZ - 1) / kBlockZ, (nx + kBlockX - 1) / kBlockX),
+                       dim3(kBlockZ, kBlockX), 0, (hipStream_t)stream, nx, nz, mu, rho, myx, myz,
+                       mxz, rho_u, rho_w, inv_rho, inv_rho_u, inv_rho_w);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TRTM_OK : fail(TRTM_ERR_HIP, "prepare f64: %s", hipGetErrorString(e));
+}
+
+int trtm_fwi_forward(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, void* stream) {
+    int rc = check_fwi(d, it_begin, it_end);
+    if (rc) return rc;
+    if ((d->systems & TRTM_PSV) && (!d->u || !d->w || !d->sxx || !d->szz || !d->sxz))
+        return fail(TRTM_ERR_ARG, "P-SV state pointer is NULL");
+    if ((d->systems & TRTM_SH) && (!d->v || !d->syx || !d->syz))
+        return fail(TRTM_ERR_ARG, "SH state pointer is NULL");
+    if (d->real_bytes == 4) return fwi_forward_t<float>(d, it_begin, it_end, (hipStream_t)stream);
+    return fwi_forward_t<double>(d, it_begin, it_end, (hipStream_t)stream);
+}
+
+int trtm_fwi_adjoint(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, void* stream) {
+    int rc = check_fwi(d, it_begin, it_end);
+    if (rc) return rc;
+    if (!d->hist) return fail(TRTM_ERR_ARG, "adjoint needs the forward history");
+    if ((d->systems & TRTM_PSV) && (!d->pu || !d->pw || !d->psxx || !d->pszz || !d->psxz ||
+                                    !d->g_lam || !d->g_mu || !d->g_mxz || !d->g_iru || !d->g_irw))
+        return fail(TRTM_ERR_ARG, "P-SV adjoint pointer is NULL");
+    if ((d->systems & TRTM_SH) && (!d->pv || !d->psyx || !d->psyz || !d->g_myx || !d->g_myz ||
+                                   !d->g_ir))
+        return fail(TRTM_ERR_ARG, "SH adjoint pointer is NULL");
+    if (d->n_rec_groups > 0 && (!d->rec_order || !d->rec_group_start ||
+                                ((d->systems & TRTM_PSV) && (!d->res_u || !d->res_w)) ||
+                                ((d->systems & TRTM_SH) && !d->res_v)))
+        return fail(TRTM_ERR_ARG, "receiver plan or data adjoint is NULL");
+    if (d->real_bytes == 4) return fwi_adjoint_t<float>(d, it_begin, it_end, (hipStream_t)stream);
+    return fwi_adjoint_t<double>(d, it_begin, it_end, (hipStream_t)stream);
+}
+
+int trtm_fwi_chain_gradient(int32_t nx, int32_t nz, const double* mu, const double* rho,
+                            const double* vp, const double* vs, const double* g_lam,
+                            const double* g_mu, const double* g_mxz, const double* g_myx,
+                            const double* g_myz, const double* g_ir, const double* g_iru,
+                            const double* g_irw, double* g_vp, double* g_vs, double* g_rho,
+                            void* stream) {
+    if (nx < 2 || nz < 2 || !mu || !rho || !vp || !vs || !g_lam || !g_mu || !g_mxz || !g_myx ||
+        !g_myz || !g_ir || !g_iru || !g_irw || !g_vp || !g_vs || !g_rho)
+        return fail(TRTM_ERR_ARG, "bad arguments");
+    hipLaunchKernelGGL(k_chain, dim3((nz + kBlockZ - 1) / kBlockZ, (nx + kBlockX - 1) / kBlockX),
+                       dim3(kBlockZ, kBlockX), 0, (hipStream_t)stream, nx, nz, mu, rho, vp, vs,
+                       g_lam, g_mu, g_mxz, g_myx, g_myz, g_ir, g_iru, g_irw, g_vp, g_vs, g_rho);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TRTM_OK : fail(TRTM_ERR_HIP, "chain: %s", hipGetErrorString(e));
+}
+
+}  // extern "C"

@@ -17,17 +17,18 @@
 // The velocity sponge cannot join k_stress: the stress update reads the
 // neighbours' velocities, which must still be the unsponged values.
 // This path is the plain baseline the fused kernel is checked against.
-#include "trtm_common.hpp"
+#include "trtm_real.hpp"
 
 namespace trtm {
 
-template <bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_surface(KArgs a) {
+
+template <class R, bool PSV, bool SH>
+__global__ __launch_bounds__(256) void k_surface(KA<R> a) {
     const int j = blockIdx.x * kBlockZ + threadIdx.x;
     const int i = blockIdx.y * kBlockX + threadIdx.y;
     if (i >= a.nx || j >= a.nz) return;
     const size_t c = (size_t)i * a.nz + j;
-    const float s = a.sm[c];
+    const R s = a.sm[c];
     if (SH) a.syz[c] = a.syz[c] * s;
     if (PSV) {
         a.sxz[c] = a.sxz[c] * s;
@@ -35,15 +36,15 @@ __global__ __launch_bounds__(256) void k_surface(KArgs a) {
     }
 }
 
-template <bool FWD, bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_velocity(KArgs a) {
+template <class R, bool FWD, bool PSV, bool SH>
+__global__ __launch_bounds__(256) void k_velocity(KA<R> a) {
     const int j = blockIdx.x * kBlockZ + threadIdx.x;
     const int i = blockIdx.y * kBlockX + threadIdx.y;
     if (i < 1 || i >= a.nx - 1 || j < 1 || j >= a.nz - 1) return;
     const size_t nz = (size_t)a.nz;
     const size_t c = (size_t)i * nz + j;
     if (PSV) {
-        float sxx_x, szz_z, sxz_x, sxz_z;
+        R sxx_x, szz_z, sxz_x, sxz_z;
         if (FWD) {
             sxx_x = (a.sxx[c] - a.sxx[c - nz]) * a.idx;
             szz_z = (a.szz[c] - a.szz[c - 1]) * a.idz;
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(256) void k_velocity(KArgs a) {
         }
     }
     if (SH) {
-        float syx_x, syz_z;
+        R syx_x, syz_z;
         if (FWD) {
             syx_x = (a.syx[c] - a.syx[c - nz]) * a.idx;
             syz_z = (a.syz[c] - a.syz[c - 1]) * a.idz;
@@ -75,8 +76,8 @@ __global__ __launch_bounds__(256) void k_velocity(KArgs a) {
 }
 
 // One thread per distinct cell; entries of a cell are added in entry order.
-template <bool FWD, bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_inject(KArgs a, InjArgs q) {
+template <class R, bool FWD, bool PSV, bool SH>
+__global__ __launch_bounds__(256) void k_inject(KA<R> a, IA<R> q) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= q.n_groups) return;
     const int e0 = q.start[g], e1 = q.start[g + 1];
@@ -84,7 +85,7 @@ __global__ __launch_bounds__(256) void k_inject(KArgs a, InjArgs q) {
     const size_t c = (size_t)q.loc[2 * k0] * a.nz + q.loc[2 * k0 + 1];
     const size_t nt = (size_t)a.nt;
     if (PSV) {
-        float x = a.u[c], z = a.w[c];
+        R x = a.u[c], z = a.w[c];
         for (int e = e0; e < e1; e++) {
             const size_t o = (size_t)q.order[e] * nt + q.col;
             if (FWD) {
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(256) void k_inject(KArgs a, InjArgs q) {
         a.w[c] = z;
     }
     if (SH) {
-        float y = a.v[c];
+        R y = a.v[c];
         for (int e = e0; e < e1; e++) {
             const size_t o = (size_t)q.order[e] * nt + q.col;
             y = FWD ? y + (q.sv[o] * a.factor) * a.ir[c] : y + q.sv[o];
@@ -108,20 +109,20 @@ __global__ __launch_bounds__(256) void k_inject(KArgs a, InjArgs q) {
     }
 }
 
-template <bool FWD, bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_stress(KArgs a) {
+template <class R, bool FWD, bool PSV, bool SH>
+__global__ __launch_bounds__(256) void k_stress(KA<R> a) {
     const int j = blockIdx.x * kBlockZ + threadIdx.x;
     const int i = blockIdx.y * kBlockX + threadIdx.y;
     if (i >= a.nx || j >= a.nz) return;
     const size_t nz = (size_t)a.nz;
     const size_t c = (size_t)i * nz + j;
-    const float A = sponge_at(i, j, a.nx, a.nz, a.fw, a.tab);
+    const R A = sponge_at_t<R>(i, j, a.nx, a.nz, a.fw, a.tab);
     const bool inner = i >= 1 && i < a.nx - 1 && j >= 1 && j < a.nz - 1;
-    const float sdt = FWD ? a.dt : -a.dt;
+    const R sdt = FWD ? a.dt : -a.dt;
     if (PSV) {
-        float sxx = a.sxx[c], szz = a.szz[c], sxz = a.sxz[c];
+        R sxx = a.sxx[c], szz = a.szz[c], sxz = a.sxz[c];
         if (inner) {
-            float u_x, u_z, w_x, w_z;
+            R u_x, u_z, w_x, w_z;
             if (FWD) {
                 u_x = (a.u[c + nz] - a.u[c]) * a.idx;
                 u_z = (a.u[c + 1] - a.u[c]) * a.idz;
@@ -133,10 +134,10 @@ __global__ __launch_bounds__(256) void k_stress(KArgs a) {
                 w_x = (a.w[c] - a.w[c - nz]) * a.idx;
                 w_z = (a.w[c] - a.w[c - 1]) * a.idz;
             }
-            const float div_uw = u_x + w_z;
-            const float lam = a.lam[c], mu = a.mu[c];
-            sxx = sxx + sdt * (lam * div_uw + (2.0f * mu) * u_x);
-            szz = szz + sdt * (lam * div_uw + (2.0f * mu) * w_z);
+            const R div_uw = u_x + w_z;
+            const R lam = a.lam[c], mu = a.mu[c];
+            sxx = sxx + sdt * (lam * div_uw + (R(2) * mu) * u_x);
+            szz = szz + sdt * (lam * div_uw + (R(2) * mu) * w_z);
             sxz = sxz + (sdt * a.mxz[c]) * (u_z + w_x);
         }
         a.sxx[c] = sxx * A;
@@ -144,9 +145,9 @@ __global__ __launch_bounds__(256) void k_stress(KArgs a) {
         a.sxz[c] = sxz * A;
     }
     if (SH) {
-        float syx = a.syx[c], syz = a.syz[c];
+        R syx = a.syx[c], syz = a.syz[c];
         if (inner) {
-            float v_x, v_z;
+            R v_x, v_z;
             if (FWD) {
                 v_x = (a.v[c + nz] - a.v[c]) * a.idx;
                 v_z = (a.v[c + 1] - a.v[c]) * a.idz;
@@ -162,17 +163,17 @@ __global__ __launch_bounds__(256) void k_stress(KArgs a) {
     }
 }
 
-template <bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_post(KArgs a, StepIO p) {
+template <class R, bool PSV, bool SH>
+__global__ __launch_bounds__(256) void k_post(KA<R> a, SIO<R> p) {
     const int j = blockIdx.x * kBlockZ + threadIdx.x;
     const int i = blockIdx.y * kBlockX + threadIdx.y;
     if (i >= a.nx || j >= a.nz) return;
     const size_t n = (size_t)a.nx * a.nz;
     const size_t c = (size_t)i * a.nz + j;
-    const float A = sponge_at(i, j, a.nx, a.nz, a.fw, a.tab);
+    const R A = sponge_at_t<R>(i, j, a.nx, a.nz, a.fw, a.tab);
     int code = 0;
     if (PSV) {
-        const float u = a.u[c] * A, w = a.w[c] * A;
+        const R u = a.u[c] * A, w = a.w[c] * A;
         a.u[c] = u;
         a.w[c] = w;
         if (p.snap_slot >= 0) {
@@ -184,24 +185,24 @@ __global__ __launch_bounds__(256) void k_post(KArgs a, StepIO p) {
             p.image_w[c] = p.image_w[c] + p.snap_w[(size_t)p.image_slot * n + c] * w;
         }
         if (p.check_slot >= 0) {
-            if (not_finite_or_big(u)) code = p.check_base + 1;
-            if (not_finite_or_big(w)) code = p.check_base + 3;
+            if (bad_value<R>(u)) code = p.check_base + 1;
+            if (bad_value<R>(w)) code = p.check_base + 3;
         }
     }
     if (SH) {
-        const float v = a.v[c] * A;
+        const R v = a.v[c] * A;
         a.v[c] = v;
         if (p.snap_slot >= 0) p.snap_v[(size_t)p.snap_slot * n + c] = v;
         if (p.image_slot >= 0)
             p.image_v[c] = p.image_v[c] + p.snap_v[(size_t)p.image_slot * n + c] * v;
-        if (p.check_slot >= 0 && not_finite_or_big(v) && code < p.check_base + 2)
+        if (p.check_slot >= 0 && bad_value<R>(v) && code < p.check_base + 2)
             code = p.check_base + 2;
     }
     if (code) atomicMax(p.check_flags + p.check_slot, code);
 }
 
-template <bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_record(KArgs a, StepIO p) {
+template <class R, bool PSV, bool SH>
+__global__ __launch_bounds__(256) void k_record(KA<R> a, SIO<R> p) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= p.n_rec) return;
     const size_t c = (size_t)p.rec_loc[2 * l] * a.nz + p.rec_loc[2 * l + 1];
@@ -213,35 +214,109 @@ __global__ __launch_bounds__(256) void k_record(KArgs a, StepIO p) {
     if (SH) p.rec_v[o] = a.v[c];
 }
 
-template <bool FWD, bool PSV, bool SH>
-static hipError_t simple_step(const KArgs& a, const InjArgs& q, const StepIO& p, hipStream_t s) {
+template <class R, bool FWD, bool PSV, bool SH>
+static hipError_t simple_step(const KA<R>& a, const IA<R>& q, const SIO<R>& p, hipStream_t s) {
     const dim3 block(kBlockZ, kBlockX);
     const dim3 grid((a.nz + kBlockZ - 1) / kBlockZ, (a.nx + kBlockX - 1) / kBlockX);
-    if (a.sm) hipLaunchKernelGGL((k_surface<PSV, SH>), grid, block, 0, s, a);
-    hipLaunchKernelGGL((k_velocity<FWD, PSV, SH>), grid, block, 0, s, a);
+    if (a.sm) hipLaunchKernelGGL((k_surface<R, PSV, SH>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_velocity<R, FWD, PSV, SH>), grid, block, 0, s, a);
     if (q.n_groups > 0)
-        hipLaunchKernelGGL((k_inject<FWD, PSV, SH>), dim3((q.n_groups + 255) / 256), dim3(256), 0,
-                           s, a, q);
-    hipLaunchKernelGGL((k_stress<FWD, PSV, SH>), grid, block, 0, s, a);
-    hipLaunchKernelGGL((k_post<PSV, SH>), grid, block, 0, s, a, p);
+        hipLaunchKernelGGL((k_inject<R, FWD, PSV, SH>), dim3((q.n_groups + 255) / 256), dim3(256),
+                           0, s, a, q);
+    hipLaunchKernelGGL((k_stress<R, FWD, PSV, SH>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_post<R, PSV, SH>), grid, block, 0, s, a, p);
     if (p.n_rec > 0 && p.rec_row >= 0)
-        hipLaunchKernelGGL((k_record<PSV, SH>), dim3((p.n_rec + 255) / 256), dim3(256), 0, s, a, p);
+        hipLaunchKernelGGL((k_record<R, PSV, SH>), dim3((p.n_rec + 255) / 256), dim3(256), 0, s,
+                           a, p);
     return hipGetLastError();
 }
 
-hipError_t launch_simple_step(bool fwd, int systems, const KArgs& a, const InjArgs& q,
-                              const StepIO& p, hipStream_t s) {
+// Forward step with the FWI history of the step: the stresses the velocity
+// update reads (hs: sxx, szz, sxz, syx, syz) and the velocities after the
+// update and the injection (hv: u, w, v); NULL entries are skipped.
+template <class R, bool PSV, bool SH>
+static hipError_t hist_step(const KA<R>& a, const IA<R>& q, const SIO<R>& p, R* const* hs,
+                            R* const* hv, hipStream_t s) {
+    const dim3 block(kBlockZ, kBlockX);
+    const dim3 grid((a.nz + kBlockZ - 1) / kBlockZ, (a.nx + kBlockX - 1) / kBlockX);
+    const size_t bytes = (size_t)a.nx * a.nz * sizeof(R);
+    if (a.sm) hipLaunchKernelGGL((k_surface<R, PSV, SH>), grid, block, 0, s, a);
+    const R* sig[5] = {a.sxx, a.szz, a.sxz, a.syx, a.syz};
+    for (int k = 0; k < 5; k++)
+        if (hs && hs[k]) hipMemcpyAsync(hs[k], sig[k], bytes, hipMemcpyDeviceToDevice, s);
+    hipLaunchKernelGGL((k_velocity<R, true, PSV, SH>), grid, block, 0, s, a);
+    if (q.n_groups > 0)
+        hipLaunchKernelGGL((k_inject<R, true, PSV, SH>), dim3((q.n_groups + 255) / 256), dim3(256),
+                           0, s, a, q);
+    const R* vel[3] = {a.u, a.w, a.v};
+    for (int k = 0; k < 3; k++)
+        if (hv && hv[k]) hipMemcpyAsync(hv[k], vel[k], bytes, hipMemcpyDeviceToDevice, s);
+    hipLaunchKernelGGL((k_stress<R, true, PSV, SH>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_post<R, PSV, SH>), grid, block, 0, s, a, p);
+    if (p.n_rec > 0 && p.rec_row >= 0)
+        hipLaunchKernelGGL((k_record<R, PSV, SH>), dim3((p.n_rec + 255) / 256), dim3(256), 0, s,
+                           a, p);
+    return hipGetLastError();
+}
+
+template <class R>
+hipError_t launch_hist_step(int systems, const KA<R>& a, const IA<R>& q, const SIO<R>& p,
+                            R* const* hs, R* const* hv, hipStream_t s) {
+    const bool psv = systems & TRTM_PSV, sh = systems & TRTM_SH;
+    if (psv && sh) return hist_step<R, true, true>(a, q, p, hs, hv, s);
+    if (psv) return hist_step<R, true, false>(a, q, p, hs, hv, s);
+    if (sh) return hist_step<R, false, true>(a, q, p, hs, hv, s);
+    return hipSuccess;
+}
+
+template hipError_t launch_hist_step<float>(int, const KA<float>&, const IA<float>&,
+                                            const SIO<float>&, float* const*, float* const*,
+                                            hipStream_t);
+template hipError_t launch_hist_step<double>(int, const KA<double>&, const IA<double>&,
+                                             const SIO<double>&, double* const*, double* const*,
+                                             hipStream_t);
+
+template <class R>
+hipError_t launch_simple_step_t(bool fwd, int systems, const KA<R>& a, const IA<R>& q,
+                                const SIO<R>& p, hipStream_t s) {
     const bool psv = systems & TRTM_PSV, sh = systems & TRTM_SH;
     if (fwd) {
-        if (psv && sh) return simple_step<true, true, true>(a, q, p, s);
-        if (psv) return simple_step<true, true, false>(a, q, p, s);
-        if (sh) return simple_step<true, false, true>(a, q, p, s);
+        if (psv && sh) return simple_step<R, true, true, true>(a, q, p, s);
+        if (psv) return simple_step<R, true, true, false>(a, q, p, s);
+        if (sh) return simple_step<R, true, false, true>(a, q, p, s);
     } else {
-        if (psv && sh) return simple_step<false, true, true>(a, q, p, s);
-        if (psv) return simple_step<false, true, false>(a, q, p, s);
-        if (sh) return simple_step<false, false, true>(a, q, p, s);
+        if (psv && sh) return simple_step<R, false, true, true>(a, q, p, s);
+        if (psv) return simple_step<R, false, true, false>(a, q, p, s);
+        if (sh) return simple_step<R, false, false, true>(a, q, p, s);
     }
     return hipSuccess;
+}
+
+template hipError_t launch_simple_step_t<double>(bool, int, const KA<double>&, const IA<double>&,
+                                                 const SIO<double>&, hipStream_t);
+
+// float entry used by the product path: the KArgs / InjArgs / StepIO layouts
+// are the float instances of KA / IA / SIO
+hipError_t launch_simple_step(bool fwd, int systems, const KArgs& a, const InjArgs& q,
+                              const StepIO& p, hipStream_t s) {
+    KA<float> ka;
+    ka.nx = a.nx; ka.nz = a.nz; ka.nt = a.nt; ka.fw = a.fw;
+    ka.dt = a.dt; ka.idx = a.idx; ka.idz = a.idz; ka.factor = a.factor;
+    ka.tab = a.tab; ka.lam = a.lam; ka.mu = a.mu; ka.mxz = a.mxz; ka.myx = a.myx; ka.myz = a.myz;
+    ka.ir = a.ir; ka.iru = a.iru; ka.irw = a.irw; ka.sm = a.sm;
+    ka.u = a.u; ka.v = a.v; ka.w = a.w; ka.sxx = a.sxx; ka.szz = a.szz; ka.sxz = a.sxz;
+    ka.syx = a.syx; ka.syz = a.syz;
+    IA<float> qa;
+    qa.n_groups = q.n_groups; qa.col = q.col; qa.loc = q.loc; qa.order = q.order;
+    qa.start = q.start; qa.su = q.su; qa.sv = q.sv; qa.sw = q.sw;
+    SIO<float> pa;
+    pa.snap_slot = p.snap_slot; pa.image_slot = p.image_slot; pa.check_slot = p.check_slot;
+    pa.check_base = p.check_base; pa.rec_row = p.rec_row;
+    pa.snap_u = p.snap_u; pa.snap_v = p.snap_v; pa.snap_w = p.snap_w;
+    pa.image_u = p.image_u; pa.image_v = p.image_v; pa.image_w = p.image_w;
+    pa.check_flags = p.check_flags; pa.n_rec = p.n_rec; pa.rec_loc = p.rec_loc;
+    pa.rec_u = p.rec_u; pa.rec_v = p.rec_v; pa.rec_w = p.rec_w;
+    return launch_simple_step_t<float>(fwd, systems, ka, qa, pa, s);
 }
 
 }  // namespace trtm

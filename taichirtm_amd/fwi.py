@@ -1,0 +1,264 @@
+"""FWI gradients through the exact discrete adjoint of the forward scheme.
+
+The reference has no gradient: its backward propagator (backward_modeling.py)
+swaps the difference directions and injects data unscaled, so it is the RTM
+back-propagator, not the adjoint of its forward step.  Here:
+
+- ``ElasticFWI.forward(vp, vs, rho)`` runs every shot with the forward scheme
+  of forward_modeling.py:539-551 (same expressions) and keeps the full state
+  every ``checkpoint_every`` steps;
+- ``ElasticFWI.gradient(data_adjoint)`` walks the checkpoint segments last to
+  first, recomputes each segment's history (the stresses and velocities the
+  updates read), runs the adjoint steps over it and accumulates the gradient in
+  coefficient space (lam, mu, mxz, myx, myz, inv_rho, inv_rho_u, inv_rho_w),
+  then maps it through the harmonic / arithmetic averages to (lam, mu, rho) and
+  to (vp, vs, rho) with mu = rho vs^2, lam = rho (vp^2 - 2 vs^2)
+  (rtm.py:901-902);
+- ``Seismograms.apply(vp, vs, rho, problem)`` is the torch.autograd.Function:
+  forward gives the records (n_shots, 3, nr, nt); backward injects the
+  incoming gradient as the data adjoint.
+
+``precision="float64"`` runs the same kernels in double, for finite-difference
+and dot-product checks of the float32 gradient.  Flat free surface only.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import List, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._propagator import SPONGE_A, _ptr
+from .batch import ShotSpec, checkpoint_interval
+from .field import as_tensor, device, stream_handle
+
+_REAL = {"float32": (torch.float32, 4), "float64": (torch.float64, 8)}
+
+
+def _plan(nx, nz, loc):
+    loc = np.ascontiguousarray(loc, np.int32).reshape(-1, 2)
+    n = len(loc)
+    order = np.zeros(max(n, 1), np.int32)
+    start = np.zeros(n + 1, np.int32)
+    ng = ctypes.c_int32()
+    _lib.call("trtm_plan_injection", nx, nz, n, loc.ctypes.data_as(ctypes.c_void_p),
+              order.ctypes.data_as(ctypes.c_void_p), start.ctypes.data_as(ctypes.c_void_p),
+              ctypes.byref(ng))
+    return (as_tensor(loc, np.int32), as_tensor(order, np.int32),
+            as_tensor(start[: ng.value + 1], np.int32), ng.value)
+
+
+def moduli(vp, vs, rho):
+    """mu = rho * vs^2 and lam = ((vp / vs)^2 - 2) * mu, in the inputs' precision,
+    with the operation order of rtm.py:901-902."""
+    mu = rho * (vs * vs)
+    r = vp / vs
+    lam = (r * r - 2) * mu
+    return mu, lam
+
+
+class ElasticFWI:
+    """Forward records and exact gradients for shots on one (nx, nz) grid."""
+
+    def __init__(self, nx, nz, dx, dz, fs, absorbing_frame, shots: Sequence[ShotSpec],
+                 precision="float32", checkpoint_every=None):
+        if precision not in _REAL:
+            raise ValueError("precision must be float32 or float64")
+        self.nx, self.nz = int(nx), int(nz)
+        self.dx, self.dz, self.fs = float(dx), float(dz), float(fs)
+        self.fw = int(absorbing_frame)
+        if self.fw < 0 or self.fw > min(nx, nz):
+            raise ValueError("absorbing_frame outside [0, min(nx, nz)]")
+        self.shots = list(shots)
+        self.dtype, self.rb = _REAL[precision]
+        self.precision = precision
+        dev = device()
+        _lib.load()
+        dt = 1.0 / self.fs
+        self.consts = (dt, 1.0 / self.dx, 1.0 / self.dz, dt * self.dx * self.dz)
+        if precision == "float32":
+            tab = np.zeros(self.fw + 1, np.float32)
+            _lib.call("trtm_sponge_table", self.fw, ctypes.c_float(SPONGE_A),
+                      tab.ctypes.data_as(ctypes.c_void_p))
+        else:
+            k = np.arange(self.fw + 1, dtype=np.float64)
+            tab = np.exp((-(SPONGE_A * SPONGE_A) * k) * k)
+        self.sponge = torch.as_tensor(tab, device=dev)
+        self.geo = []
+        for s in self.shots:
+            src = _plan(nx, nz, s.src_loc)
+            rec = _plan(nx, nz, s.rec_loc)
+            wav = [torch.as_tensor(np.asarray(w), dtype=self.dtype, device=dev).reshape(len(src[0]), -1)
+                   .contiguous() for w in s.wavelets]
+            self.geo.append({"src": src, "rec": rec, "wav": wav, "nt": s.nt,
+                             "nr": len(rec[0]), "ns": len(src[0])})
+        self.checkpoint_every = checkpoint_every
+        self._ck = None
+
+    # -- medium ---------------------------------------------------------------
+
+    def _medium(self, vp, vs, rho):
+        nx, nz = self.nx, self.nz
+        vp = vp.to(device=device(), dtype=self.dtype).contiguous()
+        vs = vs.to(device=device(), dtype=self.dtype).contiguous()
+        rho = rho.to(device=device(), dtype=self.dtype).contiguous()
+        mu, lam = moduli(vp, vs, rho)
+        mu, lam = mu.contiguous(), lam.contiguous()
+        c = {k: torch.empty((nx, nz), dtype=self.dtype, device=vp.device)
+             for k in ("myx", "myz", "mxz", "rho_u", "rho_w", "inv_rho", "inv_rho_u", "inv_rho_w")}
+        fn = "trtm_prepare_medium" if self.rb == 4 else "trtm_prepare_medium_f64"
+        _lib.call(fn, nx, nz, _ptr(mu), _ptr(rho), _ptr(c["myx"]), _ptr(c["myz"]), _ptr(c["mxz"]),
+                  _ptr(c["rho_u"]), _ptr(c["rho_w"]), _ptr(c["inv_rho"]), _ptr(c["inv_rho_u"]),
+                  _ptr(c["inv_rho_w"]), stream_handle())
+        c.update(mu=mu, lam=lam, rho=rho, vp=vp, vs=vs)
+        for k in ("lam", "mu", "mxz", "inv_rho_u", "inv_rho_w"):
+            if not bool(torch.isfinite(c[k][1:-1, 1:-1]).all()):
+                raise ValueError("non-finite %s on the interior (in SI units the float32 "
+                                 "harmonic mean of mu overflows; use g/cm^3 and kPa)" % k)
+        return c
+
+    def _desc(self, med, g):
+        d = _lib.FwiDesc()
+        d.struct_size = ctypes.sizeof(_lib.FwiDesc)
+        d.real_bytes = self.rb
+        d.nx, d.nz, d.nt, d.fw = self.nx, self.nz, g["nt"], self.fw
+        d.systems = self.systems
+        d.dt, d.inv_dx, d.inv_dz, d.src_factor = self.consts
+        d.sponge_table = _ptr(self.sponge)
+        for k in ("lam", "mu", "mxz", "myx", "myz", "inv_rho", "inv_rho_u", "inv_rho_w"):
+            setattr(d, k, _ptr(med[k]))
+        loc, order, start, ng = g["src"]
+        d.n_src, d.n_src_groups = len(loc), ng
+        d.src_loc, d.src_order, d.src_group_start = _ptr(loc), _ptr(order), _ptr(start)
+        d.wav_u, d.wav_v, d.wav_w = (_ptr(w) for w in g["wav"])
+        loc, order, start, ng = g["rec"]
+        d.n_rec, d.n_rec_groups = len(loc), ng
+        d.rec_loc, d.rec_order, d.rec_group_start = _ptr(loc), _ptr(order), _ptr(start)
+        return d
+
+    def _systems(self):
+        psv = any(bool((g["wav"][0] != 0).any()) or bool((g["wav"][2] != 0).any())
+                  for g in self.geo)
+        sh = any(bool((g["wav"][1] != 0).any()) for g in self.geo)
+        return (_lib.PSV if psv else 0) | (_lib.SH if sh else 0) or _lib.PSV
+
+    def _fields(self):
+        names = []
+        if self.systems & _lib.PSV:
+            names += ["u", "w", "sxx", "szz", "sxz"]
+        if self.systems & _lib.SH:
+            names += ["v", "syx", "syz"]
+        return names
+
+    # -- forward --------------------------------------------------------------
+
+    def forward(self, vp, vs, rho):
+        """Records of every shot, a list of (3, nr, nt) tensors (u, v, w);
+        keeps the checkpoints for ``gradient``."""
+        dev = device()
+        self.systems = self._systems()
+        self.med = self._medium(vp, vs, rho)
+        names = self._fields()
+        nt_max = max(g["nt"] for g in self.geo)
+        n_hist = (5 if self.systems & _lib.PSV else 0) + (3 if self.systems & _lib.SH else 0)
+        self.L = self.checkpoint_every or checkpoint_interval(nt_max, len(names), n_hist)
+        shape = (self.nx, self.nz)
+        self.state = {k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in names}
+        self._ck = []
+        out = []
+        s = stream_handle()
+        for g in self.geo:
+            for t in self.state.values():
+                t.zero_()
+            d = self._desc(self.med, g)
+            for k in names:
+                setattr(d, k, _ptr(self.state[k]))
+            rec = {c: torch.zeros((g["nt"], g["nr"]), dtype=self.dtype, device=dev)
+                   for c in ("u", "v", "w")}
+            d.rec_u, d.rec_v, d.rec_w = (_ptr(rec[c]) for c in ("u", "v", "w"))
+            edges = list(range(0, g["nt"], self.L)) + [g["nt"]]
+            ck = torch.empty((len(edges) - 1, len(names)) + shape, dtype=self.dtype, device=dev)
+            for k in range(len(edges) - 1):
+                for m, name in enumerate(names):
+                    ck[k, m].copy_(self.state[name])
+                _lib.call("trtm_fwi_forward", d, edges[k], edges[k + 1], s)
+            self._ck.append((edges, ck))
+            out.append(torch.stack([rec[c].t() for c in ("u", "v", "w")]).contiguous())
+        return out
+
+    # -- gradient ---------------------------------------------------------------
+
+    def gradient(self, data_adjoint: List[torch.Tensor]):
+        """Gradient of J with respect to (vp, vs, rho), float64 (nx, nz) each,
+        for dJ/d(records) given per shot as (3, nr, nt)."""
+        if self._ck is None:
+            raise RuntimeError("call forward first")
+        dev = device()
+        names = self._fields()
+        shape = (self.nx, self.nz)
+        adj = {"p" + k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in names}
+        gnames = ("g_lam", "g_mu", "g_mxz", "g_myx", "g_myz", "g_ir", "g_iru", "g_irw")
+        grads = {k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in gnames}
+        n_hist = (5 if self.systems & _lib.PSV else 0) + (3 if self.systems & _lib.SH else 0)
+        hist = torch.empty((self.L, n_hist) + shape, dtype=self.dtype, device=dev)
+        s = stream_handle()
+        for g, (edges, ck), da in zip(self.geo, self._ck, data_adjoint):
+            for t in adj.values():
+                t.zero_()
+            d = self._desc(self.med, g)
+            for k in names:
+                setattr(d, k, _ptr(self.state[k]))
+                setattr(d, "p" + k, _ptr(adj["p" + k]))
+            for k in gnames:
+                setattr(d, k, _ptr(grads[k]))
+            da = torch.as_tensor(da, device=dev).to(self.dtype)
+            res = [da[c].t().contiguous() for c in range(3)]  # (nt, nr) each
+            d.res_u, d.res_v, d.res_w = (_ptr(r) for r in res)
+            d.hist = _ptr(hist)
+            for k in range(len(edges) - 2, -1, -1):
+                a, b = edges[k], edges[k + 1]
+                for m, name in enumerate(names):
+                    self.state[name].copy_(ck[k, m])
+                d.hist_base, d.hist_len = a, b - a
+                _lib.call("trtm_fwi_forward", d, a, b, s)
+                _lib.call("trtm_fwi_adjoint", d, a, b, s)
+        m = self.med
+        dbl = lambda t: t.to(torch.float64).contiguous()
+        out = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(3)]
+        args = [dbl(m["mu"]), dbl(m["rho"]), dbl(m["vp"]), dbl(m["vs"])] + \
+               [dbl(grads[k]) for k in gnames]
+        _lib.call("trtm_fwi_chain_gradient", self.nx, self.nz, *[_ptr(t) for t in args],
+                  *[_ptr(t) for t in out], s)
+        self.coefficient_gradient = grads
+        return tuple(out)
+
+
+class Seismograms(torch.autograd.Function):
+    """(vp, vs, rho) -> records (n_shots, 3, nr, nt); all shots need the same
+    nr and nt.  Backward: the exact adjoint gradient."""
+
+    @staticmethod
+    def forward(ctx, vp, vs, rho, problem: ElasticFWI):
+        recs = problem.forward(vp.detach(), vs.detach(), rho.detach())
+        if len({tuple(r.shape) for r in recs}) != 1:
+            raise ValueError("autograd records need equal receiver counts and record lengths")
+        ctx.problem = problem
+        ctx.dtypes = (vp.dtype, vs.dtype, rho.dtype)
+        return torch.stack(recs).to(vp.dtype)
+
+    @staticmethod
+    def backward(ctx, grad):
+        g_vp, g_vs, g_rho = ctx.problem.gradient([grad[q] for q in range(grad.shape[0])])
+        return (g_vp.to(ctx.dtypes[0]), g_vs.to(ctx.dtypes[1]), g_rho.to(ctx.dtypes[2]), None)
+
+
+def l2_misfit_and_gradient(problem: ElasticFWI, vp, vs, rho, observed):
+    """J = 1/2 sum (d - d_obs)^2 over every shot, component, receiver and
+    sample, and its gradient (vp, vs, rho) in float64."""
+    recs = problem.forward(vp, vs, rho)
+    res = [r - torch.as_tensor(o, device=r.device).to(r.dtype) for r, o in zip(recs, observed)]
+    J = 0.5 * sum(float((x.double() ** 2).sum()) for x in res)
+    return J, problem.gradient(res)

@@ -20,7 +20,7 @@ HEADER = os.path.join(ROOT, "include", "taichirtm_hip.h")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(trtm_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(trtm_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -36,11 +36,13 @@ def test_abi_version():
     assert _lib.load().trtm_abi_version() == _lib.ABI_VERSION == 2
 
 
-def test_descriptor_layout_matches_c():
-    fields = [f for f, _ in _lib.StepDesc._fields_]
+@pytest.mark.parametrize("cname,pystruct", [("trtm_step_desc", _lib.StepDesc),
+                                            ("trtm_fwi_desc", _lib.FwiDesc)])
+def test_descriptor_layout_matches_c(cname, pystruct):
+    fields = [f for f, _ in pystruct._fields_]
     prog = ['#include <stdio.h>', '#include <stddef.h>', '#include "taichirtm_hip.h"',
-            "int main(void) {", '  printf("%zu\\n", sizeof(trtm_step_desc));']
-    prog += ['  printf("%%zu\\n", offsetof(trtm_step_desc, %s));' % f for f in fields]
+            "int main(void) {", '  printf("%%zu\\n", sizeof(%s));' % cname]
+    prog += ['  printf("%%zu\\n", offsetof(%s, %s));' % (cname, f) for f in fields]
     prog += ["  return 0;", "}"]
     with tempfile.TemporaryDirectory() as td:
         src = os.path.join(td, "layout.c")
@@ -49,9 +51,9 @@ def test_descriptor_layout_matches_c():
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
         out = [int(x) for x in subprocess.run([exe], capture_output=True, text=True,
                                               check=True).stdout.split()]
-    assert out[0] == ctypes.sizeof(_lib.StepDesc)
+    assert out[0] == ctypes.sizeof(pystruct)
     for f, off in zip(fields, out[1:]):
-        assert getattr(_lib.StepDesc, f).offset == off, f
+        assert getattr(pystruct, f).offset == off, f
 
 
 @pytest.mark.parametrize("fw", [0, 1, 20, 40, 100])
