@@ -198,19 +198,25 @@ def main():
     ms_per_step = wall * 1000.0 / args.steps
     gcells = 2.0 * cells * nt / (ms_per_step / 1000.0) / 1e9
     # time steps per launch of the step kernel (1 for the fused and simple kernels)
-    k = shot.bw.block_config()[3] if args.kernel in ("blocked", "auto") else 1
+    blocked = args.kernel in ("blocked", "wave", "auto")
+    k = shot.bw.block_config()[3] if blocked else 1
     launches = (nt + k - 1) // k
     bw_launch_us = 1000.0 * float(np.mean(bw_ms)) / launches
     fw_launch_us = 1000.0 * float(np.mean(fw_ms)) / launches
     b_bytes = ALGO_BYTES[(system, "backward")] * cells * k
     f_bytes = ALGO_BYTES[(system, "forward")] * cells * k
     achieved = b_bytes / (bw_launch_us * 1e-6) / 1e9
-    kname = ("k_block backward, %d steps per launch, image accumulated in registers" % k
-             if k > 1 else "k_fused backward step with imaging")
+    if args.kernel == "wave":
+        kname = ("k_wave backward, %d steps per launch, one wavefront per tile, state and "
+                 "image in registers" % k)
+    elif k > 1:
+        kname = "k_block backward, %d steps per launch, image accumulated in registers" % k
+    else:
+        kname = "k_fused backward step with imaging"
     # HBM bytes per launch of the default blocked SH configuration, from the
     # FETCH_SIZE / WRITE_SIZE passes recorded under profiles/
-    traffic = MEASURED_TRAFFIC.get((cfg.name, args.kernel, shot.bw.block_config()
-                                    if args.kernel in ("blocked", "auto") else None))
+    traffic = MEASURED_TRAFFIC.get((cfg.name, args.kernel,
+                                    shot.bw.block_config() if blocked else None))
     out = {
         "metric": METRIC,
         "value": round(gcells, 3),

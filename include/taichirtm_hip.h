@@ -49,6 +49,7 @@ extern "C" {
 #define TRTM_KERNEL_SIMPLE 1 /* one launch per reference kernel group */
 #define TRTM_KERNEL_FUSED 2  /* one LDS-tiled launch per time step */
 #define TRTM_KERNEL_BLOCKED 3 /* one launch per block_steps time steps (temporal blocking) */
+#define TRTM_KERNEL_WAVE 4    /* temporal blocking, one wavefront per tile, state in registers */
 
 /* ABI version compiled into the library (== TRTM_ABI_VERSION). */
 int trtm_abi_version(void);

@@ -15,7 +15,7 @@ ABI_VERSION = 2
 OK = 0
 ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3
 PSV, SH = 1, 2
-KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_FUSED, KERNEL_BLOCKED = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_FUSED, KERNEL_BLOCKED, KERNEL_WAVE = 0, 1, 2, 3, 4
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32

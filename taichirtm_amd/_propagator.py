@@ -21,7 +21,7 @@ STATE = ("u", "v", "w", "sxx", "szz", "sxz", "syx", "syz")
 PSV_STATE = ("u", "w", "sxx", "szz", "sxz")
 SH_STATE = ("v", "syx", "syz")
 KERNELS = {"auto": _lib.KERNEL_AUTO, "simple": _lib.KERNEL_SIMPLE, "fused": _lib.KERNEL_FUSED,
-           "blocked": _lib.KERNEL_BLOCKED}
+           "blocked": _lib.KERNEL_BLOCKED, "wave": _lib.KERNEL_WAVE}
 
 
 def _ptr(t):

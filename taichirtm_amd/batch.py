@@ -86,16 +86,19 @@ class BatchRTM:
     lam_field, rho_field, optional surface_matrix_field), absorbing_frame, the
     shots, the snapshot interval isnap, imaging ("snapshot" or "checkpoint"),
     checkpoint_every (segment length; default from the memory model),
-    stability_check_interval, and the blocked kernel's tile_x / tile_z /
-    block_steps (0 = library default).
+    stability_check_interval, the temporally blocked kernel ("blocked": LDS
+    tiles, or "wave": one wavefront per tile with the state in registers) and
+    its tile_x / tile_z / block_steps (0 = library default).
     """
 
     def __init__(self, nx, nz, dx, dz, fs, mu_field, lam_field, rho_field, absorbing_frame,
                  shots: List[ShotSpec], isnap=1, imaging="snapshot", checkpoint_every=None,
                  stability_check_interval=100, surface_matrix_field=None, tile_x=0, tile_z=0,
-                 block_steps=0):
+                 block_steps=0, kernel="blocked"):
         if imaging not in ("snapshot", "checkpoint"):
             raise ValueError("imaging must be 'snapshot' or 'checkpoint'")
+        if kernel not in ("blocked", "wave"):
+            raise ValueError("kernel must be 'blocked' or 'wave'")
         if not shots:
             raise ValueError("no shots")
         self.medium = Medium(nx, nz, dx, dz, fs, absorbing_frame, mu_field, lam_field, rho_field,
@@ -106,7 +109,7 @@ class BatchRTM:
         self.imaging = imaging
         self.checkpoint_every = checkpoint_every
         self.sci = int(stability_check_interval)
-        self.block = dict(tile_x=tile_x, tile_z=tile_z, block_steps=block_steps)
+        self.block = dict(tile_x=tile_x, tile_z=tile_z, block_steps=block_steps, kernel=kernel)
         self.codes = []
         self.fw: List[ForwardModeling] = []
         self.bw: List[BackwardModeling] = []
@@ -119,7 +122,7 @@ class BatchRTM:
         return dict(nx=self.nx, nz=self.nz, dx=self.medium.dx, dz=self.medium.dz, nt=shot.nt,
                     fs=self.medium.fs, absorbing_frame=self.medium.fw, medium=self.medium,
                     src_loc_field=src, recv_loc_field=rec, num_sources=len(src),
-                    num_receivers=len(rec), kernel="blocked", **self.block)
+                    num_receivers=len(rec), **self.block)
 
     def _batch(self, fn, descs, it_begin, it_end, *extra):
         arr = _desc_array(descs)

@@ -165,7 +165,7 @@ static int check_desc(const trtm_step_desc* d, int32_t it_begin, int32_t it_end)
     }
     if (d->check_interval < 0) return fail(TRTM_ERR_ARG, "check_interval < 0");
     if (d->check_interval > 0 && !d->check_flags) return fail(TRTM_ERR_ARG, "check_flags is NULL");
-    if (d->kernel < TRTM_KERNEL_AUTO || d->kernel > TRTM_KERNEL_BLOCKED)
+    if (d->kernel < TRTM_KERNEL_AUTO || d->kernel > TRTM_KERNEL_WAVE)
         return fail(TRTM_ERR_ARG, "bad kernel selector %d", d->kernel);
     return TRTM_OK;
 }
@@ -222,7 +222,7 @@ static int run_steps(const trtm_step_desc* d, int32_t it_begin, int32_t it_end, 
     const int kernel = pick_kernel(d);
     if (d->step_offset != 0)
         return fail(TRTM_ERR_ARG, "step_offset is for batch calls; use 0 here");
-    if (kernel == TRTM_KERNEL_BLOCKED) {
+    if (kernel == TRTM_KERNEL_BLOCKED || kernel == TRTM_KERNEL_WAVE) {
         if (!d->workspace || d->workspace_bytes < block_workspace_bytes(d))
             return fail(TRTM_ERR_ARG, "blocked kernel needs %lld workspace bytes",
                         (long long)block_workspace_bytes(d));

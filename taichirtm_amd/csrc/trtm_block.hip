@@ -23,43 +23,11 @@
 #include <algorithm>
 #include <vector>
 
-#include "trtm_common.hpp"
+#include "trtm_block.hpp"
 
 namespace trtm {
 
 int fail(int code, const char* fmt, ...);
-
-struct ShotDev {
-    float* in[8];   // u, v, w, sxx, szz, sxz, syx, syz read at launch start
-    float* out[8];  // written at launch end
-    int nt, it_end, save, isnap;
-    int num_snaps, snap_base, check_interval, n_groups;
-    int n_rec, it_lo, it_off, pad2;
-    const int *loc, *order, *start;
-    const float *su, *sv, *sw;
-    const int* rec_loc;
-    float *rec_u, *rec_v, *rec_w;
-    const int *inj_tstart, *inj_titems, *rec_tstart, *rec_titems;
-    float *snap_u, *snap_v, *snap_w;
-    const int* isnaps;
-    float *image_u, *image_v, *image_w;
-    int* check_flags;
-};
-
-__device__ __forceinline__ int xcd_tile_b(int bid, int tiles) {
-    const int per = (tiles + 7) / 8;
-    return (bid % 8) * per + bid / 8;
-}
-
-// Pointers read from the shot table are generic; cast them to the global
-// address space so the compiler emits global_* (not flat_*) memory
-// instructions, which do not share the LDS wait counter.
-template <class T>
-using gptr = __attribute__((address_space(1))) T*;
-template <class T>
-__device__ __forceinline__ gptr<T> G(T* p) {
-    return (gptr<T>)p;
-}
 
 template <bool FWD, bool PSV, bool SH, int TX, int TZ, int NT, int K>
 __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict__ shots, int it0,
@@ -423,10 +391,6 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
 // ---------------------------------------------------------------------------
 // host side
 
-struct BlockCfg {
-    int tx, tz, nt, k;  // tile rows, tile columns, threads per workgroup, steps per launch
-};
-
 // configurations compiled per wave system
 static const BlockCfg kCfgSH[] = {{16, 64, 256, 4},   {16, 64, 256, 8},    {16, 128, 512, 4},
                                   {16, 128, 512, 8},  {32, 128, 1024, 4},  {32, 128, 1024, 8}};
@@ -455,16 +419,26 @@ static BlockCfg default_cfg(int systems) {
     return {16, 128, 512, 4};
 }
 
-// first compiled configuration matching every nonzero request field
+static BlockCfg default_wave_cfg(int systems) {
+    if (systems == TRTM_SH) return {16, 56, 64, 4};
+    if (systems == TRTM_PSV) return {16, 56, 64, 4};
+    return {8, 56, 64, 4};
+}
+
+// first compiled configuration of the requested kernel family matching every
+// nonzero request field
 int block_choose(const trtm_step_desc* d, BlockCfg* out) {
-    const BlockCfg def = default_cfg(d->systems);
+    const bool wave = d->kernel == TRTM_KERNEL_WAVE;
     if (d->tile_x <= 0 && d->tile_z <= 0 && d->block_steps <= 0) {
-        *out = def;
+        *out = wave ? default_wave_cfg(d->systems) : default_cfg(d->systems);
         return TRTM_OK;
     }
     const BlockCfg* list;
     int n;
-    cfg_list(d->systems, &list, &n);
+    if (wave)
+        wave_configs(d->systems, &list, &n);
+    else
+        cfg_list(d->systems, &list, &n);
     for (int i = 0; i < n; i++) {
         const BlockCfg& c = list[i];
         if ((d->tile_x <= 0 || c.tx == d->tile_x) && (d->tile_z <= 0 || c.tz == d->tile_z) &&
@@ -474,14 +448,15 @@ int block_choose(const trtm_step_desc* d, BlockCfg* out) {
         }
     }
     return fail(TRTM_ERR_UNSUPPORTED,
-                "blocked kernel has no tile_x=%d tile_z=%d block_steps=%d build for systems=%d",
-                d->tile_x, d->tile_z, d->block_steps, d->systems);
+                "%s kernel has no tile_x=%d tile_z=%d block_steps=%d build for systems=%d",
+                wave ? "wave" : "blocked", d->tile_x, d->tile_z, d->block_steps, d->systems);
 }
 
 static size_t align256b(size_t x) { return (x + 255) & ~size_t(255); }
 
-// workspace of one shot: second state buffer, tile lists sized for the
-// smallest tiles (16 x 64), device copy of isnaps, two shot-table entries
+// workspace of one shot: second state buffer, tile lists sized for every
+// compiled configuration of both kernel families, device copy of isnaps, two
+// shot-table entries
 struct BlockLayout {
     size_t alt, inj_start, inj_cursor, inj_items, rec_start, rec_cursor, rec_items, isnaps, shot,
         total;
@@ -493,16 +468,31 @@ static BlockLayout block_layout(const trtm_step_desc* d, BlockCfg c) {
     const size_t n = (size_t)d->nx * d->nz;
     L.tiles_x = (d->nx + c.tx - 1) / c.tx;
     L.tiles_z = (d->nz + c.tz - 1) / c.tz;
+    // LDS-tiled kernel: lists per tile, tiles no smaller than 16 x 64; a cell
+    // lies in at most 2 x 2 regions when 2K <= min(TX, TZ)
     const size_t tiles_max = (size_t)((d->nx + 15) / 16) * ((d->nz + 63) / 64);
-    const size_t mult = 4;  // a cell lies in at most 2 x 2 regions when 2K <= min(TX, TZ)
+    size_t ib = tiles_max + 1, rb = tiles_max + 1;
+    size_t ii = 4 * (size_t)std::max(d->n_inj_groups, 0) * sizeof(int);
+    size_t ri = (size_t)std::max(d->n_rec, 0) * sizeof(int);
+    // wavefront kernel: lists per tile row
+    const BlockCfg* list;
+    int nw;
+    wave_configs(d->systems, &list, &nw);
+    for (int k = 0; k < nw; k++) {
+        const ListSizes w = wave_list_sizes(d, list[k]);
+        ib = std::max(ib, w.inj_buckets);
+        rb = std::max(rb, w.rec_buckets);
+        ii = std::max(ii, w.inj_item_bytes);
+        ri = std::max(ri, w.rec_item_bytes);
+    }
     size_t off = 0;
     L.alt = off;        off = align256b(off + 8 * n * sizeof(float));
-    L.inj_start = off;  off = align256b(off + (tiles_max + 1) * sizeof(int));
-    L.inj_cursor = off; off = align256b(off + tiles_max * sizeof(int));
-    L.inj_items = off;  off = align256b(off + mult * (size_t)std::max(d->n_inj_groups, 0) * sizeof(int));
-    L.rec_start = off;  off = align256b(off + (tiles_max + 1) * sizeof(int));
-    L.rec_cursor = off; off = align256b(off + tiles_max * sizeof(int));
-    L.rec_items = off;  off = align256b(off + (size_t)std::max(d->n_rec, 0) * sizeof(int));
+    L.inj_start = off;  off = align256b(off + ib * sizeof(int));
+    L.inj_cursor = off; off = align256b(off + ib * sizeof(int));
+    L.inj_items = off;  off = align256b(off + ii);
+    L.rec_start = off;  off = align256b(off + rb * sizeof(int));
+    L.rec_cursor = off; off = align256b(off + rb * sizeof(int));
+    L.rec_items = off;  off = align256b(off + ri);
     L.isnaps = off;     off = align256b(off + (size_t)std::max(d->num_snaps, 0) * sizeof(int));
     L.shot = off;       off = align256b(off + 2 * sizeof(ShotDev));
     L.total = off;
@@ -511,34 +501,6 @@ static BlockLayout block_layout(const trtm_step_desc* d, BlockCfg c) {
 
 int64_t block_workspace_bytes(const trtm_step_desc* d) {
     return (int64_t)block_layout(d, BlockCfg{16, 64, 256, 1}).total;
-}
-
-__global__ void k_fill_int_b(int* p, int n, int value) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) p[k] = value;
-}
-
-__global__ __launch_bounds__(1024) void k_scan_b(const int* count, int* start, int n) {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const int chunk = (n + 1023) / 1024;
-    const int b = t * chunk, e = min(n, b + chunk);
-    int s = 0;
-    for (int k = b; k < e; k++) s += count[k];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    int run = t > 0 ? part[t - 1] : 0;
-    for (int k = b; k < e; k++) {
-        start[k] = run;
-        run += count[k];
-    }
-    if (t == 1023) start[n] = part[1023];
 }
 
 // mode 0 counts, mode 1 fills.  Groups go to every tile whose region (tile +
@@ -583,23 +545,33 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
     int* ritems = reinterpret_cast<int*>(ws + L.rec_items);
     const int ng = d->n_inj_groups;
     const int nr = (d->rec_u || d->rec_v) ? d->n_rec : 0;
-    const dim3 tb(256), tg((tiles + 255) / 256), lg((std::max(ng, nr) + 255) / 256);
-    hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, icur, tiles, 0);
-    hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, rcur, tiles, 0);
-    if (ng > 0 || nr > 0)
-        hipLaunchKernelGGL(k_block_lists, lg, tb, 0, s, 0, tx, cfg.tz, K, L.tiles_x, L.tiles_z, ng,
-                           d->inj_loc, d->inj_order, d->inj_group_start, icur, (const int*)nullptr,
-                           (int*)nullptr, nr, d->rec_loc, rcur, (const int*)nullptr, (int*)nullptr);
-    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, s, icur, istart, tiles);
-    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, s, rcur, rstart, tiles);
-    hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, icur, tiles, 0);
-    hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, rcur, tiles, 0);
-    if (ng > 0 || nr > 0)
-        hipLaunchKernelGGL(k_block_lists, lg, tb, 0, s, 1, tx, cfg.tz, K, L.tiles_x, L.tiles_z, ng,
-                           d->inj_loc, d->inj_order, d->inj_group_start, icur, (const int*)istart,
-                           iitems, nr, d->rec_loc, rcur, (const int*)rstart, ritems);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(TRTM_ERR_HIP, "blocked tile lists: %s", hipGetErrorString(e));
+    hipError_t e;
+    if (is_wave(cfg)) {
+        const int rc = wave_build_lists(d, cfg, istart, icur, ws + L.inj_items, rstart, rcur,
+                                        ws + L.rec_items, s);
+        if (rc) return rc;
+    } else {
+        const dim3 tb(256), tg((tiles + 255) / 256), lg((std::max(ng, nr) + 255) / 256);
+        hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, icur, tiles, 0);
+        hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, rcur, tiles, 0);
+        if (ng > 0 || nr > 0)
+            hipLaunchKernelGGL(k_block_lists, lg, tb, 0, s, 0, tx, cfg.tz, K, L.tiles_x, L.tiles_z,
+                               ng, d->inj_loc, d->inj_order, d->inj_group_start, icur,
+                               (const int*)nullptr, (int*)nullptr, nr, d->rec_loc, rcur,
+                               (const int*)nullptr, (int*)nullptr);
+        hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, s, icur, istart, tiles);
+        hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, s, rcur, rstart, tiles);
+        hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, icur, tiles, 0);
+        hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, rcur, tiles, 0);
+        if (ng > 0 || nr > 0)
+            hipLaunchKernelGGL(k_block_lists, lg, tb, 0, s, 1, tx, cfg.tz, K, L.tiles_x,
+                               L.tiles_z, ng, d->inj_loc, d->inj_order, d->inj_group_start, icur,
+                               (const int*)istart, iitems, nr, d->rec_loc, rcur,
+                               (const int*)rstart, ritems);
+        e = hipGetLastError();
+        if (e != hipSuccess)
+            return fail(TRTM_ERR_HIP, "blocked tile lists: %s", hipGetErrorString(e));
+    }
 
     ShotDev& x = *sd;
     float* state[8] = {d->u, d->v, d->w, d->sxx, d->szz, d->sxz, d->syx, d->syz};
@@ -686,6 +658,7 @@ static bool launch_sys(BlockCfg c, const KArgs& m, const ShotDev* shots, int n_s
 
 static bool launch_block(bool fwd, int systems, BlockCfg c, const KArgs& m, const ShotDev* shots,
                          int n_shots, int it0, int tiles_x, int tiles_z, hipStream_t s) {
+    if (is_wave(c)) return launch_wave(fwd, systems, c, m, shots, n_shots, it0, tiles_x, tiles_z, s);
     const bool psv = systems & TRTM_PSV, sh = systems & TRTM_SH;
     if (fwd) {
         if (psv && sh) return launch_sys<true, true, true>(c, m, shots, n_shots, it0, tiles_x, tiles_z, s);

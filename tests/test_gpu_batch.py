@@ -42,14 +42,15 @@ def _kw(cfg, mu, lam, shot):
                 num_sources=len(shot.src_loc), num_receivers=len(shot.rec_loc))
 
 
+@pytest.mark.parametrize("kernel", ["blocked", "wave"])
 @pytest.mark.parametrize("system", ["psv", "sh", "both"])
 @pytest.mark.parametrize("imaging,isnap", [("snapshot", 1), ("snapshot", 3), ("checkpoint", 1),
                                            ("checkpoint", 3)])
-def test_batch_equals_single_shots(system, imaging, isnap):
-    _case("batch %s %s isnap=%d" % (system, imaging, isnap))
+def test_batch_equals_single_shots(system, imaging, isnap, kernel):
+    _case("batch %s %s isnap=%d kernel=%s" % (system, imaging, isnap, kernel))
     cfg, mu, lam, shots = survey(system)
     b = BatchRTM(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, mu, lam, cfg.rho, cfg.fw, shots,
-                 isnap=isnap, imaging=imaging,
+                 isnap=isnap, imaging=imaging, kernel=kernel,
                  checkpoint_every=(12 if isnap == 3 else 16) if imaging == "checkpoint" else None)
     assert b.run() == [0] * len(shots)
     imgs = b.images()

@@ -10,7 +10,7 @@ from taichirtm_amd import configs
 from tests.parity import assert_bitwise, run_hip, run_oracle, same
 
 pytestmark = pytest.mark.gpu
-KERNELS = ["blocked", "fused", "simple"]
+KERNELS = ["blocked", "wave", "fused", "simple"]
 
 
 def _case(name):
@@ -226,5 +226,31 @@ def test_blocked_configurations_bitwise(system, cfg):
     wav = tuple(np.stack([x, 0.5 * x]) for x in comps)
     hk = dict(kw, tile_x=tx, tile_z=tz, block_steps=k)
     h = run_hip(hk, wav, isnap=3, kernel="blocked")
+    o = run_oracle(kw, wav, isnap=3)
+    assert_bitwise(h, o)
+
+
+WAVE_CFGS = {"psv": [(16, 4), (8, 4), (16, 2), (8, 2)],
+             "sh": [(16, 4), (8, 4), (24, 4), (16, 2)],
+             "both": [(8, 4), (8, 2)]}
+
+
+@pytest.mark.parametrize("system,cfg", [(s, c) for s, cs in WAVE_CFGS.items() for c in cs])
+def test_wave_configurations_bitwise(system, cfg):
+    """Every compiled shape of the one-wavefront-per-tile kernel, with nt not a
+    multiple of the block depth, duplicate sources, border receivers and a grid
+    that is not a multiple of the tile (partial tiles at the bottom and right)."""
+    tx, k = cfg
+    _case("wave %s tile_x=%d block_steps=%d" % (system, tx, k))
+    c = small_psv(nt=97) if system != "sh" else small_sh(nt=97)
+    kw = c.kwargs(c.shots[0])
+    kw["src_loc_field"] = np.array([[30, 5], [30, 5]], np.int32)
+    kw["num_sources"] = 2
+    w = configs.ricker(40.0, c.nt, c.dt)
+    z = np.zeros_like(w)
+    comps = {"psv": (w, z, w), "sh": (z, w, z), "both": (w, w, w)}[system]
+    wav = tuple(np.stack([x, 0.5 * x]) for x in comps)
+    hk = dict(kw, tile_x=tx, tile_z=64 - 2 * k, block_steps=k)
+    h = run_hip(hk, wav, isnap=3, kernel="wave")
     o = run_oracle(kw, wav, isnap=3)
     assert_bitwise(h, o)
