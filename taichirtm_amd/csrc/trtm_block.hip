@@ -458,8 +458,8 @@ static size_t align256b(size_t x) { return (x + 255) & ~size_t(255); }
 // compiled configuration of both kernel families, device copy of isnaps, two
 // shot-table entries
 struct BlockLayout {
-    size_t alt, inj_start, inj_cursor, inj_items, rec_start, rec_cursor, rec_items, isnaps, shot,
-        total;
+    size_t alt, inj_start, inj_cursor, inj_items, rec_start, rec_cursor, rec_items, maps, isnaps,
+        shot, total;
     int tiles_x, tiles_z;
 };
 
@@ -474,7 +474,8 @@ static BlockLayout block_layout(const trtm_step_desc* d, BlockCfg c) {
     size_t ib = tiles_max + 1, rb = tiles_max + 1;
     size_t ii = 4 * (size_t)std::max(d->n_inj_groups, 0) * sizeof(int);
     size_t ri = (size_t)std::max(d->n_rec, 0) * sizeof(int);
-    // wavefront kernel: lists per tile row
+    size_t mb = 0;
+    // wavefront kernel: lists per tile row, per-lane maps
     const BlockCfg* list;
     int nw;
     wave_configs(d->systems, &list, &nw);
@@ -484,6 +485,7 @@ static BlockLayout block_layout(const trtm_step_desc* d, BlockCfg c) {
         rb = std::max(rb, w.rec_buckets);
         ii = std::max(ii, w.inj_item_bytes);
         ri = std::max(ri, w.rec_item_bytes);
+        mb = std::max(mb, w.map_bytes);
     }
     size_t off = 0;
     L.alt = off;        off = align256b(off + 8 * n * sizeof(float));
@@ -493,6 +495,7 @@ static BlockLayout block_layout(const trtm_step_desc* d, BlockCfg c) {
     L.rec_start = off;  off = align256b(off + rb * sizeof(int));
     L.rec_cursor = off; off = align256b(off + rb * sizeof(int));
     L.rec_items = off;  off = align256b(off + ri);
+    L.maps = off;       off = align256b(off + mb);
     L.isnaps = off;     off = align256b(off + (size_t)std::max(d->num_snaps, 0) * sizeof(int));
     L.shot = off;       off = align256b(off + 2 * sizeof(ShotDev));
     L.total = off;
@@ -548,7 +551,7 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
     hipError_t e;
     if (is_wave(cfg)) {
         const int rc = wave_build_lists(d, cfg, istart, icur, ws + L.inj_items, rstart, rcur,
-                                        ws + L.rec_items, s);
+                                        ws + L.rec_items, ws + L.maps, s);
         if (rc) return rc;
     } else {
         const dim3 tb(256), tg((tiles + 255) / 256), lg((std::max(ng, nr) + 255) / 256);
@@ -608,6 +611,14 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
     x.inj_titems = iitems;
     x.rec_tstart = rstart;
     x.rec_titems = ritems;
+    x.imap_slot = x.imap = x.rmap_slot = x.rmap = nullptr;
+    if (is_wave(cfg)) {
+        const WaveMaps wm = wave_maps(d, cfg, ws + L.maps);
+        x.imap_slot = wm.imap_slot;
+        x.imap = wm.imap;
+        x.rmap_slot = wm.rmap_slot;
+        x.rmap = wm.rmap;
+    }
     x.snap_u = d->snap_u;
     x.snap_v = d->snap_v;
     x.snap_w = d->snap_w;

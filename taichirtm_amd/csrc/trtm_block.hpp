@@ -18,6 +18,9 @@ struct ShotDev {
     const int* rec_loc;
     float *rec_u, *rec_v, *rec_w;
     const int *inj_tstart, *inj_titems, *rec_tstart, *rec_titems;
+    // wavefront kernel: per-tile slot (-1: none) of a per-lane map of the
+    // single-entry injection cells (wavelet row) and of the receivers (index)
+    const int *imap_slot, *imap, *rmap_slot, *rmap;
     float *snap_u, *snap_v, *snap_w;
     const int* isnaps;
     float *image_u, *image_v, *image_w;
@@ -92,12 +95,18 @@ inline bool is_wave(const BlockCfg& c) { return c.nt == kWaveThreads; }
 // per-tile list sizes of a configuration (buckets are tiles, or tile rows for
 // the wavefront kernel)
 struct ListSizes {
-    size_t inj_buckets, rec_buckets, inj_item_bytes, rec_item_bytes;
+    size_t inj_buckets, rec_buckets, inj_item_bytes, rec_item_bytes, map_bytes;
+};
+
+// per-lane maps of the wavefront kernel, carved from one workspace region
+struct WaveMaps {
+    int *imap_slot, *imap, *rmap_slot, *rmap, *tmp0, *tmp1, *dup;
 };
 
 ListSizes wave_list_sizes(const trtm_step_desc* d, BlockCfg c);
+WaveMaps wave_maps(const trtm_step_desc* d, BlockCfg c, void* region);
 int wave_build_lists(const trtm_step_desc* d, BlockCfg c, int* istart, int* icur, void* iitems,
-                     int* rstart, int* rcur, void* ritems, hipStream_t s);
+                     int* rstart, int* rcur, void* ritems, void* maps, hipStream_t s);
 bool launch_wave(bool fwd, int systems, BlockCfg c, const KArgs& m, const ShotDev* shots,
                  int n_shots, int it0, int tiles_x, int tiles_z, hipStream_t s);
 void wave_configs(int systems, const BlockCfg** list, int* n);

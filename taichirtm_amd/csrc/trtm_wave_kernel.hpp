@@ -193,6 +193,25 @@ __global__ __launch_bounds__(kWaveThreads) void k_wave(KArgs m, const ShotDev* _
             if (uni(rstart[t]) < uni(rstart[t + 1])) rmask |= 1ull << t;
     }
 
+    // per-lane maps: the wavelet row of a single-entry injection cell in each
+    // region row, the receiver of each owned row (-1: none); loaded once
+    int io[RX], ro[TX];
+    uint64_t iomask = 0, romask = 0;
+    {
+        const int islot = n_groups > 0 ? uni(C(sh->imap_slot)[tile]) : -1;
+        const int rslot = n_rec > 0 ? uni(C(sh->rmap_slot)[tile]) : -1;
+#pragma unroll
+        for (int r = 0; r < RX; r++) {
+            io[r] = islot >= 0 ? G(sh->imap)[((size_t)islot * RX + r) * 64 + lane] : -1;
+            if (__builtin_amdgcn_ballot_w64(io[r] >= 0)) iomask |= 1ull << r;
+        }
+#pragma unroll
+        for (int t = 0; t < TX; t++) {
+            ro[t] = rslot >= 0 ? G(sh->rmap)[((size_t)rslot * TX + t) * 64 + lane] : -1;
+            if (__builtin_amdgcn_ballot_w64(ro[t] >= 0)) romask |= 1ull << t;
+        }
+    }
+
     const bool has_sm = m.sm != nullptr;
     const int save = sh->save, isnap = sh->isnap, num_snaps = sh->num_snaps;
     const int snap_base = sh->snap_base, cint = sh->check_interval;
@@ -373,6 +392,23 @@ __global__ __launch_bounds__(kWaveThreads) void k_wave(KArgs m, const ShotDev* _
             }
         }
 
+        // samples injected into the map cells this step (issued before the
+        // velocity update, which hides their latency)
+        float qU[RX], qW[RX], qV[RX];
+        if (iomask) {
+            const size_t col = (size_t)(FWD ? it : tb);
+#pragma unroll
+            for (int r = 0; r < RX; r++) {
+                if (!((iomask >> r) & 1ull)) continue;
+                const size_t o = (size_t)max(io[r], 0) * nt + col;
+                if (PSV) {
+                    qU[r] = G(sh->su)[o];
+                    qW[r] = G(sh->sw)[o];
+                }
+                if (SH) qV[r] = G(sh->sv)[o];
+            }
+        }
+
         // topography: the stresses the velocity update reads are masked in place
         if (has_sm) {
             const gptr<const float> smp = G(m.sm);
@@ -399,7 +435,27 @@ __global__ __launch_bounds__(kWaveThreads) void k_wave(KArgs m, const ShotDev* _
                 if (r >= s + v_lo && r < RX - s - v_hi) velocity(r, std::false_type{});
         }
 
-        // ---- injections (entries of a cell added in entry order)
+        // ---- injections: single-entry cells from the map ...
+        if (iomask) {
+#pragma unroll
+            for (int r = 0; r < RX; r++) {
+                if (!((iomask >> r) & 1ull)) continue;
+                const uint64_t mk = __builtin_amdgcn_ballot_w64(io[r] >= 0);
+                if (PSV) {
+                    const float x = FWD ? U[r] + (qU[r] * m.factor) * opaque(sC[0][r][lane])
+                                        : U[r] + qU[r];
+                    const float z = FWD ? W[r] + (qW[r] * m.factor) * opaque(sC[1][r][lane])
+                                        : W[r] + qW[r];
+                    U[r] = pick(mk, x, U[r]);
+                    W[r] = pick(mk, z, W[r]);
+                }
+                if (SH) {
+                    const float y = FWD ? V[r] + (qV[r] * m.factor) * cI[r] : V[r] + qV[r];
+                    V[r] = pick(mk, y, V[r]);
+                }
+            }
+        }
+        // ... and cells with several entries, added in entry order
         if (imask) {
             const cptr<WaveInj> items = C(reinterpret_cast<const WaveInj*>(sh->inj_titems));
             const cptr<int> order = C(sh->order);
@@ -509,7 +565,25 @@ __global__ __launch_bounds__(kWaveThreads) void k_wave(KArgs m, const ShotDev* _
             if (wc && lane == 0) atomicMax(sh->check_flags + chk, wc + (FWD ? 0 : 3));
         }
 
-        // ---- receivers of the owned rows, after the sponge
+        // ---- receivers of the owned rows, after the sponge: the first receiver
+        // of each cell from the map ...
+        if (romask) {
+            const size_t rrow = (size_t)(FWD ? it : tb) * n_rec;
+            const gptr<float> ru = G(sh->rec_u) + rrow, rv = G(sh->rec_v) + rrow;
+            const gptr<float> rw = G(sh->rec_w) + rrow;
+#pragma unroll
+            for (int t = 0; t < TX; t++) {
+                if (!((romask >> t) & 1ull)) continue;
+                if (ro[t] >= 0) {
+                    if (PSV) {
+                        ru[ro[t]] = U[K + t];
+                        rw[ro[t]] = W[K + t];
+                    }
+                    if (SH) rv[ro[t]] = V[K + t];
+                }
+            }
+        }
+        // ... and further receivers on the same cell
         if (rmask) {
             const cptr<WaveRec> ritems = C(reinterpret_cast<const WaveRec*>(sh->rec_titems));
             const size_t rrow = (size_t)(FWD ? it : tb) * n_rec;
