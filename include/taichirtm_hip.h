@@ -246,7 +246,17 @@ typedef struct trtm_fwi_desc {
     void *hist;
     /* gradient accumulators in coefficient space, REAL (nx, nz), += */
     void *g_lam, *g_mu, *g_mxz, *g_myx, *g_myz, *g_ir, *g_iru, *g_irw;
+    /* workspace int32 (2, nx, nz): source group / receiver group of each cell
+     * (-1: none), filled by trtm_fwi_prepare for this geometry */
+    int32_t *cell_map;
 } trtm_fwi_desc;
+
+/*
+ * Fills cell_map from the source and receiver plans (src_*, rec_*): the
+ * forward and adjoint steps read it to inject and record per cell, in the
+ * entry order of each group.  Call once per shot geometry.
+ */
+int trtm_fwi_prepare(const trtm_fwi_desc *d, void *stream);
 
 /*
  * Forward steps [it_begin, it_end) of the forward scheme (same expressions as

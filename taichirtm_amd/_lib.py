@@ -77,6 +77,7 @@ class FwiDesc(ctypes.Structure):
         ("hist", _p),
         ("g_lam", _p), ("g_mu", _p), ("g_mxz", _p), ("g_myx", _p), ("g_myz", _p), ("g_ir", _p),
         ("g_iru", _p), ("g_irw", _p),
+        ("cell_map", _p),
     ]
 
 
@@ -99,6 +100,7 @@ SIGNATURES = {
     "trtm_backward_steps": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _i32, _p]),
     "trtm_check_finite": (ctypes.c_int, [_i32, _i32, _p, _p, _p, _i32, _p, _p]),
     "trtm_prepare_medium_f64": (ctypes.c_int, [_i32, _i32] + [_p] * 10 + [_p]),
+    "trtm_fwi_prepare": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _p]),
     "trtm_fwi_forward": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _p]),
     "trtm_fwi_adjoint": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _p]),
     "trtm_fwi_chain_gradient": (ctypes.c_int, [_i32, _i32] + [_p] * 15 + [_p]),

@@ -37,43 +37,35 @@ struct AA {
     R *g_lam, *g_mu, *g_mxz, *g_myx, *g_myz, *g_ir, *g_iru, *g_irw;
     // history plane offsets (in cells) of sxx, szz, sxz, u, w, syx, syz, v
     long long h_sxx, h_szz, h_sxz, h_u, h_w, h_syx, h_syz, h_v;
+    const int *src_map, *rec_map;  // cell -> source / receiver group, -1 none
 };
 
-// data adjoint into the adjoint state at the receivers, one thread per cell
+// Per cell: the data adjoint of step n at the receivers (entries in order),
+// the sponge transpose + stress-update transpose (new velocity adjoint), the
+// gradients of lam, mu, mxz (P-SV) / myx, myz (SH), then the source-term
+// gradient of inv_rho_u / inv_rho_w / inv_rho (u += (wavelet * factor) *
+// inv_rho_u at a source cell).  The velocity adjoint of a cell is read and
+// written only by its own thread.
 template <class R, bool PSV, bool SH>
-__global__ void k_adj_inject(AA<R> a, int n) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.n_rec_groups) return;
-    const int e0 = a.rec_start[g], e1 = a.rec_start[g + 1];
-    const int k0 = a.rec_order[e0];
-    const size_t c = (size_t)a.rec_loc[2 * k0] * a.nz + a.rec_loc[2 * k0 + 1];
-    const size_t row = (size_t)n * a.n_rec;
-    if (PSV) {
-        R x = a.pu[c], z = a.pw[c];
-        for (int e = e0; e < e1; e++) {
-            const int l = a.rec_order[e];
-            x += a.res_u[row + l];
-            z += a.res_w[row + l];
-        }
-        a.pu[c] = x;
-        a.pw[c] = z;
-    }
-    if (SH) {
-        R y = a.pv[c];
-        for (int e = e0; e < e1; e++) y += a.res_v[row + a.rec_order[e]];
-        a.pv[c] = y;
-    }
-}
-
-// sponge transpose + stress-update transpose: new velocity adjoint, and the
-// gradients of lam, mu, mxz (P-SV) / myx, myz (SH)
-template <class R, bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_adj_stress(AA<R> a, const R* __restrict__ h) {
+__global__ __launch_bounds__(256) void k_adj_stress(AA<R> a, const R* __restrict__ h, int n) {
     const int j = blockIdx.x * kBlockZ + threadIdx.x;
     const int i = blockIdx.y * kBlockX + threadIdx.y;
     const int nx = a.nx, nz = a.nz;
     if (i >= nx || j >= nz) return;
     const size_t c = (size_t)i * nz + j;
+    R pu0 = PSV ? a.pu[c] : R(0), pw0 = PSV ? a.pw[c] : R(0), pv0 = SH ? a.pv[c] : R(0);
+    const int rg = a.rec_map ? a.rec_map[c] : -1;
+    if (rg >= 0) {
+        const size_t row = (size_t)n * a.n_rec;
+        for (int e = a.rec_start[rg]; e < a.rec_start[rg + 1]; e++) {
+            const int l = a.rec_order[e];
+            if (PSV) {
+                pu0 += a.res_u[row + l];
+                pw0 += a.res_w[row + l];
+            }
+            if (SH) pv0 += a.res_v[row + l];
+        }
+    }
     const R A = sponge_at_t<R>(i, j, nx, nz, a.fw, a.tab);
     auto inner = [&](int ii, int jj) { return ii >= 1 && ii < nx - 1 && jj >= 1 && jj < nz - 1; };
     auto sp = [&](const R* f, int ii, int jj) {
@@ -97,8 +89,8 @@ __global__ __launch_bounds__(256) void k_adj_stress(AA<R> a, const R* __restrict
             if (!inner(ii, jj)) return R(0);
             return a.mxz[(size_t)ii * nz + jj] * dt * sp(a.psxz, ii, jj);
         };
-        const R pu = A * a.pu[c] + a.idx * (qx(i - 1, j) - qx(i, j)) + a.idz * (ss(i, j - 1) - ss(i, j));
-        const R pw = A * a.pw[c] + a.idx * (ss(i - 1, j) - ss(i, j)) + a.idz * (qz(i, j - 1) - qz(i, j));
+        const R pu = A * pu0 + a.idx * (qx(i - 1, j) - qx(i, j)) + a.idz * (ss(i, j - 1) - ss(i, j));
+        const R pw = A * pw0 + a.idx * (ss(i - 1, j) - ss(i, j)) + a.idz * (qz(i, j - 1) - qz(i, j));
         a.pu[c] = pu;
         a.pw[c] = pw;
         if (inner(i, j)) {
@@ -121,7 +113,7 @@ __global__ __launch_bounds__(256) void k_adj_stress(AA<R> a, const R* __restrict
             if (!inner(ii, jj)) return R(0);
             return a.myz[(size_t)ii * nz + jj] * dt * sp(a.psyz, ii, jj);
         };
-        a.pv[c] = A * a.pv[c] + a.idx * (qy(i - 1, j) - qy(i, j)) + a.idz * (qv(i, j - 1) - qv(i, j));
+        a.pv[c] = A * pv0 + a.idx * (qy(i - 1, j) - qy(i, j)) + a.idz * (qv(i, j - 1) - qv(i, j));
         if (inner(i, j)) {
             const R* hv = h + a.h_v;
             const R vx = (hv[c + nz] - hv[c]) * a.idx, vz = (hv[c + 1] - hv[c]) * a.idz;
@@ -129,26 +121,18 @@ __global__ __launch_bounds__(256) void k_adj_stress(AA<R> a, const R* __restrict
             a.g_myz[c] += dt * A * a.psyz[c] * vz;
         }
     }
-}
-
-// source-term gradient: u += (wavelet * factor) * inv_rho_u at the source cell
-template <class R, bool PSV, bool SH>
-__global__ void k_adj_source(AA<R> a, int n) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.n_src_groups) return;
-    const int e0 = a.src_start[g], e1 = a.src_start[g + 1];
-    const int k0 = a.src_order[e0];
-    const size_t c = (size_t)a.src_loc[2 * k0] * a.nz + a.src_loc[2 * k0 + 1];
-    for (int e = e0; e < e1; e++) {
-        const size_t o = (size_t)a.src_order[e] * a.nt + n;
-        if (PSV) {
-            a.g_iru[c] += a.pu[c] * a.wav_u[o] * a.factor;
-            a.g_irw[c] += a.pw[c] * a.wav_w[o] * a.factor;
+    const int sg = a.src_map ? a.src_map[c] : -1;
+    if (sg >= 0) {
+        for (int e = a.src_start[sg]; e < a.src_start[sg + 1]; e++) {
+            const size_t o = (size_t)a.src_order[e] * a.nt + n;
+            if (PSV) {
+                a.g_iru[c] += a.pu[c] * a.wav_u[o] * a.factor;
+                a.g_irw[c] += a.pw[c] * a.wav_w[o] * a.factor;
+            }
+            if (SH) a.g_ir[c] += a.pv[c] * a.wav_v[o] * a.factor;
         }
-        if (SH) a.g_ir[c] += a.pv[c] * a.wav_v[o] * a.factor;
     }
 }
-
 // sponge transpose on the stresses + velocity-update transpose: new stress
 // adjoint, and the gradients of inv_rho_u, inv_rho_w (P-SV) / inv_rho (SH)
 template <class R, bool PSV, bool SH>
@@ -208,15 +192,184 @@ template <class R, bool PSV, bool SH>
 static hipError_t adjoint_step(const AA<R>& a, const R* h, int n, hipStream_t s) {
     const dim3 block(kBlockZ, kBlockX);
     const dim3 grid((a.nz + kBlockZ - 1) / kBlockZ, (a.nx + kBlockX - 1) / kBlockX);
-    if (a.n_rec_groups > 0)
-        hipLaunchKernelGGL((k_adj_inject<R, PSV, SH>), dim3((a.n_rec_groups + 255) / 256),
-                           dim3(256), 0, s, a, n);
-    hipLaunchKernelGGL((k_adj_stress<R, PSV, SH>), grid, block, 0, s, a, h);
-    if (a.n_src_groups > 0)
-        hipLaunchKernelGGL((k_adj_source<R, PSV, SH>), dim3((a.n_src_groups + 255) / 256),
-                           dim3(256), 0, s, a, n);
+    hipLaunchKernelGGL((k_adj_stress<R, PSV, SH>), grid, block, 0, s, a, h, n);
     hipLaunchKernelGGL((k_adj_velocity<R, PSV, SH>), grid, block, 0, s, a, h);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Forward step of the FWI pass (forward_modeling.py:539-551, flat free
+// surface), three launches: velocity update + source injection (+ history of
+// the pre-step stresses and the injected velocities), stress update + stress
+// sponge, velocity sponge + recording.  Same expressions, in the same order, as
+// the reference-shaped kernels.
+
+template <class R>
+struct FF {
+    int nx, nz, nt, fw, n_rec;
+    R dt, idx, idz, factor;
+    const R* tab;
+    const R *lam, *mu, *mxz, *myx, *myz, *ir, *iru, *irw;
+    R *u, *v, *w, *sxx, *szz, *sxz, *syx, *syz;
+    const int *src_map, *rec_map;
+    const int *src_order, *src_start, *rec_order, *rec_start;
+    const R *wav_u, *wav_v, *wav_w;
+    R *rec_u, *rec_v, *rec_w;
+    R* h;  // this step's history planes or nullptr
+    long long h_sxx, h_szz, h_sxz, h_u, h_w, h_syx, h_syz, h_v;
+};
+
+template <class R, bool PSV, bool SH>
+__global__ __launch_bounds__(256) void k_ff_velocity(FF<R> a, int it) {
+    const int j = blockIdx.x * kBlockZ + threadIdx.x;
+    const int i = blockIdx.y * kBlockX + threadIdx.y;
+    const int nx = a.nx, nz = a.nz;
+    if (i >= nx || j >= nz) return;
+    const size_t c = (size_t)i * nz + j;
+    const bool inner = i >= 1 && i < nx - 1 && j >= 1 && j < nz - 1;
+    const int sg = a.src_map ? a.src_map[c] : -1;
+    if (PSV) {
+        if (a.h) {
+            a.h[a.h_sxx + c] = a.sxx[c];
+            a.h[a.h_szz + c] = a.szz[c];
+            a.h[a.h_sxz + c] = a.sxz[c];
+        }
+        R u = a.u[c], w = a.w[c];
+        if (inner) {
+            const R sxx_x = (a.sxx[c] - a.sxx[c - nz]) * a.idx;
+            const R szz_z = (a.szz[c] - a.szz[c - 1]) * a.idz;
+            const R sxz_x = (a.sxz[c] - a.sxz[c - nz]) * a.idx;
+            const R sxz_z = (a.sxz[c] - a.sxz[c - 1]) * a.idz;
+            u = u + ((sxx_x + sxz_z) * a.dt) * a.iru[c];
+            w = w + ((sxz_x + szz_z) * a.dt) * a.irw[c];
+        }
+        if (sg >= 0)
+            for (int e = a.src_start[sg]; e < a.src_start[sg + 1]; e++) {
+                const size_t o = (size_t)a.src_order[e] * a.nt + it;
+                u = u + (a.wav_u[o] * a.factor) * a.iru[c];
+                w = w + (a.wav_w[o] * a.factor) * a.irw[c];
+            }
+        a.u[c] = u;
+        a.w[c] = w;
+        if (a.h) {
+            a.h[a.h_u + c] = u;
+            a.h[a.h_w + c] = w;
+        }
+    }
+    if (SH) {
+        if (a.h) {
+            a.h[a.h_syx + c] = a.syx[c];
+            a.h[a.h_syz + c] = a.syz[c];
+        }
+        R v = a.v[c];
+        if (inner) {
+            const R syx_x = (a.syx[c] - a.syx[c - nz]) * a.idx;
+            const R syz_z = (a.syz[c] - a.syz[c - 1]) * a.idz;
+            v = v + ((syx_x + syz_z) * a.dt) * a.ir[c];
+        }
+        if (sg >= 0)
+            for (int e = a.src_start[sg]; e < a.src_start[sg + 1]; e++)
+                v = v + (a.wav_v[(size_t)a.src_order[e] * a.nt + it] * a.factor) * a.ir[c];
+        a.v[c] = v;
+        if (a.h) a.h[a.h_v + c] = v;
+    }
+}
+
+template <class R, bool PSV, bool SH>
+__global__ __launch_bounds__(256) void k_ff_stress(FF<R> a) {
+    const int j = blockIdx.x * kBlockZ + threadIdx.x;
+    const int i = blockIdx.y * kBlockX + threadIdx.y;
+    const int nx = a.nx, nz = a.nz;
+    if (i >= nx || j >= nz) return;
+    const size_t c = (size_t)i * nz + j;
+    const R A = sponge_at_t<R>(i, j, nx, nz, a.fw, a.tab);
+    const bool inner = i >= 1 && i < nx - 1 && j >= 1 && j < nz - 1;
+    const R sdt = a.dt;
+    if (PSV) {
+        R sxx = a.sxx[c], szz = a.szz[c], sxz = a.sxz[c];
+        if (inner) {
+            const R u_x = (a.u[c + nz] - a.u[c]) * a.idx;
+            const R u_z = (a.u[c + 1] - a.u[c]) * a.idz;
+            const R w_x = (a.w[c + nz] - a.w[c]) * a.idx;
+            const R w_z = (a.w[c + 1] - a.w[c]) * a.idz;
+            const R div_uw = u_x + w_z;
+            const R lam = a.lam[c], mu = a.mu[c];
+            sxx = sxx + sdt * (lam * div_uw + (R(2) * mu) * u_x);
+            szz = szz + sdt * (lam * div_uw + (R(2) * mu) * w_z);
+            sxz = sxz + (sdt * a.mxz[c]) * (u_z + w_x);
+        }
+        a.sxx[c] = sxx * A;
+        a.szz[c] = szz * A;
+        a.sxz[c] = sxz * A;
+    }
+    if (SH) {
+        R syx = a.syx[c], syz = a.syz[c];
+        if (inner) {
+            const R v_x = (a.v[c + nz] - a.v[c]) * a.idx;
+            const R v_z = (a.v[c + 1] - a.v[c]) * a.idz;
+            syx = syx + (sdt * a.myx[c]) * v_x;
+            syz = syz + (sdt * a.myz[c]) * v_z;
+        }
+        a.syx[c] = syx * A;
+        a.syz[c] = syz * A;
+    }
+}
+
+template <class R, bool PSV, bool SH>
+__global__ __launch_bounds__(256) void k_ff_post(FF<R> a, int it) {
+    const int j = blockIdx.x * kBlockZ + threadIdx.x;
+    const int i = blockIdx.y * kBlockX + threadIdx.y;
+    const int nx = a.nx, nz = a.nz;
+    if (i >= nx || j >= nz) return;
+    const size_t c = (size_t)i * nz + j;
+    const R A = sponge_at_t<R>(i, j, nx, nz, a.fw, a.tab);
+    R u = 0, w = 0, v = 0;
+    if (PSV) {
+        u = a.u[c] * A;
+        w = a.w[c] * A;
+        a.u[c] = u;
+        a.w[c] = w;
+    }
+    if (SH) {
+        v = a.v[c] * A;
+        a.v[c] = v;
+    }
+    const int rg = a.rec_map ? a.rec_map[c] : -1;
+    if (rg >= 0) {
+        const size_t row = (size_t)it * a.n_rec;
+        for (int e = a.rec_start[rg]; e < a.rec_start[rg + 1]; e++) {
+            const int l = a.rec_order[e];
+            if (PSV && a.rec_u) {
+                a.rec_u[row + l] = u;
+                a.rec_w[row + l] = w;
+            }
+            if (SH && a.rec_v) a.rec_v[row + l] = v;
+        }
+    }
+}
+
+template <class R, bool PSV, bool SH>
+static hipError_t forward_step(const FF<R>& a, int it, hipStream_t s) {
+    const dim3 block(kBlockZ, kBlockX);
+    const dim3 grid((a.nz + kBlockZ - 1) / kBlockZ, (a.nx + kBlockX - 1) / kBlockX);
+    hipLaunchKernelGGL((k_ff_velocity<R, PSV, SH>), grid, block, 0, s, a, it);
+    hipLaunchKernelGGL((k_ff_stress<R, PSV, SH>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_ff_post<R, PSV, SH>), grid, block, 0, s, a, it);
+    return hipGetLastError();
+}
+
+// cell -> group of a plan (one thread per group)
+__global__ void k_group_map(int n_groups, int nz, const int* loc, const int* order,
+                            const int* start, int* map) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_groups) return;
+    const int k0 = order[start[g]];
+    map[(size_t)loc[2 * k0] * nz + loc[2 * k0 + 1]] = g;
+}
+
+__global__ void k_fill_i32(int* p, size_t n, int value) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = value;
 }
 
 // coefficient space -> (lam, mu, rho) -> (vp, vs, rho), double precision
@@ -327,6 +480,12 @@ static int check_fwi(const trtm_fwi_desc* d, int it_begin, int it_end) {
     if (!d->sponge_table || !d->lam || !d->mu || !d->mxz || !d->inv_rho_u || !d->inv_rho_w ||
         !d->myx || !d->myz || !d->inv_rho)
         return fail(TRTM_ERR_ARG, "medium pointer is NULL");
+    if ((d->n_src_groups > 0 || d->n_rec_groups > 0) && !d->cell_map)
+        return fail(TRTM_ERR_ARG, "cell_map is NULL (call trtm_fwi_prepare)");
+    if (d->n_src_groups > 0 && (!d->src_order || !d->src_group_start))
+        return fail(TRTM_ERR_ARG, "source plan is NULL");
+    if ((d->rec_u || d->rec_v) && d->n_rec > 0 && (d->n_rec_groups <= 0 || !d->rec_order || !d->rec_group_start))
+        return fail(TRTM_ERR_ARG, "recording needs the receiver plan");
     return TRTM_OK;
 }
 
@@ -334,63 +493,49 @@ static int hist_fields(int systems) {
     return ((systems & TRTM_PSV) ? 5 : 0) + ((systems & TRTM_SH) ? 3 : 0);
 }
 
+static void hist_offsets(int systems, long long n, long long* o) {
+    // sxx, szz, sxz, u, w, syx, syz, v
+    int f = 0;
+    for (int k = 0; k < 8; k++) o[k] = 0;
+    if (systems & TRTM_PSV)
+        for (int k = 0; k < 5; k++) o[k] = (f++) * n;
+    if (systems & TRTM_SH)
+        for (int k = 5; k < 8; k++) o[k] = (f++) * n;
+}
+
 template <class R>
-static KA<R> fwi_kargs(const trtm_fwi_desc* d) {
-    KA<R> a;
-    a.nx = d->nx; a.nz = d->nz; a.nt = d->nt; a.fw = d->fw;
+static int fwi_forward_t(const trtm_fwi_desc* d, int it_begin, int it_end, hipStream_t s) {
+    FF<R> a;
+    a.nx = d->nx; a.nz = d->nz; a.nt = d->nt; a.fw = d->fw; a.n_rec = d->n_rec;
     a.dt = (R)d->dt; a.idx = (R)d->inv_dx; a.idz = (R)d->inv_dz; a.factor = (R)d->src_factor;
     a.tab = (const R*)d->sponge_table;
     a.lam = (const R*)d->lam; a.mu = (const R*)d->mu; a.mxz = (const R*)d->mxz;
     a.myx = (const R*)d->myx; a.myz = (const R*)d->myz; a.ir = (const R*)d->inv_rho;
     a.iru = (const R*)d->inv_rho_u; a.irw = (const R*)d->inv_rho_w;
-    a.sm = nullptr;
     a.u = (R*)d->u; a.v = (R*)d->v; a.w = (R*)d->w;
     a.sxx = (R*)d->sxx; a.szz = (R*)d->szz; a.sxz = (R*)d->sxz;
     a.syx = (R*)d->syx; a.syz = (R*)d->syz;
-    return a;
-}
-
-template <class R>
-static int fwi_forward_t(const trtm_fwi_desc* d, int it_begin, int it_end, hipStream_t s) {
-    KA<R> a = fwi_kargs<R>(d);
-    IA<R> q;
-    q.n_groups = d->n_src_groups; q.loc = d->src_loc; q.order = d->src_order;
-    q.start = d->src_group_start;
-    q.su = (const R*)d->wav_u; q.sv = (const R*)d->wav_v; q.sw = (const R*)d->wav_w;
-    SIO<R> p;
-    p.snap_slot = p.image_slot = p.check_slot = -1;
-    p.check_base = 0;
-    p.snap_u = p.snap_v = p.snap_w = p.image_u = p.image_v = p.image_w = nullptr;
-    p.check_flags = nullptr;
-    p.n_rec = (d->rec_u || d->rec_v) ? d->n_rec : 0;
-    p.rec_loc = d->rec_loc;
-    p.rec_u = (R*)d->rec_u; p.rec_v = (R*)d->rec_v; p.rec_w = (R*)d->rec_w;
+    const long long n = (long long)d->nx * d->nz;
+    a.src_map = d->n_src_groups > 0 ? d->cell_map : nullptr;
+    a.rec_map = (d->n_rec_groups > 0 && (d->rec_u || d->rec_v)) ? d->cell_map + n : nullptr;
+    a.src_order = d->src_order; a.src_start = d->src_group_start;
+    a.rec_order = d->rec_order; a.rec_start = d->rec_group_start;
+    a.wav_u = (const R*)d->wav_u; a.wav_v = (const R*)d->wav_v; a.wav_w = (const R*)d->wav_w;
+    a.rec_u = (R*)d->rec_u; a.rec_v = (R*)d->rec_v; a.rec_w = (R*)d->rec_w;
+    long long o[8];
+    hist_offsets(d->systems, n, o);
+    a.h_sxx = o[0]; a.h_szz = o[1]; a.h_sxz = o[2]; a.h_u = o[3]; a.h_w = o[4];
+    a.h_syx = o[5]; a.h_syz = o[6]; a.h_v = o[7];
     const bool psv = d->systems & TRTM_PSV, sh = d->systems & TRTM_SH;
-    const size_t n = (size_t)d->nx * d->nz;
     const int nf = hist_fields(d->systems);
     for (int it = it_begin; it < it_end; it++) {
-        q.col = it;
-        p.rec_row = it;
-        R* hs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-        R* hv[3] = {nullptr, nullptr, nullptr};
         const int slot = it - d->hist_base;
-        if (d->hist && slot >= 0 && slot < d->hist_len) {
-            R* base = (R*)d->hist + (size_t)slot * nf * n;
-            int f = 0;
-            if (psv) {
-                hs[0] = base + (f++) * n;  // sxx
-                hs[1] = base + (f++) * n;  // szz
-                hs[2] = base + (f++) * n;  // sxz
-                hv[0] = base + (f++) * n;  // u
-                hv[1] = base + (f++) * n;  // w
-            }
-            if (sh) {
-                hs[3] = base + (f++) * n;  // syx
-                hs[4] = base + (f++) * n;  // syz
-                hv[2] = base + (f++) * n;  // v
-            }
-        }
-        hipError_t e = launch_hist_step<R>(d->systems, a, q, p, hs, hv, s);
+        a.h = (d->hist && slot >= 0 && slot < d->hist_len) ? (R*)d->hist + (size_t)slot * nf * n
+                                                           : nullptr;
+        hipError_t e;
+        if (psv && sh) e = forward_step<R, true, true>(a, it, s);
+        else if (psv) e = forward_step<R, true, false>(a, it, s);
+        else e = forward_step<R, false, true>(a, it, s);
         if (e != hipSuccess) return fail(TRTM_ERR_HIP, "fwi forward step %d: %s", it, hipGetErrorString(e));
     }
     return TRTM_OK;
@@ -418,16 +563,13 @@ static int fwi_adjoint_t(const trtm_fwi_desc* d, int it_begin, int it_end, hipSt
     a.g_iru = (R*)d->g_iru; a.g_irw = (R*)d->g_irw;
     const bool psv = d->systems & TRTM_PSV, sh = d->systems & TRTM_SH;
     const long long n = (long long)d->nx * d->nz;
-    int f = 0;
-    a.h_sxx = a.h_szz = a.h_sxz = a.h_u = a.h_w = a.h_syx = a.h_syz = a.h_v = 0;
-    if (psv) {
-        a.h_sxx = (f++) * n; a.h_szz = (f++) * n; a.h_sxz = (f++) * n;
-        a.h_u = (f++) * n; a.h_w = (f++) * n;
-    }
-    if (sh) {
-        a.h_syx = (f++) * n; a.h_syz = (f++) * n; a.h_v = (f++) * n;
-    }
-    const int nf = f;
+    long long o[8];
+    hist_offsets(d->systems, n, o);
+    a.h_sxx = o[0]; a.h_szz = o[1]; a.h_sxz = o[2]; a.h_u = o[3]; a.h_w = o[4];
+    a.h_syx = o[5]; a.h_syz = o[6]; a.h_v = o[7];
+    a.src_map = d->n_src_groups > 0 ? d->cell_map : nullptr;
+    a.rec_map = d->n_rec_groups > 0 ? d->cell_map + n : nullptr;
+    const int nf = hist_fields(d->systems);
     for (int it = it_end - 1; it >= it_begin; it--) {
         const int slot = it - d->hist_base;
         if (slot < 0 || slot >= d->hist_len)
@@ -461,6 +603,30 @@ int trtm_prepare_medium_f64(int32_t nx, int32_t nz, const double* mu, const doub
                        mxz, rho_u, rho_w, inv_rho, inv_rho_u, inv_rho_w);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TRTM_OK : fail(TRTM_ERR_HIP, "prepare f64: %s", hipGetErrorString(e));
+}
+
+int trtm_fwi_prepare(const trtm_fwi_desc* d, void* stream) {
+    if (!d || d->struct_size != (int32_t)sizeof(trtm_fwi_desc))
+        return fail(TRTM_ERR_ARG, "descriptor is NULL or of another ABI");
+    if (d->nx < 2 || d->nz < 2) return fail(TRTM_ERR_ARG, "bad grid");
+    if (!d->cell_map) return fail(TRTM_ERR_ARG, "cell_map is NULL");
+    if ((d->n_src_groups > 0 && (!d->src_loc || !d->src_order || !d->src_group_start)) ||
+        (d->n_rec_groups > 0 && (!d->rec_loc || !d->rec_order || !d->rec_group_start)))
+        return fail(TRTM_ERR_ARG, "plan pointer is NULL");
+    const size_t n = (size_t)d->nx * d->nz;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((2 * n + 255) / 256)), dim3(256), 0, s,
+                       d->cell_map, 2 * n, -1);
+    if (d->n_src_groups > 0)
+        hipLaunchKernelGGL(k_group_map, dim3((d->n_src_groups + 255) / 256), dim3(256), 0, s,
+                           d->n_src_groups, d->nz, d->src_loc, d->src_order, d->src_group_start,
+                           d->cell_map);
+    if (d->n_rec_groups > 0)
+        hipLaunchKernelGGL(k_group_map, dim3((d->n_rec_groups + 255) / 256), dim3(256), 0, s,
+                           d->n_rec_groups, d->nz, d->rec_loc, d->rec_order, d->rec_group_start,
+                           d->cell_map + n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TRTM_OK : fail(TRTM_ERR_HIP, "fwi prepare: %s", hipGetErrorString(e));
 }
 
 int trtm_fwi_forward(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, void* stream) {

@@ -51,8 +51,4 @@ __device__ __forceinline__ bool bad_value(R x) {
     return (x != x) || (fabs((double)x) > (double)1e30f);
 }
 
-template <class R>
-hipError_t launch_hist_step(int systems, const KA<R>& a, const IA<R>& q, const SIO<R>& p,
-                            R* const* hs, R* const* hv, hipStream_t s);
-
 }  // namespace trtm

@@ -231,51 +231,6 @@ static hipError_t simple_step(const KA<R>& a, const IA<R>& q, const SIO<R>& p, h
     return hipGetLastError();
 }
 
-// Forward step with the FWI history of the step: the stresses the velocity
-// update reads (hs: sxx, szz, sxz, syx, syz) and the velocities after the
-// update and the injection (hv: u, w, v); NULL entries are skipped.
-template <class R, bool PSV, bool SH>
-static hipError_t hist_step(const KA<R>& a, const IA<R>& q, const SIO<R>& p, R* const* hs,
-                            R* const* hv, hipStream_t s) {
-    const dim3 block(kBlockZ, kBlockX);
-    const dim3 grid((a.nz + kBlockZ - 1) / kBlockZ, (a.nx + kBlockX - 1) / kBlockX);
-    const size_t bytes = (size_t)a.nx * a.nz * sizeof(R);
-    if (a.sm) hipLaunchKernelGGL((k_surface<R, PSV, SH>), grid, block, 0, s, a);
-    const R* sig[5] = {a.sxx, a.szz, a.sxz, a.syx, a.syz};
-    for (int k = 0; k < 5; k++)
-        if (hs && hs[k]) hipMemcpyAsync(hs[k], sig[k], bytes, hipMemcpyDeviceToDevice, s);
-    hipLaunchKernelGGL((k_velocity<R, true, PSV, SH>), grid, block, 0, s, a);
-    if (q.n_groups > 0)
-        hipLaunchKernelGGL((k_inject<R, true, PSV, SH>), dim3((q.n_groups + 255) / 256), dim3(256),
-                           0, s, a, q);
-    const R* vel[3] = {a.u, a.w, a.v};
-    for (int k = 0; k < 3; k++)
-        if (hv && hv[k]) hipMemcpyAsync(hv[k], vel[k], bytes, hipMemcpyDeviceToDevice, s);
-    hipLaunchKernelGGL((k_stress<R, true, PSV, SH>), grid, block, 0, s, a);
-    hipLaunchKernelGGL((k_post<R, PSV, SH>), grid, block, 0, s, a, p);
-    if (p.n_rec > 0 && p.rec_row >= 0)
-        hipLaunchKernelGGL((k_record<R, PSV, SH>), dim3((p.n_rec + 255) / 256), dim3(256), 0, s,
-                           a, p);
-    return hipGetLastError();
-}
-
-template <class R>
-hipError_t launch_hist_step(int systems, const KA<R>& a, const IA<R>& q, const SIO<R>& p,
-                            R* const* hs, R* const* hv, hipStream_t s) {
-    const bool psv = systems & TRTM_PSV, sh = systems & TRTM_SH;
-    if (psv && sh) return hist_step<R, true, true>(a, q, p, hs, hv, s);
-    if (psv) return hist_step<R, true, false>(a, q, p, hs, hv, s);
-    if (sh) return hist_step<R, false, true>(a, q, p, hs, hv, s);
-    return hipSuccess;
-}
-
-template hipError_t launch_hist_step<float>(int, const KA<float>&, const IA<float>&,
-                                            const SIO<float>&, float* const*, float* const*,
-                                            hipStream_t);
-template hipError_t launch_hist_step<double>(int, const KA<double>&, const IA<double>&,
-                                             const SIO<double>&, double* const*, double* const*,
-                                             hipStream_t);
-
 template <class R>
 hipError_t launch_simple_step_t(bool fwd, int systems, const KA<R>& a, const IA<R>& q,
                                 const SIO<R>& p, hipStream_t s) {
@@ -292,8 +247,6 @@ hipError_t launch_simple_step_t(bool fwd, int systems, const KA<R>& a, const IA<
     return hipSuccess;
 }
 
-template hipError_t launch_simple_step_t<double>(bool, int, const KA<double>&, const IA<double>&,
-                                                 const SIO<double>&, hipStream_t);
 
 // float entry used by the product path: the KArgs / InjArgs / StepIO layouts
 // are the float instances of KA / IA / SIO

@@ -94,7 +94,18 @@ class ElasticFWI:
             wav = [torch.as_tensor(np.asarray(w), dtype=self.dtype, device=dev).reshape(len(src[0]), -1)
                    .contiguous() for w in s.wavelets]
             self.geo.append({"src": src, "rec": rec, "wav": wav, "nt": s.nt,
-                             "nr": len(rec[0]), "ns": len(src[0])})
+                             "nr": len(rec[0]), "ns": len(src[0]),
+                             "map": torch.empty(2 * nx * nz, dtype=torch.int32, device=dev)})
+        for g in self.geo:  # cell -> source / receiver group
+            d = _lib.FwiDesc()
+            d.struct_size = ctypes.sizeof(_lib.FwiDesc)
+            d.nx, d.nz = self.nx, self.nz
+            loc, order, start, ng = g["src"]
+            d.n_src_groups, d.src_loc, d.src_order, d.src_group_start = ng, _ptr(loc), _ptr(order), _ptr(start)
+            loc, order, start, ng = g["rec"]
+            d.n_rec_groups, d.rec_loc, d.rec_order, d.rec_group_start = ng, _ptr(loc), _ptr(order), _ptr(start)
+            d.cell_map = _ptr(g["map"])
+            _lib.call("trtm_fwi_prepare", d, stream_handle())
         self.checkpoint_every = checkpoint_every
         self._ck = None
 
@@ -137,6 +148,7 @@ class ElasticFWI:
         loc, order, start, ng = g["rec"]
         d.n_rec, d.n_rec_groups = len(loc), ng
         d.rec_loc, d.rec_order, d.rec_group_start = _ptr(loc), _ptr(order), _ptr(start)
+        d.cell_map = _ptr(g["map"])
         return d
 
     def _systems(self):
