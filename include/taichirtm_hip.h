@@ -51,6 +51,15 @@ extern "C" {
 #define TRTM_KERNEL_BLOCKED 3 /* one launch per block_steps time steps (temporal blocking) */
 #define TRTM_KERNEL_WAVE 4    /* temporal blocking, one wavefront per tile, state in registers */
 
+/*
+ * Compiled shapes of a temporally blocked kernel family (TRTM_KERNEL_BLOCKED or
+ * TRTM_KERNEL_WAVE) for the given wave systems: writes up to max_n rows of
+ * (tile_x, tile_z, threads per workgroup, block_steps) into out and returns
+ * the number of shapes (no counterpart in the reference: the autotuner's
+ * search space).
+ */
+int trtm_block_configs(int32_t systems, int32_t kernel, int32_t *out, int32_t max_n);
+
 /* ABI version compiled into the library (== TRTM_ABI_VERSION). */
 int trtm_abi_version(void);
 

@@ -92,6 +92,7 @@ SIGNATURES = {
     "trtm_workspace_bytes": (ctypes.c_int64, [ctypes.POINTER(StepDesc)]),
     "trtm_batch_workspace_bytes": (ctypes.c_int64, [_i32]),
     "trtm_block_config": (ctypes.c_int, [ctypes.POINTER(StepDesc), _p]),
+    "trtm_block_configs": (ctypes.c_int, [_i32, _i32, _p, _i32]),
     "trtm_forward_steps_batch": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _p, ctypes.c_int64,
                                                 _i32, _i32, _i32, _p]),
     "trtm_backward_steps_batch": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _p,

@@ -21,7 +21,9 @@ STATE = ("u", "v", "w", "sxx", "szz", "sxz", "syx", "syz")
 PSV_STATE = ("u", "w", "sxx", "szz", "sxz")
 SH_STATE = ("v", "syx", "syz")
 KERNELS = {"auto": _lib.KERNEL_AUTO, "simple": _lib.KERNEL_SIMPLE, "fused": _lib.KERNEL_FUSED,
-           "blocked": _lib.KERNEL_BLOCKED, "wave": _lib.KERNEL_WAVE}
+           "blocked": _lib.KERNEL_BLOCKED, "wave": _lib.KERNEL_WAVE,
+           # shape and family chosen by the autotuner on the first run
+           "tuned": _lib.KERNEL_AUTO}
 
 
 def _ptr(t):
@@ -82,6 +84,14 @@ class Medium:
         self.factor32 = np.float32(dt * self.dx * self.dz)
 
 
+def set_shape(d, shape):
+    """Kernel family and block shape (family, tile_x, tile_z, block_steps) into a
+    step descriptor."""
+    family, tx, tz, k = shape
+    d.kernel = KERNELS[family]
+    d.tile_x, d.tile_z, d.block_steps = tx, tz, k
+
+
 class Propagator:
     """Medium, sponge and geometry of one forward or backward propagator."""
 
@@ -105,6 +115,7 @@ class Propagator:
         self.tile_x = int(kwargs.get("tile_x", 0))
         self.block_steps = int(kwargs.get("block_steps", 0))
         self.tile_z = int(kwargs.get("tile_z", 0))
+        self._tuned = None  # (family, tile_x, tile_z, block_steps) once tuned
         nx, nz, nt = self.nx, self.nz, self.nt
         if nx < 2 or nz < 2 or nt < 1:
             raise ValueError("grid must be at least 2 x 2 with nt >= 1")
@@ -246,6 +257,8 @@ class Propagator:
         d.systems = self.systems
         d.kernel = KERNELS[self.kernel]
         d.tile_x, d.block_steps, d.tile_z = self.tile_x, self.block_steps, self.tile_z
+        if self._tuned is not None:
+            set_shape(d, self._tuned)
         d.sponge_table = _ptr(self._sponge_table)
         c = self._coef
         d.lam, d.mu = _ptr(self.lam), _ptr(self.mu)
@@ -264,6 +277,26 @@ class Propagator:
         d.n_rec = len(rl)
         d.rec_loc = _ptr(self._rec_dev_loc()) if len(rl) else None
         return d
+
+    def _autotune(self, d, direction, launch):
+        """kernel="tuned": choose the blocked kernel family and shape by timing
+        every compiled shape on this propagator's own buffers (``launch(d,
+        steps)`` runs steps [0, steps)), then zero the state and every output
+        the trials wrote.  The caller clears its check flags."""
+        if self.kernel != "tuned" or self._tuned is not None:
+            return
+        from . import autotune
+        key = (direction, self.nx, self.nz, self.absorbing_frame, self.systems,
+               int(d.num_snaps > 0), self.surface_matrix is not None, self.num_receivers > 0)
+        window = min(self.nt, 64)
+
+        def trial(shape):
+            set_shape(d, shape)
+            launch(d, window)
+
+        self._tuned = autotune.tune(key, self.systems, trial)
+        set_shape(d, self._tuned)
+        self.reset_state()
 
     def block_config(self):
         """(tile rows, tile columns, threads, steps per launch) of the blocked

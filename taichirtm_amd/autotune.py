@@ -1,0 +1,67 @@
+"""Autotuner of the temporally blocked step kernels.
+
+Every compiled shape of the two temporally blocked kernel families (LDS
+tiles, ``"blocked"``; one wavefront per tile with the state in registers,
+``"wave"``) gives bit-identical results, so the choice only changes the speed.
+For a problem, ``tune`` runs a short window of time steps with every compiled
+shape (tile rows, tile columns, steps per launch) on the problem's own
+buffers, timed with events on the launch stream, and keeps the fastest.  The
+choice is cached per problem key for the life of the process; the caller
+restores whatever the trial steps wrote.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Callable, Dict, List, Tuple
+
+import torch
+
+from . import _lib
+
+Shape = Tuple[str, int, int, int]  # kernel family, tile_x, tile_z, block_steps
+
+FAMILIES = (("blocked", _lib.KERNEL_BLOCKED), ("wave", _lib.KERNEL_WAVE))
+_CACHE: Dict[tuple, Shape] = {}
+TIMINGS: Dict[tuple, List[Tuple[Shape, float]]] = {}  # last tuning of each key, ms per window
+
+
+def candidates(systems: int) -> List[Shape]:
+    """Every compiled shape of both families for the given wave systems."""
+    out = []
+    for name, kid in FAMILIES:
+        buf = (ctypes.c_int32 * (4 * 64))()
+        n = _lib.call("trtm_block_configs", systems, kid, buf, 64)
+        for i in range(min(n, 64)):
+            tx, tz, _threads, k = buf[4 * i:4 * i + 4]
+            out.append((name, int(tx), int(tz), int(k)))
+    return out
+
+
+def tune(key: tuple, systems: int, trial: Callable[[Shape], None], repeats: int = 2) -> Shape:
+    """Fastest shape for ``key``; ``trial(shape)`` runs the timing window once
+    with that shape on the current stream."""
+    if key in _CACHE:
+        return _CACHE[key]
+    times = []
+    for shape in candidates(systems):
+        trial(shape)  # first call: tile lists, caches
+        best = None
+        for _ in range(repeats):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            trial(shape)
+            e1.record()
+            e1.synchronize()
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None else min(best, ms)
+        times.append((shape, best))
+    times.sort(key=lambda x: x[1])
+    TIMINGS[key] = times
+    _CACHE[key] = times[0][0]
+    return times[0][0]
+
+
+def clear():
+    _CACHE.clear()
+    TIMINGS.clear()

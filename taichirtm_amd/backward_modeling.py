@@ -144,6 +144,9 @@ class BackwardModeling(Propagator):
             d.image_u, d.image_v, d.image_w = (_ptr(self._img.get(c)) for c in ("u", "v", "w"))
         d.rec_u, d.rec_v, d.rec_w = (_ptr(self._syn.get(c)) for c in ("u", "v", "w"))
         ws = self._workspace(d)
+        self._autotune(d, "backward",
+                       lambda dd, n: _lib.call("trtm_backward_steps", dd, 0, n, stream_handle()))
+        flags.zero_()
         if imaging and sysmask != self.systems:
             # a propagating system without forward data: its image stays 0,
             # run it without imaging in a second descriptor

@@ -87,8 +87,9 @@ class BatchRTM:
     shots, the snapshot interval isnap, imaging ("snapshot" or "checkpoint"),
     checkpoint_every (segment length; default from the memory model),
     stability_check_interval, the temporally blocked kernel ("blocked": LDS
-    tiles, or "wave": one wavefront per tile with the state in registers) and
-    its tile_x / tile_z / block_steps (0 = library default).
+    tiles, "wave": one wavefront per tile with the state in registers, or
+    "tuned": family and shape chosen by the autotuner on this batch) and its
+    tile_x / tile_z / block_steps (0 = library default).
     """
 
     def __init__(self, nx, nz, dx, dz, fs, mu_field, lam_field, rho_field, absorbing_frame,
@@ -97,8 +98,8 @@ class BatchRTM:
                  block_steps=0, kernel="blocked"):
         if imaging not in ("snapshot", "checkpoint"):
             raise ValueError("imaging must be 'snapshot' or 'checkpoint'")
-        if kernel not in ("blocked", "wave"):
-            raise ValueError("kernel must be 'blocked' or 'wave'")
+        if kernel not in ("blocked", "wave", "tuned"):
+            raise ValueError("kernel must be 'blocked', 'wave' or 'tuned'")
         if not shots:
             raise ValueError("no shots")
         self.medium = Medium(nx, nz, dx, dz, fs, absorbing_frame, mu_field, lam_field, rho_field,
@@ -110,6 +111,7 @@ class BatchRTM:
         self.checkpoint_every = checkpoint_every
         self.sci = int(stability_check_interval)
         self.block = dict(tile_x=tile_x, tile_z=tile_z, block_steps=block_steps, kernel=kernel)
+        self.tuned = {}
         self.codes = []
         self.fw: List[ForwardModeling] = []
         self.bw: List[BackwardModeling] = []
@@ -123,6 +125,29 @@ class BatchRTM:
                     fs=self.medium.fs, absorbing_frame=self.medium.fw, medium=self.medium,
                     src_loc_field=src, recv_loc_field=rec, num_sources=len(src),
                     num_receivers=len(rec), **self.block)
+
+    def _tune(self, name, fn, descs, it_begin, it_end, extra, reset):
+        """kernel="tuned": time every compiled shape on a window of this
+        batch's own steps, keep the fastest for every descriptor, then undo
+        the trial steps with reset()."""
+        if self.block.get("kernel") != "tuned":
+            return
+        from . import autotune
+        from ._propagator import set_shape
+        key = ("batch", name, self.nx, self.nz, self.medium.fw, self.systems, len(descs),
+               self.imaging, self.isnap)
+        hi = min(it_end, it_begin + 64)
+
+        def trial(shape):
+            for d in descs:
+                set_shape(d, shape)
+            self._batch(fn, descs, it_begin, hi, *extra)
+
+        shape = autotune.tune(key, self.systems, trial)
+        for d in descs:
+            set_shape(d, shape)
+        self.tuned[name] = shape
+        reset()
 
     def _batch(self, fn, descs, it_begin, it_end, *extra):
         arr = _desc_array(descs)
@@ -180,6 +205,14 @@ class BatchRTM:
             descs.append(d)
             self.flags_f.append(fl)
         self._fw_descs, self._fw_ws = descs, keep
+
+        def reset_fw():
+            for f, fl in zip(self.fw, self.flags_f):
+                f.reset_state()
+                fl.zero_()
+
+        self._tune("forward", "trtm_forward_steps_batch", descs, 0, nt_max,
+                   (1 if snapshot else 0,), reset_fw)
         if snapshot:
             self._batch("trtm_forward_steps_batch", descs, 0, nt_max, 1)
             return
@@ -245,7 +278,14 @@ class BatchRTM:
             descs.append(d)
             self.flags_b.append(fl)
         self._bw_ws = keep
+
+        def reset_bw():
+            for b, fl in zip(self.bw, self.flags_b):
+                b.reset_state()
+                fl.zero_()
+
         if snapshot:
+            self._tune("backward", "trtm_backward_steps_batch", descs, 0, nt_max, (), reset_bw)
             self._batch("trtm_backward_steps_batch", descs, 0, nt_max)
             return
         # checkpoint mode: segments last to first
@@ -286,6 +326,8 @@ class BatchRTM:
                     _ptr(bufs[q][c]) for c in ("u", "v", "w"))
                 descs[q].isnaps_host = isn.ctypes.data_as(ctypes.c_void_p)
                 descs[q].step_offset = f.nt
+            if k == len(self.edges) - 2:
+                self._tune("backward", "trtm_backward_steps_batch", descs, -b, -a, (), reset_bw)
             self._batch("trtm_backward_steps_batch", descs, -b, -a)
 
     # -- driver -------------------------------------------------------------

@@ -25,6 +25,7 @@ struct ShotDev;
 int64_t block_workspace_bytes(const trtm_step_desc* d);
 int64_t block_shot_table_bytes(int n);
 int block_config_query(const trtm_step_desc* d, int32_t* out);
+int block_config_list(int systems, int kernel, int32_t* out, int max_n);
 ShotDev* block_shot_slot(const trtm_step_desc* d);
 int run_block(const trtm_step_desc* descs, int n, bool fwd, bool save, int32_t it_begin,
               int32_t it_end, ShotDev* shots_dev, const KArgs& m, hipStream_t s);
@@ -370,6 +371,13 @@ int64_t trtm_workspace_bytes(const trtm_step_desc* d) {
     if (d->kernel == TRTM_KERNEL_SIMPLE) return 0;
     if (d->kernel == TRTM_KERNEL_FUSED) return fused_workspace_bytes(d);
     return std::max(block_workspace_bytes(d), fused_workspace_bytes(d));
+}
+
+int trtm_block_configs(int32_t systems, int32_t kernel, int32_t* out, int32_t max_n) {
+    if (systems <= 0 || (systems & ~(TRTM_PSV | TRTM_SH)) || !out || max_n < 0 ||
+        (kernel != TRTM_KERNEL_BLOCKED && kernel != TRTM_KERNEL_WAVE))
+        return fail(TRTM_ERR_ARG, "bad arguments");
+    return block_config_list(systems, kernel, out, max_n);
 }
 
 int trtm_block_config(const trtm_step_desc* d, int32_t* out) {

@@ -766,6 +766,22 @@ int block_config_query(const trtm_step_desc* d, int32_t* out) {
     return TRTM_OK;
 }
 
+int block_config_list(int systems, int kernel, int32_t* out, int max_n) {
+    const BlockCfg* list;
+    int n;
+    if (kernel == TRTM_KERNEL_WAVE)
+        wave_configs(systems, &list, &n);
+    else
+        cfg_list(systems, &list, &n);
+    for (int i = 0; i < n && i < max_n; i++) {
+        out[4 * i + 0] = list[i].tx;
+        out[4 * i + 1] = list[i].tz;
+        out[4 * i + 2] = list[i].nt;
+        out[4 * i + 3] = list[i].k;
+    }
+    return n;
+}
+
 int64_t block_shot_table_bytes(int n) { return (int64_t)(2 * n * sizeof(ShotDev) + 256); }
 
 ShotDev* block_shot_slot(const trtm_step_desc* d) {

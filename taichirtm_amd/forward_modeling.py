@@ -110,9 +110,12 @@ class ForwardModeling(Propagator):
                                             for c in ("u", "v", "w"))
         d.rec_u, d.rec_v, d.rec_w = (_ptr(self._seis.get(c)) for c in ("u", "v", "w"))
         ws = self._workspace(d)  # noqa: F841  (kept alive for the queued kernels)
+        s = stream_handle()
+        self._autotune(d, "forward",
+                       lambda dd, n: _lib.call("trtm_forward_steps", dd, 0, n, int(bool(save)), s))
+        flags.zero_()
         stops = range(0, nt, self.isnap) if display_callback is not None else ()
         code, it_fail = 0, None
-        s = stream_handle()
         for a, b in self._chunks(nt, sci, stops):
             _lib.call("trtm_forward_steps", d, a, b, int(bool(save)), s)
             host_flags = flags.cpu().numpy()

@@ -122,3 +122,19 @@ def test_propagator_fails_loudly_without_device():
     with pytest.raises(RuntimeError, match="HIP device"):
         taichirtm_amd.ForwardModeling(wavelet_u_field=wu, wavelet_v_field=wv, wavelet_w_field=ww,
                                       **kw)
+
+
+@pytest.mark.parametrize("systems", [_lib.PSV, _lib.SH, _lib.PSV | _lib.SH])
+def test_block_configs_enumerate_both_families(systems):
+    """The autotuner's search space comes from the library: every compiled
+    shape of each temporally blocked family, wave shapes with 64 - 2K columns."""
+    from taichirtm_amd import autotune
+    shapes = autotune.candidates(systems)
+    fams = {s[0] for s in shapes}
+    assert fams == {"blocked", "wave"}
+    for fam, tx, tz, k in shapes:
+        assert tx > 0 and tz > 0 and k > 0
+        if fam == "wave":
+            assert tz == 64 - 2 * k
+    buf = (ctypes.c_int32 * 4)()
+    assert _lib.load().trtm_block_configs(systems, 99, buf, 1) < 0  # not a blocked family

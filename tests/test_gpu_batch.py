@@ -42,7 +42,7 @@ def _kw(cfg, mu, lam, shot):
                 num_sources=len(shot.src_loc), num_receivers=len(shot.rec_loc))
 
 
-@pytest.mark.parametrize("kernel", ["blocked", "wave"])
+@pytest.mark.parametrize("kernel", ["blocked", "wave", "tuned"])
 @pytest.mark.parametrize("system", ["psv", "sh", "both"])
 @pytest.mark.parametrize("imaging,isnap", [("snapshot", 1), ("snapshot", 3), ("checkpoint", 1),
                                            ("checkpoint", 3)])

@@ -10,7 +10,7 @@ from taichirtm_amd import configs
 from tests.parity import assert_bitwise, run_hip, run_oracle, same
 
 pytestmark = pytest.mark.gpu
-KERNELS = ["blocked", "wave", "fused", "simple"]
+KERNELS = ["blocked", "wave", "tuned", "fused", "simple"]
 
 
 def _case(name):
