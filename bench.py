@@ -17,6 +17,13 @@ roofline: the dominant kernel is the backward step with imaging; its average
 cpu_baseline: the oracle (C/OpenMP restatement of the reference's unfused
          per-step kernel sequence, float32, both wave systems updated as the
          reference does) timed on a bounded sample of the same workload.
+kernel:  "tuned" (default) lets the autotuner pick the blocked kernel family
+         and shape per direction during the warmup step.
+
+Other workloads (--workload psv_batch | fwi, --shots N): the batched
+checkpointed P-SV RTM of the 2000 x 1000 faulted model and the P-SV FWI
+gradient of the 600 x 300 x 2500 problem; one step is the whole job, the
+roofline is job-level (useful cell-updates x algorithmic bytes / wall).
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -60,6 +67,87 @@ def build_workload(name):
     if name == "psv_anomaly":
         return configs.psv_anomaly(), "psv"
     raise SystemExit("unknown workload %r" % name)
+
+
+# algorithmic bytes per useful cell-update of a whole job (forward and
+# backward/adjoint updates averaged): P-SV RTM with imaging every step
+# (68 + 84) / 2; P-SV FWI forward 60 and adjoint-with-gradient 120
+JOB_BYTES = {"psv_batch": (68 + 84) / 2.0, "fwi": (60 + 120) / 2.0}
+
+
+def _shot_specs(cfg, n):
+    from taichirtm_amd import configs
+    from taichirtm_amd.batch import ShotSpec
+    out = []
+    for s in cfg.shots[:n]:
+        w = configs.ricker(cfg.f0, s.nt, cfg.dt)[None]
+        z = np.zeros_like(w)
+        out.append(ShotSpec(np.array([s.src], np.int32), (z, z, w), np.asarray(s.rec, np.int32)))
+    return out
+
+
+def job_main(args):
+    """Batched checkpointed P-SV RTM (psv_batch) or the P-SV FWI gradient
+    (fwi): one step = the whole job over --shots shots."""
+    import torch
+    from taichirtm_amd import autotune, configs
+    if args.workload == "psv_batch":
+        from taichirtm_amd.batch import BatchRTM
+        cfg = configs.psv_faulted()
+        shots = _shot_specs(cfg, args.shots)
+        mu, lam = cfg.moduli()
+
+        def step():
+            b = BatchRTM(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, mu, lam, cfg.rho, cfg.fw, shots,
+                         isnap=args.isnap, imaging="checkpoint", kernel=args.kernel,
+                         tile_x=args.tile_x, tile_z=args.tile_z, block_steps=args.block_steps)
+            if b.run() != [0] * len(shots):
+                raise RuntimeError("non-finite wavefield")
+            b.stacked_image()
+            return b.tuned
+        desc = ("batched checkpointed P-SV RTM, %d shots of %s (uneven record lengths and "
+                "receiver spreads), isnap=%d, stacked image" % (len(shots), cfg.name, args.isnap))
+    else:
+        from taichirtm_amd.fwi import ElasticFWI, l2_misfit_and_gradient
+        cfg, start = configs.psv_fwi()
+        shots = _shot_specs(cfg, args.shots)
+        prob = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, shots,
+                          precision="float32")
+        t = lambda x: torch.as_tensor(np.asarray(x, np.float32), device="cuda")
+        obs = [o.clone() for o in prob.forward(t(cfg.vp), t(cfg.vs), t(cfg.rho))]
+        m0 = [t(x) for x in (start.vp, start.vs, start.rho)]
+
+        def step():
+            l2_misfit_and_gradient(prob, *m0, obs)
+            return {}
+        desc = ("P-SV FWI gradient (forward + checkpointed exact adjoint), %d shots of %s"
+                % (len(shots), cfg.name))
+    for _ in range(args.warmup):
+        tuned = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tuned = step()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    useful = 2.0 * cfg.nx * cfg.nz * sum(s.nt for s in shots)
+    g = useful / wall / 1e9
+    achieved = useful * JOB_BYTES[args.workload] / wall / 1e9
+    out = {
+        "metric": METRIC, "value": round(g, 3), "unit": "GCell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "grid": [cfg.nx, cfg.nz], "shots": len(shots),
+                   "kernel": args.kernel,
+                   "tuned": {k: list(v) for k, v in (tuned or {}).items()}},
+        "shots_per_hour": round(3600.0 * len(shots) / wall, 1),
+        "roofline": {"bound": "hbm", "basis": "job (useful updates x algorithmic bytes / wall)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_update": JOB_BYTES[args.workload]},
+    }
+    print(json.dumps(out), flush=True)
 
 
 class RTMShot:
@@ -163,7 +251,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sh_layered")
-    ap.add_argument("--kernel", default="blocked")
+    ap.add_argument("--kernel", default="tuned")
+    ap.add_argument("--shots", type=int, default=8)
     ap.add_argument("--tile-x", type=int, default=0)
     ap.add_argument("--block-steps", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
@@ -178,6 +267,8 @@ def main():
 
     import torch
     torch.cuda.set_device(0)
+    if args.workload in JOB_BYTES:
+        return job_main(args)
     cfg, system = build_workload(args.workload)
     shot = RTMShot(cfg, args.isnap, args.kernel, args.tile_x, args.block_steps, args.tile_z)
     for _ in range(args.warmup):
@@ -198,7 +289,7 @@ def main():
     ms_per_step = wall * 1000.0 / args.steps
     gcells = 2.0 * cells * nt / (ms_per_step / 1000.0) / 1e9
     # time steps per launch of the step kernel (1 for the fused and simple kernels)
-    blocked = args.kernel in ("blocked", "wave", "auto")
+    blocked = args.kernel in ("blocked", "wave", "auto", "tuned")
     k = shot.bw.block_config()[3] if blocked else 1
     launches = (nt + k - 1) // k
     bw_launch_us = 1000.0 * float(np.mean(bw_ms)) / launches
@@ -206,7 +297,8 @@ def main():
     b_bytes = ALGO_BYTES[(system, "backward")] * cells * k
     f_bytes = ALGO_BYTES[(system, "forward")] * cells * k
     achieved = b_bytes / (bw_launch_us * 1e-6) / 1e9
-    if args.kernel == "wave":
+    family = shot.bw._tuned[0] if shot.bw._tuned else args.kernel
+    if family == "wave":
         kname = ("k_wave backward, %d steps per launch, one wavefront per tile, state and "
                  "image in registers" % k)
     elif k > 1:
@@ -215,7 +307,7 @@ def main():
         kname = "k_fused backward step with imaging"
     # HBM bytes per launch of the default blocked SH configuration, from the
     # FETCH_SIZE / WRITE_SIZE passes recorded under profiles/
-    traffic = MEASURED_TRAFFIC.get((cfg.name, args.kernel,
+    traffic = MEASURED_TRAFFIC.get((cfg.name, family,
                                     shot.bw.block_config() if blocked else None))
     out = {
         "metric": METRIC,
@@ -235,7 +327,9 @@ def main():
                    "grid": [cfg.nx, cfg.nz], "nt": nt, "dx": cfg.dx, "dt": cfg.dt,
                    "absorbing_frame": cfg.fw, "receivers": len(cfg.shots[0].rec),
                    "kernel": args.kernel, "tile_x": args.tile_x, "tile_z": args.tile_z,
-                   "block_steps": args.block_steps, "system": system},
+                   "block_steps": args.block_steps, "system": system,
+                   "chosen": {"forward": list(shot.fw._tuned or ()),
+                              "backward": list(shot.bw._tuned or ())}},
         "shots_per_hour": round(3600.0 / (ms_per_step / 1000.0), 1),
         "phases_ms": {"forward": round(float(np.mean(fw_ms)), 3),
                       "backward": round(float(np.mean(bw_ms)), 3)},
