@@ -284,6 +284,18 @@ int trtm_fwi_forward(const trtm_fwi_desc *d, int32_t it_begin, int32_t it_end, v
 int trtm_fwi_adjoint(const trtm_fwi_desc *d, int32_t it_begin, int32_t it_end, void *stream);
 
 /*
+ * Batched forms: n shots of one grid, precision and wave systems advance
+ * together, up to 8 shots per launch (the shot is the grid's third
+ * dimension); each shot keeps its own state, adjoint state, history and
+ * gradient accumulators, and skips the steps at or beyond its own nt.  The
+ * descriptors must share hist_base, hist_len and whether hist is set.
+ */
+int trtm_fwi_forward_batch(const trtm_fwi_desc *descs, int32_t n, int32_t it_begin,
+                           int32_t it_end, void *stream);
+int trtm_fwi_adjoint_batch(const trtm_fwi_desc *descs, int32_t n, int32_t it_begin,
+                           int32_t it_end, void *stream);
+
+/*
  * Coefficient-space gradient to (lam, mu, rho) through the harmonic and
  * arithmetic averages (forward_modeling.py:243-307), then to (vp, vs, rho)
  * with mu = rho vs^2, lam = rho (vp^2 - 2 vs^2) (rtm.py:901-902).  Double

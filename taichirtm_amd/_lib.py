@@ -103,6 +103,8 @@ SIGNATURES = {
     "trtm_prepare_medium_f64": (ctypes.c_int, [_i32, _i32] + [_p] * 10 + [_p]),
     "trtm_fwi_prepare": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _p]),
     "trtm_fwi_forward": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _p]),
+    "trtm_fwi_forward_batch": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _i32, _p]),
+    "trtm_fwi_adjoint_batch": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _i32, _p]),
     "trtm_fwi_adjoint": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _p]),
     "trtm_fwi_chain_gradient": (ctypes.c_int, [_i32, _i32] + [_p] * 15 + [_p]),
     "trtm_transpose": (ctypes.c_int, [_i32, _i32, _p, _p, _p]),

@@ -16,12 +16,28 @@
 // the forward updates, so each kernel writes only its own cell.
 #include <algorithm>
 #include <stdarg.h>
+#include <string>
+#include <vector>
 
 #include "trtm_real.hpp"
 
 namespace trtm {
 
 int fail(int code, const char* fmt, ...);
+
+// Kernel parameters come either by value (one shot) or from a device table of
+// per-shot entries indexed by blockIdx.z (a batch of shots, one launch).
+template <class T>
+struct ByValue {
+    T v;
+    __device__ __forceinline__ const T& get() const { return v; }
+};
+constexpr int kShotsPerLaunch = 8;  // per-shot entries fit the 4 KB kernel-argument limit
+template <class T>
+struct ByArray {
+    T v[kShotsPerLaunch];
+    __device__ __forceinline__ const T& get() const { return v[blockIdx.z]; }
+};
 
 template <class R>
 struct AA {
@@ -38,6 +54,8 @@ struct AA {
     // history plane offsets (in cells) of sxx, szz, sxz, u, w, syx, syz, v
     long long h_sxx, h_szz, h_sxz, h_u, h_w, h_syx, h_syz, h_v;
     const int *src_map, *rec_map;  // cell -> source / receiver group, -1 none
+    const R* hist;                 // history of the current segment
+    long long hist_step;           // cells per history slot
 };
 
 // Per cell: the data adjoint of step n at the receivers (entries in order),
@@ -46,12 +64,14 @@ struct AA {
 // gradient of inv_rho_u / inv_rho_w / inv_rho (u += (wavelet * factor) *
 // inv_rho_u at a source cell).  The velocity adjoint of a cell is read and
 // written only by its own thread.
-template <class R, bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_adj_stress(AA<R> a, const R* __restrict__ h, int n) {
+template <class R, bool PSV, bool SH, class S>
+__global__ __launch_bounds__(256) void k_adj_stress(S src, int n, int slot) {
+    const AA<R> a = src.get();
     const int j = blockIdx.x * kBlockZ + threadIdx.x;
     const int i = blockIdx.y * kBlockX + threadIdx.y;
     const int nx = a.nx, nz = a.nz;
-    if (i >= nx || j >= nz) return;
+    if (i >= nx || j >= nz || n >= a.nt) return;
+    const R* __restrict__ h = a.hist + (size_t)slot * a.hist_step;
     const size_t c = (size_t)i * nz + j;
     R pu0 = PSV ? a.pu[c] : R(0), pw0 = PSV ? a.pw[c] : R(0), pv0 = SH ? a.pv[c] : R(0);
     const int rg = a.rec_map ? a.rec_map[c] : -1;
@@ -135,12 +155,14 @@ __global__ __launch_bounds__(256) void k_adj_stress(AA<R> a, const R* __restrict
 }
 // sponge transpose on the stresses + velocity-update transpose: new stress
 // adjoint, and the gradients of inv_rho_u, inv_rho_w (P-SV) / inv_rho (SH)
-template <class R, bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_adj_velocity(AA<R> a, const R* __restrict__ h) {
+template <class R, bool PSV, bool SH, class S>
+__global__ __launch_bounds__(256) void k_adj_velocity(S src, int n, int slot) {
+    const AA<R> a = src.get();
     const int j = blockIdx.x * kBlockZ + threadIdx.x;
     const int i = blockIdx.y * kBlockX + threadIdx.y;
     const int nx = a.nx, nz = a.nz;
-    if (i >= nx || j >= nz) return;
+    if (i >= nx || j >= nz || n >= a.nt) return;
+    const R* __restrict__ h = a.hist + (size_t)slot * a.hist_step;
     const size_t c = (size_t)i * nz + j;
     const R A = sponge_at_t<R>(i, j, nx, nz, a.fw, a.tab);
     auto inner = [&](int ii, int jj) { return ii >= 1 && ii < nx - 1 && jj >= 1 && jj < nz - 1; };
@@ -188,12 +210,12 @@ __global__ __launch_bounds__(256) void k_adj_velocity(AA<R> a, const R* __restri
     }
 }
 
-template <class R, bool PSV, bool SH>
-static hipError_t adjoint_step(const AA<R>& a, const R* h, int n, hipStream_t s) {
+template <class R, bool PSV, bool SH, class S>
+static hipError_t adjoint_step(S src, int nx, int nz, int shots, int n, int slot, hipStream_t s) {
     const dim3 block(kBlockZ, kBlockX);
-    const dim3 grid((a.nz + kBlockZ - 1) / kBlockZ, (a.nx + kBlockX - 1) / kBlockX);
-    hipLaunchKernelGGL((k_adj_stress<R, PSV, SH>), grid, block, 0, s, a, h, n);
-    hipLaunchKernelGGL((k_adj_velocity<R, PSV, SH>), grid, block, 0, s, a, h);
+    const dim3 grid((nz + kBlockZ - 1) / kBlockZ, (nx + kBlockX - 1) / kBlockX, shots);
+    hipLaunchKernelGGL((k_adj_stress<R, PSV, SH, S>), grid, block, 0, s, src, n, slot);
+    hipLaunchKernelGGL((k_adj_velocity<R, PSV, SH, S>), grid, block, 0, s, src, n, slot);
     return hipGetLastError();
 }
 
@@ -215,24 +237,27 @@ struct FF {
     const int *src_order, *src_start, *rec_order, *rec_start;
     const R *wav_u, *wav_v, *wav_w;
     R *rec_u, *rec_v, *rec_w;
-    R* h;  // this step's history planes or nullptr
+    R* hist;  // history of the current segment or nullptr
+    long long hist_step;
     long long h_sxx, h_szz, h_sxz, h_u, h_w, h_syx, h_syz, h_v;
 };
 
-template <class R, bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_ff_velocity(FF<R> a, int it) {
+template <class R, bool PSV, bool SH, class S>
+__global__ __launch_bounds__(256) void k_ff_velocity(S src, int it, int slot) {
+    const FF<R> a = src.get();
     const int j = blockIdx.x * kBlockZ + threadIdx.x;
     const int i = blockIdx.y * kBlockX + threadIdx.y;
     const int nx = a.nx, nz = a.nz;
-    if (i >= nx || j >= nz) return;
+    if (i >= nx || j >= nz || it >= a.nt) return;
+    R* const hh = (a.hist && slot >= 0) ? a.hist + (size_t)slot * a.hist_step : nullptr;
     const size_t c = (size_t)i * nz + j;
     const bool inner = i >= 1 && i < nx - 1 && j >= 1 && j < nz - 1;
     const int sg = a.src_map ? a.src_map[c] : -1;
     if (PSV) {
-        if (a.h) {
-            a.h[a.h_sxx + c] = a.sxx[c];
-            a.h[a.h_szz + c] = a.szz[c];
-            a.h[a.h_sxz + c] = a.sxz[c];
+        if (hh) {
+            hh[a.h_sxx + c] = a.sxx[c];
+            hh[a.h_szz + c] = a.szz[c];
+            hh[a.h_sxz + c] = a.sxz[c];
         }
         R u = a.u[c], w = a.w[c];
         if (inner) {
@@ -251,15 +276,15 @@ __global__ __launch_bounds__(256) void k_ff_velocity(FF<R> a, int it) {
             }
         a.u[c] = u;
         a.w[c] = w;
-        if (a.h) {
-            a.h[a.h_u + c] = u;
-            a.h[a.h_w + c] = w;
+        if (hh) {
+            hh[a.h_u + c] = u;
+            hh[a.h_w + c] = w;
         }
     }
     if (SH) {
-        if (a.h) {
-            a.h[a.h_syx + c] = a.syx[c];
-            a.h[a.h_syz + c] = a.syz[c];
+        if (hh) {
+            hh[a.h_syx + c] = a.syx[c];
+            hh[a.h_syz + c] = a.syz[c];
         }
         R v = a.v[c];
         if (inner) {
@@ -271,16 +296,17 @@ __global__ __launch_bounds__(256) void k_ff_velocity(FF<R> a, int it) {
             for (int e = a.src_start[sg]; e < a.src_start[sg + 1]; e++)
                 v = v + (a.wav_v[(size_t)a.src_order[e] * a.nt + it] * a.factor) * a.ir[c];
         a.v[c] = v;
-        if (a.h) a.h[a.h_v + c] = v;
+        if (hh) hh[a.h_v + c] = v;
     }
 }
 
-template <class R, bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_ff_stress(FF<R> a) {
+template <class R, bool PSV, bool SH, class S>
+__global__ __launch_bounds__(256) void k_ff_stress(S src, int it) {
+    const FF<R> a = src.get();
     const int j = blockIdx.x * kBlockZ + threadIdx.x;
     const int i = blockIdx.y * kBlockX + threadIdx.y;
     const int nx = a.nx, nz = a.nz;
-    if (i >= nx || j >= nz) return;
+    if (i >= nx || j >= nz || it >= a.nt) return;
     const size_t c = (size_t)i * nz + j;
     const R A = sponge_at_t<R>(i, j, nx, nz, a.fw, a.tab);
     const bool inner = i >= 1 && i < nx - 1 && j >= 1 && j < nz - 1;
@@ -315,12 +341,13 @@ __global__ __launch_bounds__(256) void k_ff_stress(FF<R> a) {
     }
 }
 
-template <class R, bool PSV, bool SH>
-__global__ __launch_bounds__(256) void k_ff_post(FF<R> a, int it) {
+template <class R, bool PSV, bool SH, class S>
+__global__ __launch_bounds__(256) void k_ff_post(S src, int it) {
+    const FF<R> a = src.get();
     const int j = blockIdx.x * kBlockZ + threadIdx.x;
     const int i = blockIdx.y * kBlockX + threadIdx.y;
     const int nx = a.nx, nz = a.nz;
-    if (i >= nx || j >= nz) return;
+    if (i >= nx || j >= nz || it >= a.nt) return;
     const size_t c = (size_t)i * nz + j;
     const R A = sponge_at_t<R>(i, j, nx, nz, a.fw, a.tab);
     R u = 0, w = 0, v = 0;
@@ -348,13 +375,13 @@ __global__ __launch_bounds__(256) void k_ff_post(FF<R> a, int it) {
     }
 }
 
-template <class R, bool PSV, bool SH>
-static hipError_t forward_step(const FF<R>& a, int it, hipStream_t s) {
+template <class R, bool PSV, bool SH, class S>
+static hipError_t forward_step(S src, int nx, int nz, int shots, int it, int slot, hipStream_t s) {
     const dim3 block(kBlockZ, kBlockX);
-    const dim3 grid((a.nz + kBlockZ - 1) / kBlockZ, (a.nx + kBlockX - 1) / kBlockX);
-    hipLaunchKernelGGL((k_ff_velocity<R, PSV, SH>), grid, block, 0, s, a, it);
-    hipLaunchKernelGGL((k_ff_stress<R, PSV, SH>), grid, block, 0, s, a);
-    hipLaunchKernelGGL((k_ff_post<R, PSV, SH>), grid, block, 0, s, a, it);
+    const dim3 grid((nz + kBlockZ - 1) / kBlockZ, (nx + kBlockX - 1) / kBlockX, shots);
+    hipLaunchKernelGGL((k_ff_velocity<R, PSV, SH, S>), grid, block, 0, s, src, it, slot);
+    hipLaunchKernelGGL((k_ff_stress<R, PSV, SH, S>), grid, block, 0, s, src, it);
+    hipLaunchKernelGGL((k_ff_post<R, PSV, SH, S>), grid, block, 0, s, src, it);
     return hipGetLastError();
 }
 
@@ -504,7 +531,7 @@ static void hist_offsets(int systems, long long n, long long* o) {
 }
 
 template <class R>
-static int fwi_forward_t(const trtm_fwi_desc* d, int it_begin, int it_end, hipStream_t s) {
+static FF<R> make_ff(const trtm_fwi_desc* d) {
     FF<R> a;
     a.nx = d->nx; a.nz = d->nz; a.nt = d->nt; a.fw = d->fw; a.n_rec = d->n_rec;
     a.dt = (R)d->dt; a.idx = (R)d->inv_dx; a.idz = (R)d->inv_dz; a.factor = (R)d->src_factor;
@@ -526,23 +553,13 @@ static int fwi_forward_t(const trtm_fwi_desc* d, int it_begin, int it_end, hipSt
     hist_offsets(d->systems, n, o);
     a.h_sxx = o[0]; a.h_szz = o[1]; a.h_sxz = o[2]; a.h_u = o[3]; a.h_w = o[4];
     a.h_syx = o[5]; a.h_syz = o[6]; a.h_v = o[7];
-    const bool psv = d->systems & TRTM_PSV, sh = d->systems & TRTM_SH;
-    const int nf = hist_fields(d->systems);
-    for (int it = it_begin; it < it_end; it++) {
-        const int slot = it - d->hist_base;
-        a.h = (d->hist && slot >= 0 && slot < d->hist_len) ? (R*)d->hist + (size_t)slot * nf * n
-                                                           : nullptr;
-        hipError_t e;
-        if (psv && sh) e = forward_step<R, true, true>(a, it, s);
-        else if (psv) e = forward_step<R, true, false>(a, it, s);
-        else e = forward_step<R, false, true>(a, it, s);
-        if (e != hipSuccess) return fail(TRTM_ERR_HIP, "fwi forward step %d: %s", it, hipGetErrorString(e));
-    }
-    return TRTM_OK;
+    a.hist = (R*)d->hist;
+    a.hist_step = (long long)hist_fields(d->systems) * n;
+    return a;
 }
 
 template <class R>
-static int fwi_adjoint_t(const trtm_fwi_desc* d, int it_begin, int it_end, hipStream_t s) {
+static AA<R> make_aa(const trtm_fwi_desc* d) {
     AA<R> a;
     a.nx = d->nx; a.nz = d->nz; a.nt = d->nt; a.fw = d->fw;
     a.n_rec = d->n_rec; a.n_rec_groups = d->n_rec_groups; a.n_src_groups = d->n_src_groups;
@@ -561,7 +578,6 @@ static int fwi_adjoint_t(const trtm_fwi_desc* d, int it_begin, int it_end, hipSt
     a.g_lam = (R*)d->g_lam; a.g_mu = (R*)d->g_mu; a.g_mxz = (R*)d->g_mxz;
     a.g_myx = (R*)d->g_myx; a.g_myz = (R*)d->g_myz; a.g_ir = (R*)d->g_ir;
     a.g_iru = (R*)d->g_iru; a.g_irw = (R*)d->g_irw;
-    const bool psv = d->systems & TRTM_PSV, sh = d->systems & TRTM_SH;
     const long long n = (long long)d->nx * d->nz;
     long long o[8];
     hist_offsets(d->systems, n, o);
@@ -569,19 +585,78 @@ static int fwi_adjoint_t(const trtm_fwi_desc* d, int it_begin, int it_end, hipSt
     a.h_syx = o[5]; a.h_syz = o[6]; a.h_v = o[7];
     a.src_map = d->n_src_groups > 0 ? d->cell_map : nullptr;
     a.rec_map = d->n_rec_groups > 0 ? d->cell_map + n : nullptr;
-    const int nf = hist_fields(d->systems);
-    for (int it = it_end - 1; it >= it_begin; it--) {
-        const int slot = it - d->hist_base;
-        if (slot < 0 || slot >= d->hist_len)
-            return fail(TRTM_ERR_ARG, "adjoint step %d has no history (base %d, length %d)", it,
-                        d->hist_base, d->hist_len);
-        const R* h = (const R*)d->hist + (size_t)slot * nf * n;
+    a.hist = (const R*)d->hist;
+    a.hist_step = (long long)hist_fields(d->systems) * n;
+    return a;
+}
+
+template <class R, class S>
+static int forward_loop(S src, const trtm_fwi_desc* d0, int shots, int it_begin, int it_end,
+                        hipStream_t s) {
+    const bool psv = d0->systems & TRTM_PSV, sh = d0->systems & TRTM_SH;
+    for (int it = it_begin; it < it_end; it++) {
+        const int slot = it - d0->hist_base;
+        const int sl = (d0->hist && slot >= 0 && slot < d0->hist_len) ? slot : -1;
         hipError_t e;
-        if (psv && sh) e = adjoint_step<R, true, true>(a, h, it, s);
-        else if (psv) e = adjoint_step<R, true, false>(a, h, it, s);
-        else e = adjoint_step<R, false, true>(a, h, it, s);
+        if (psv && sh) e = forward_step<R, true, true, S>(src, d0->nx, d0->nz, shots, it, sl, s);
+        else if (psv) e = forward_step<R, true, false, S>(src, d0->nx, d0->nz, shots, it, sl, s);
+        else e = forward_step<R, false, true, S>(src, d0->nx, d0->nz, shots, it, sl, s);
+        if (e != hipSuccess) return fail(TRTM_ERR_HIP, "fwi forward step %d: %s", it, hipGetErrorString(e));
+    }
+    return TRTM_OK;
+}
+
+template <class R, class S>
+static int adjoint_loop(S src, const trtm_fwi_desc* d0, int shots, int it_begin, int it_end,
+                        hipStream_t s) {
+    const bool psv = d0->systems & TRTM_PSV, sh = d0->systems & TRTM_SH;
+    for (int it = it_end - 1; it >= it_begin; it--) {
+        const int slot = it - d0->hist_base;
+        if (slot < 0 || slot >= d0->hist_len)
+            return fail(TRTM_ERR_ARG, "adjoint step %d has no history (base %d, length %d)", it,
+                        d0->hist_base, d0->hist_len);
+        hipError_t e;
+        if (psv && sh) e = adjoint_step<R, true, true, S>(src, d0->nx, d0->nz, shots, it, slot, s);
+        else if (psv) e = adjoint_step<R, true, false, S>(src, d0->nx, d0->nz, shots, it, slot, s);
+        else e = adjoint_step<R, false, true, S>(src, d0->nx, d0->nz, shots, it, slot, s);
         if (e != hipSuccess) return fail(TRTM_ERR_HIP, "adjoint step %d: %s", it, hipGetErrorString(e));
     }
+    return TRTM_OK;
+}
+
+static_assert(sizeof(ByArray<AA<double>>) <= 4096 && sizeof(ByArray<FF<double>>) <= 4096,
+              "per-shot kernel arguments exceed 4 KB");
+static_assert(sizeof(ByArray<AA<float>>) <= 4096 && sizeof(ByArray<FF<float>>) <= 4096,
+              "per-shot kernel arguments exceed 4 KB");
+
+// One shot: parameters by value.  Several: groups of up to kShotsPerLaunch
+// shots, their parameters an array kernel argument indexed by blockIdx.z.
+// The groups of a step are launched back to back.
+template <class R>
+static int fwi_forward_t(const trtm_fwi_desc* d, int n, int it_begin, int it_end, hipStream_t s) {
+    if (n == 1) return forward_loop<R>(ByValue<FF<R>>{make_ff<R>(d)}, d, 1, it_begin, it_end, s);
+    std::vector<ByArray<FF<R>>> groups((n + kShotsPerLaunch - 1) / kShotsPerLaunch);
+    for (int q = 0; q < n; q++) groups[q / kShotsPerLaunch].v[q % kShotsPerLaunch] = make_ff<R>(&d[q]);
+    for (int it = it_begin; it < it_end; it++)
+        for (size_t g = 0; g < groups.size(); g++) {
+            const int m = std::min(kShotsPerLaunch, n - (int)g * kShotsPerLaunch);
+            const int rc = forward_loop<R>(groups[g], d, m, it, it + 1, s);
+            if (rc) return rc;
+        }
+    return TRTM_OK;
+}
+
+template <class R>
+static int fwi_adjoint_t(const trtm_fwi_desc* d, int n, int it_begin, int it_end, hipStream_t s) {
+    if (n == 1) return adjoint_loop<R>(ByValue<AA<R>>{make_aa<R>(d)}, d, 1, it_begin, it_end, s);
+    std::vector<ByArray<AA<R>>> groups((n + kShotsPerLaunch - 1) / kShotsPerLaunch);
+    for (int q = 0; q < n; q++) groups[q / kShotsPerLaunch].v[q % kShotsPerLaunch] = make_aa<R>(&d[q]);
+    for (int it = it_end - 1; it >= it_begin; it--)
+        for (size_t g = 0; g < groups.size(); g++) {
+            const int m = std::min(kShotsPerLaunch, n - (int)g * kShotsPerLaunch);
+            const int rc = adjoint_loop<R>(groups[g], d, m, it, it + 1, s);
+            if (rc) return rc;
+        }
     return TRTM_OK;
 }
 
@@ -629,18 +704,19 @@ int trtm_fwi_prepare(const trtm_fwi_desc* d, void* stream) {
     return e == hipSuccess ? TRTM_OK : fail(TRTM_ERR_HIP, "fwi prepare: %s", hipGetErrorString(e));
 }
 
-int trtm_fwi_forward(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, void* stream) {
+}  // extern "C"
+
+static int check_fwd(const trtm_fwi_desc* d, int it_begin, int it_end) {
     int rc = check_fwi(d, it_begin, it_end);
     if (rc) return rc;
     if ((d->systems & TRTM_PSV) && (!d->u || !d->w || !d->sxx || !d->szz || !d->sxz))
         return fail(TRTM_ERR_ARG, "P-SV state pointer is NULL");
     if ((d->systems & TRTM_SH) && (!d->v || !d->syx || !d->syz))
         return fail(TRTM_ERR_ARG, "SH state pointer is NULL");
-    if (d->real_bytes == 4) return fwi_forward_t<float>(d, it_begin, it_end, (hipStream_t)stream);
-    return fwi_forward_t<double>(d, it_begin, it_end, (hipStream_t)stream);
+    return TRTM_OK;
 }
 
-int trtm_fwi_adjoint(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, void* stream) {
+static int check_adj(const trtm_fwi_desc* d, int it_begin, int it_end) {
     int rc = check_fwi(d, it_begin, it_end);
     if (rc) return rc;
     if (!d->hist) return fail(TRTM_ERR_ARG, "adjoint needs the forward history");
@@ -654,8 +730,66 @@ int trtm_fwi_adjoint(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, v
                                 ((d->systems & TRTM_PSV) && (!d->res_u || !d->res_w)) ||
                                 ((d->systems & TRTM_SH) && !d->res_v)))
         return fail(TRTM_ERR_ARG, "receiver plan or data adjoint is NULL");
-    if (d->real_bytes == 4) return fwi_adjoint_t<float>(d, it_begin, it_end, (hipStream_t)stream);
-    return fwi_adjoint_t<double>(d, it_begin, it_end, (hipStream_t)stream);
+    return TRTM_OK;
+}
+
+// shots of a batch: same grid, precision, wave systems and history window;
+// each shot's own record length bounds its steps
+static int check_batch(const trtm_fwi_desc* d, int n, int it_begin, int it_end, bool adjoint) {
+    if (!d || n < 1) return fail(TRTM_ERR_ARG, "no shots");
+    int nt_max = 0;
+    for (int q = 0; q < n; q++) {
+        const trtm_fwi_desc* e = &d[q];
+        const int hi = std::min(it_end, e->nt > 0 ? e->nt : it_end);
+        const int rc = adjoint ? check_adj(e, std::min(it_begin, hi), hi) : check_fwd(e, std::min(it_begin, hi), hi);
+        if (rc) {
+            const std::string why = trtm_last_error();
+            return fail(rc, "shot %d: %s", q, why.c_str());
+        }
+        if (e->nx != d->nx || e->nz != d->nz || e->real_bytes != d->real_bytes ||
+            e->systems != d->systems || e->hist_base != d->hist_base ||
+            e->hist_len != d->hist_len || (e->hist == nullptr) != (d->hist == nullptr))
+            return fail(TRTM_ERR_ARG, "shot %d does not share the grid, precision, systems and "
+                                      "history window of shot 0", q);
+        nt_max = std::max(nt_max, e->nt);
+    }
+    if (it_begin < 0 || it_end > nt_max || it_begin > it_end)
+        return fail(TRTM_ERR_ARG, "step range [%d, %d) outside [0, %d]", it_begin, it_end, nt_max);
+    return TRTM_OK;
+}
+
+extern "C" {
+
+int trtm_fwi_forward(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, void* stream) {
+    const int rc = check_fwd(d, it_begin, it_end);
+    if (rc) return rc;
+    if (d->real_bytes == 4) return fwi_forward_t<float>(d, 1, it_begin, it_end, (hipStream_t)stream);
+    return fwi_forward_t<double>(d, 1, it_begin, it_end, (hipStream_t)stream);
+}
+
+int trtm_fwi_adjoint(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, void* stream) {
+    const int rc = check_adj(d, it_begin, it_end);
+    if (rc) return rc;
+    if (d->real_bytes == 4) return fwi_adjoint_t<float>(d, 1, it_begin, it_end, (hipStream_t)stream);
+    return fwi_adjoint_t<double>(d, 1, it_begin, it_end, (hipStream_t)stream);
+}
+
+int trtm_fwi_forward_batch(const trtm_fwi_desc* descs, int32_t n, int32_t it_begin,
+                           int32_t it_end, void* stream) {
+    const int rc = check_batch(descs, n, it_begin, it_end, false);
+    if (rc) return rc;
+    if (descs->real_bytes == 4)
+        return fwi_forward_t<float>(descs, n, it_begin, it_end, (hipStream_t)stream);
+    return fwi_forward_t<double>(descs, n, it_begin, it_end, (hipStream_t)stream);
+}
+
+int trtm_fwi_adjoint_batch(const trtm_fwi_desc* descs, int32_t n, int32_t it_begin,
+                           int32_t it_end, void* stream) {
+    const int rc = check_batch(descs, n, it_begin, it_end, true);
+    if (rc) return rc;
+    if (descs->real_bytes == 4)
+        return fwi_adjoint_t<float>(descs, n, it_begin, it_end, (hipStream_t)stream);
+    return fwi_adjoint_t<double>(descs, n, it_begin, it_end, (hipStream_t)stream);
 }
 
 int trtm_fwi_chain_gradient(int32_t nx, int32_t nz, const double* mu, const double* rho,

@@ -167,9 +167,14 @@ class ElasticFWI:
 
     # -- forward --------------------------------------------------------------
 
+    def _call_batch(self, fn, descs, a, b):
+        arr = (_lib.FwiDesc * len(descs))(*descs)
+        _lib.call(fn, arr, len(descs), a, b, stream_handle())
+
     def forward(self, vp, vs, rho):
         """Records of every shot, a list of (3, nr, nt) tensors (u, v, w);
-        keeps the checkpoints for ``gradient``."""
+        keeps the checkpoints for ``gradient``.  All shots advance together
+        (one batched call per checkpoint segment)."""
         dev = device()
         self.systems = self._systems()
         self.med = self._medium(vp, vs, rho)
@@ -178,65 +183,77 @@ class ElasticFWI:
         n_hist = (5 if self.systems & _lib.PSV else 0) + (3 if self.systems & _lib.SH else 0)
         self.L = self.checkpoint_every or checkpoint_interval(nt_max, len(names), n_hist)
         shape = (self.nx, self.nz)
-        self.state = {k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in names}
-        self._ck = []
-        out = []
-        s = stream_handle()
-        for g in self.geo:
-            for t in self.state.values():
-                t.zero_()
+        self.edges = list(range(0, nt_max, self.L)) + [nt_max]
+        nseg = len(self.edges) - 1
+        self.state = [{k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in names}
+                      for _ in self.geo]
+        self._ck = [torch.empty((nseg, len(names)) + shape, dtype=self.dtype, device=dev)
+                    for _ in self.geo]
+        recs, descs = [], []
+        for g, st in zip(self.geo, self.state):
             d = self._desc(self.med, g)
             for k in names:
-                setattr(d, k, _ptr(self.state[k]))
+                setattr(d, k, _ptr(st[k]))
             rec = {c: torch.zeros((g["nt"], g["nr"]), dtype=self.dtype, device=dev)
                    for c in ("u", "v", "w")}
             d.rec_u, d.rec_v, d.rec_w = (_ptr(rec[c]) for c in ("u", "v", "w"))
-            edges = list(range(0, g["nt"], self.L)) + [g["nt"]]
-            ck = torch.empty((len(edges) - 1, len(names)) + shape, dtype=self.dtype, device=dev)
-            for k in range(len(edges) - 1):
+            recs.append(rec)
+            descs.append(d)
+        for k in range(nseg):
+            for st, ck in zip(self.state, self._ck):
                 for m, name in enumerate(names):
-                    ck[k, m].copy_(self.state[name])
-                _lib.call("trtm_fwi_forward", d, edges[k], edges[k + 1], s)
-            self._ck.append((edges, ck))
-            out.append(torch.stack([rec[c].t() for c in ("u", "v", "w")]).contiguous())
-        return out
+                    ck[k, m].copy_(st[name])
+            self._call_batch("trtm_fwi_forward_batch", descs, self.edges[k], self.edges[k + 1])
+        return [torch.stack([rec[c].t() for c in ("u", "v", "w")]).contiguous() for rec in recs]
 
     # -- gradient ---------------------------------------------------------------
 
     def gradient(self, data_adjoint: List[torch.Tensor]):
         """Gradient of J with respect to (vp, vs, rho), float64 (nx, nz) each,
-        for dJ/d(records) given per shot as (3, nr, nt)."""
+        for dJ/d(records) given per shot as (3, nr, nt).  Segments last to
+        first: every shot's history is recomputed from its checkpoint and the
+        adjoint steps run over it, all shots in one batched call each; each
+        shot accumulates its own coefficient-space gradient, summed in shot
+        order at the end."""
         if self._ck is None:
             raise RuntimeError("call forward first")
         dev = device()
         names = self._fields()
         shape = (self.nx, self.nz)
-        adj = {"p" + k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in names}
         gnames = ("g_lam", "g_mu", "g_mxz", "g_myx", "g_myz", "g_ir", "g_iru", "g_irw")
-        grads = {k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in gnames}
         n_hist = (5 if self.systems & _lib.PSV else 0) + (3 if self.systems & _lib.SH else 0)
-        hist = torch.empty((self.L, n_hist) + shape, dtype=self.dtype, device=dev)
-        s = stream_handle()
-        for g, (edges, ck), da in zip(self.geo, self._ck, data_adjoint):
-            for t in adj.values():
-                t.zero_()
+        keep, descs = [], []
+        for g, st, da in zip(self.geo, self.state, data_adjoint):
+            adj = {"p" + k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in names}
+            grads = {k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in gnames}
+            hist = torch.empty((self.L, n_hist) + shape, dtype=self.dtype, device=dev)
+            da = torch.as_tensor(da, device=dev).to(self.dtype)
+            res = [da[c].t().contiguous() for c in range(3)]  # (nt, nr) each
             d = self._desc(self.med, g)
             for k in names:
-                setattr(d, k, _ptr(self.state[k]))
+                setattr(d, k, _ptr(st[k]))
                 setattr(d, "p" + k, _ptr(adj["p" + k]))
             for k in gnames:
                 setattr(d, k, _ptr(grads[k]))
-            da = torch.as_tensor(da, device=dev).to(self.dtype)
-            res = [da[c].t().contiguous() for c in range(3)]  # (nt, nr) each
             d.res_u, d.res_v, d.res_w = (_ptr(r) for r in res)
             d.hist = _ptr(hist)
-            for k in range(len(edges) - 2, -1, -1):
-                a, b = edges[k], edges[k + 1]
-                for m, name in enumerate(names):
-                    self.state[name].copy_(ck[k, m])
+            keep.append((adj, grads, hist, res))
+            descs.append(d)
+        for k in range(len(self.edges) - 2, -1, -1):
+            a, b = self.edges[k], self.edges[k + 1]
+            for g, st, ck in zip(self.geo, self.state, self._ck):
+                if a < g["nt"]:
+                    for m, name in enumerate(names):
+                        st[name].copy_(ck[k, m])
+            for d in descs:
                 d.hist_base, d.hist_len = a, b - a
-                _lib.call("trtm_fwi_forward", d, a, b, s)
-                _lib.call("trtm_fwi_adjoint", d, a, b, s)
+            self._call_batch("trtm_fwi_forward_batch", descs, a, b)
+            self._call_batch("trtm_fwi_adjoint_batch", descs, a, b)
+        grads = {k: keep[0][1][k].clone() for k in gnames}
+        for _, gq, _, _ in keep[1:]:
+            for k in gnames:
+                grads[k] += gq[k]
+        s = stream_handle()
         m = self.med
         dbl = lambda t: t.to(torch.float64).contiguous()
         out = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(3)]
