@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TRTM_ABI_VERSION 2
+#define TRTM_ABI_VERSION 3
 
 #define TRTM_OK 0
 #define TRTM_ERR_ARG (-1)         /* invalid argument or shape */
@@ -258,7 +258,25 @@ typedef struct trtm_fwi_desc {
     /* workspace int32 (2, nx, nz): source group / receiver group of each cell
      * (-1: none), filled by trtm_fwi_prepare for this geometry */
     int32_t *cell_map;
+    /* step kernels: TRTM_KERNEL_SIMPLE (one step per launch, float or double),
+     * TRTM_KERNEL_BLOCKED (float only: block_steps steps per launch, LDS
+     * tiles with a halo, needs the workspace), TRTM_KERNEL_AUTO (blocked for
+     * float with a workspace, else one step per launch).  tile_x / tile_z /
+     * block_steps pick a compiled blocked shape (0 = default; listed by
+     * trtm_fwi_block_configs).  Both give the same results bit for bit. */
+    int32_t kernel, tile_x, tile_z, block_steps;
+    /* blocked kernels: second state buffer of trtm_fwi_workspace_bytes() bytes */
+    void *workspace;
 } trtm_fwi_desc;
+
+/* Workspace of the blocked FWI kernels for d's grid (8 REAL fields). */
+int64_t trtm_fwi_workspace_bytes(const trtm_fwi_desc *d);
+
+/*
+ * Compiled shapes of the blocked FWI kernels: writes up to max_n rows of
+ * (tile_x, tile_z, threads, block_steps) to out, returns the number compiled.
+ */
+int trtm_fwi_block_configs(int32_t systems, int32_t *out, int32_t max_n);
 
 /*
  * Fills cell_map from the source and receiver plans (src_*, rec_*): the

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtaichirtm_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 OK = 0
 ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3
 PSV, SH = 1, 2
@@ -78,6 +78,8 @@ class FwiDesc(ctypes.Structure):
         ("g_lam", _p), ("g_mu", _p), ("g_mxz", _p), ("g_myx", _p), ("g_myz", _p), ("g_ir", _p),
         ("g_iru", _p), ("g_irw", _p),
         ("cell_map", _p),
+        ("kernel", _i32), ("tile_x", _i32), ("tile_z", _i32), ("block_steps", _i32),
+        ("workspace", _p),
     ]
 
 
@@ -102,6 +104,8 @@ SIGNATURES = {
     "trtm_check_finite": (ctypes.c_int, [_i32, _i32, _p, _p, _p, _i32, _p, _p]),
     "trtm_prepare_medium_f64": (ctypes.c_int, [_i32, _i32] + [_p] * 10 + [_p]),
     "trtm_fwi_prepare": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _p]),
+    "trtm_fwi_workspace_bytes": (ctypes.c_int64, [ctypes.POINTER(FwiDesc)]),
+    "trtm_fwi_block_configs": (ctypes.c_int, [_i32, _p, _i32]),
     "trtm_fwi_forward": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _p]),
     "trtm_fwi_forward_batch": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _i32, _p]),
     "trtm_fwi_adjoint_batch": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _i32, _p]),

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtaichirtm_hip.so")
 SOURCES = ["trtm_abi.hip", "trtm_simple.hip", "trtm_fused.hip", "trtm_block.hip",
-           "trtm_adjoint.hip", "trtm_wave.hip"] + [
+           "trtm_adjoint.hip", "trtm_fwi_block.hip", "trtm_wave.hip"] + [
     "trtm_wave_%s_%s.hip" % (sys_, dir_) for sys_ in ("sh", "psv", "both")
     for dir_ in ("fwd", "bwd")]
 ARCH = os.environ.get("TRTM_OFFLOAD_ARCH", "gfx950")

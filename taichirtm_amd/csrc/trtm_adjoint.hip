@@ -19,25 +19,12 @@
 #include <string>
 #include <vector>
 
+#include "trtm_fwi.hpp"
 #include "trtm_real.hpp"
 
 namespace trtm {
 
 int fail(int code, const char* fmt, ...);
-
-// Kernel parameters come either by value (one shot) or from a device table of
-// per-shot entries indexed by blockIdx.z (a batch of shots, one launch).
-template <class T>
-struct ByValue {
-    T v;
-    __device__ __forceinline__ const T& get() const { return v; }
-};
-constexpr int kShotsPerLaunch = 8;  // per-shot entries fit the 4 KB kernel-argument limit
-template <class T>
-struct ByArray {
-    T v[kShotsPerLaunch];
-    __device__ __forceinline__ const T& get() const { return v[blockIdx.z]; }
-};
 
 template <class R>
 struct AA {
@@ -513,7 +500,18 @@ static int check_fwi(const trtm_fwi_desc* d, int it_begin, int it_end) {
         return fail(TRTM_ERR_ARG, "source plan is NULL");
     if ((d->rec_u || d->rec_v) && d->n_rec > 0 && (d->n_rec_groups <= 0 || !d->rec_order || !d->rec_group_start))
         return fail(TRTM_ERR_ARG, "recording needs the receiver plan");
+    if (d->kernel != TRTM_KERNEL_AUTO && d->kernel != TRTM_KERNEL_SIMPLE &&
+        d->kernel != TRTM_KERNEL_BLOCKED)
+        return fail(TRTM_ERR_ARG, "kernel must be TRTM_KERNEL_AUTO, _SIMPLE or _BLOCKED");
+    if (d->kernel == TRTM_KERNEL_BLOCKED && (d->real_bytes != 4 || !d->workspace))
+        return fail(TRTM_ERR_ARG, "the blocked FWI kernel needs float and the workspace");
     return TRTM_OK;
+}
+
+// blocked kernels: float with a workspace, unless one step per launch is asked for
+static bool use_block(const trtm_fwi_desc* d) {
+    if (d->kernel == TRTM_KERNEL_SIMPLE) return false;
+    return d->real_bytes == 4 && d->workspace != nullptr;
 }
 
 static int hist_fields(int systems) {
@@ -751,6 +749,9 @@ static int check_batch(const trtm_fwi_desc* d, int n, int it_begin, int it_end, 
             e->hist_len != d->hist_len || (e->hist == nullptr) != (d->hist == nullptr))
             return fail(TRTM_ERR_ARG, "shot %d does not share the grid, precision, systems and "
                                       "history window of shot 0", q);
+        if (use_block(e) != use_block(d) || e->tile_x != d->tile_x || e->tile_z != d->tile_z ||
+            e->block_steps != d->block_steps)
+            return fail(TRTM_ERR_ARG, "shot %d does not share the step kernel of shot 0", q);
         nt_max = std::max(nt_max, e->nt);
     }
     if (it_begin < 0 || it_end > nt_max || it_begin > it_end)
@@ -760,16 +761,29 @@ static int check_batch(const trtm_fwi_desc* d, int n, int it_begin, int it_end, 
 
 extern "C" {
 
+// the adjoint reads the history of every step it takes
+static int check_window(const trtm_fwi_desc* d, int it_begin, int it_end) {
+    if (it_begin < it_end && (it_begin < d->hist_base || it_end > d->hist_base + d->hist_len))
+        return fail(TRTM_ERR_ARG, "adjoint steps [%d, %d) outside the history [%d, %d)", it_begin,
+                    it_end, d->hist_base, d->hist_base + d->hist_len);
+    return TRTM_OK;
+}
+
 int trtm_fwi_forward(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, void* stream) {
     const int rc = check_fwd(d, it_begin, it_end);
     if (rc) return rc;
+    if (use_block(d)) return fwi_block_forward(d, 1, it_begin, it_end, (hipStream_t)stream);
     if (d->real_bytes == 4) return fwi_forward_t<float>(d, 1, it_begin, it_end, (hipStream_t)stream);
     return fwi_forward_t<double>(d, 1, it_begin, it_end, (hipStream_t)stream);
 }
 
 int trtm_fwi_adjoint(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, void* stream) {
-    const int rc = check_adj(d, it_begin, it_end);
+    int rc = check_adj(d, it_begin, it_end);
     if (rc) return rc;
+    if (use_block(d)) {
+        if ((rc = check_window(d, it_begin, it_end))) return rc;
+        return fwi_block_adjoint(d, 1, it_begin, it_end, (hipStream_t)stream);
+    }
     if (d->real_bytes == 4) return fwi_adjoint_t<float>(d, 1, it_begin, it_end, (hipStream_t)stream);
     return fwi_adjoint_t<double>(d, 1, it_begin, it_end, (hipStream_t)stream);
 }
@@ -778,6 +792,7 @@ int trtm_fwi_forward_batch(const trtm_fwi_desc* descs, int32_t n, int32_t it_beg
                            int32_t it_end, void* stream) {
     const int rc = check_batch(descs, n, it_begin, it_end, false);
     if (rc) return rc;
+    if (use_block(descs)) return fwi_block_forward(descs, n, it_begin, it_end, (hipStream_t)stream);
     if (descs->real_bytes == 4)
         return fwi_forward_t<float>(descs, n, it_begin, it_end, (hipStream_t)stream);
     return fwi_forward_t<double>(descs, n, it_begin, it_end, (hipStream_t)stream);
@@ -785,11 +800,26 @@ int trtm_fwi_forward_batch(const trtm_fwi_desc* descs, int32_t n, int32_t it_beg
 
 int trtm_fwi_adjoint_batch(const trtm_fwi_desc* descs, int32_t n, int32_t it_begin,
                            int32_t it_end, void* stream) {
-    const int rc = check_batch(descs, n, it_begin, it_end, true);
+    int rc = check_batch(descs, n, it_begin, it_end, true);
     if (rc) return rc;
+    if (use_block(descs)) {
+        if ((rc = check_window(descs, it_begin, it_end))) return rc;
+        return fwi_block_adjoint(descs, n, it_begin, it_end, (hipStream_t)stream);
+    }
     if (descs->real_bytes == 4)
         return fwi_adjoint_t<float>(descs, n, it_begin, it_end, (hipStream_t)stream);
     return fwi_adjoint_t<double>(descs, n, it_begin, it_end, (hipStream_t)stream);
+}
+
+int64_t trtm_fwi_workspace_bytes(const trtm_fwi_desc* d) {
+    if (!d || d->nx < 1 || d->nz < 1 || (d->real_bytes != 4 && d->real_bytes != 8))
+        return fail(TRTM_ERR_ARG, "bad descriptor");
+    return (int64_t)8 * d->nx * d->nz * d->real_bytes;
+}
+
+int trtm_fwi_block_configs(int32_t systems, int32_t* out, int32_t max_n) {
+    if (!out || max_n < 0) return fail(TRTM_ERR_ARG, "bad arguments");
+    return fwi_block_configs(systems, out, max_n);
 }
 
 int trtm_fwi_chain_gradient(int32_t nx, int32_t nz, const double* mu, const double* rho,

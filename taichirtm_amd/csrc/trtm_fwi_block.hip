@@ -1,0 +1,755 @@
+// Temporally blocked FWI time stepping (float): K steps per launch, up to
+// kShotsPerLaunch shots per launch (shot = blockIdx.z).
+//
+// Forward (k_fwi_fwd_blk): the forward step of forward_modeling.py:539-551
+// (flat free surface) with the layout of the blocked RTM kernel: a workgroup
+// loads its tile plus a K-cell halo of the state into LDS, keeps the region's
+// coefficients and sponge factors in registers and advances K steps; halo
+// cells are recomputed redundantly with the same expressions, so every value
+// is bit-identical to the one-step-per-launch FWI kernels.  Sources and
+// receivers come from the per-cell group maps (source groups are injected at
+// every region cell that holds one, records are written by the tile that owns
+// the cell).  With HIST the tile also writes the step's history (pre-step
+// stresses, post-injection velocities) that the adjoint reads.
+//
+// Adjoint (k_fwi_adj_blk): the exact adjoint step (residual injection, sponge
+// transpose, stress-update transpose with the lam / mu / mxz (myx / myz)
+// gradients, source-term gradient, velocity-update transpose with the
+// inv_rho_u / inv_rho_w (inv_rho) gradients), in gather form, K steps per
+// launch in descending time.  Each thread keeps the adjoint state of its region
+// cells in registers; only the values neighbours gather (the stress
+// transposes q and the scaled velocity adjoints b) pass through LDS.  The
+// gradients of the tile's cells are accumulated in registers in the same
+// order as the per-step kernels and written once per launch.
+//
+// State ping-pongs between the caller's buffers and a workspace copy (tiles
+// read the halo that neighbouring tiles write); a shot whose state ends in the
+// workspace is copied back at the end of the call.
+#include <algorithm>
+#include <vector>
+
+#include "trtm_block.hpp"
+#include "trtm_fwi.hpp"
+
+namespace trtm {
+
+struct FB {
+    int nx, nz, nt, fw;
+    int n_rec, hist_base, hist_len, flip;  // flip: the current state is in alt
+    float dt, idx, idz, factor;
+    const float* tab;
+    const float *lam, *mu, *mxz, *myx, *myz, *ir, *iru, *irw;
+    float* st[8];   // caller's state: u v w sxx szz sxz syx syz (adjoint: pu ... psyz)
+    float* alt[8];  // workspace copy
+    const int *src_map, *rec_map;
+    const int *src_order, *src_start, *rec_order, *rec_start;
+    const float *wav_u, *wav_v, *wav_w;
+    float *rec_u, *rec_v, *rec_w;        // forward records, time-major (nt, n_rec)
+    const float *res_u, *res_v, *res_w;  // data adjoint, time-major (nt, n_rec)
+    float* hist;
+    float *g_lam, *g_mu, *g_mxz, *g_myx, *g_myz, *g_ir, *g_iru, *g_irw;
+};
+static_assert(sizeof(ByArray<FB>) + 32 <= 4096, "per-shot kernel arguments exceed 4 KB");
+
+// history fields of one step, in the order sxx, szz, sxz, u, w, syx, syz, v
+// over the active systems
+template <bool PSV, bool SH>
+struct HF {
+    static constexpr int sxx = 0, szz = 1, sxz = 2, u = 3, w = 4;
+    static constexpr int syx = PSV ? 5 : 0, syz = PSV ? 6 : 1, v = PSV ? 7 : 2;
+    static constexpr int n = (PSV ? 5 : 0) + (SH ? 3 : 0);
+};
+
+// ---------------------------------------------------------------------------
+// forward, steps t_a .. t_b-1 (t_b - t_a <= K)
+
+template <bool PSV, bool SH, int TX, int TZ, int NT, int K, bool HIST, class S>
+__global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int tiles_x,
+                                                    int tiles_z) {
+    constexpr int RX = TX + 2 * K, RZ = TZ + 2 * K, RN = RX * RZ;
+    constexpr int NC = (RN + NT - 1) / NT;
+    constexpr int NP = PSV ? RN : 1, NS = SH ? RN : 1;
+    using H = HF<PSV, SH>;
+    __shared__ float sU[NP], sW[NP], sXX[NP], sZZ[NP], sXZ[NP];
+    __shared__ float sV[NS], sYX[NS], sYZ[NS];
+
+    const FB& a = src.get();
+    const int tiles = tiles_x * tiles_z;
+    const int tile = xcd_tile_b(blockIdx.x, tiles);
+    const int nt = a.nt;
+    if (tile >= tiles || t_a >= nt) return;
+    const int i0 = (tile / tiles_z) * TX, j0 = (tile % tiles_z) * TZ;
+    const int ri0 = i0 - K, rj0 = j0 - K;
+    const int nx = a.nx, nz = a.nz;
+    const int t = threadIdx.x;
+    const bool has_src = a.src_map != nullptr, has_rec = a.rec_map != nullptr;
+
+    int gc[NC], sg[NC], rg[NC];
+    unsigned fl[NC];  // bit0 in grid, bit1 interior, bit2 in tile, bit3 V ok, bit4 S ok
+    float A[NC], c0[NC], c1[NC], c2[NC], c3[NC], c4[NC], d0[NC], d1[NC], d2[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const int e = t + NT * k;
+        const int r = e / RZ, z = e - r * RZ;
+        const int i = ri0 + r, j = rj0 + z;
+        unsigned f = 0;
+        gc[k] = 0;
+        sg[k] = rg[k] = -1;
+        A[k] = 1.0f;
+        c0[k] = c1[k] = c2[k] = c3[k] = c4[k] = d0[k] = d1[k] = d2[k] = 0.0f;
+        if (e < RN && i >= 0 && i < nx && j >= 0 && j < nz) {
+            f |= 1u;
+            const int c = i * nz + j;
+            gc[k] = c;
+            if (i >= 1 && i < nx - 1 && j >= 1 && j < nz - 1) f |= 2u;
+            const bool own = i >= i0 && i < i0 + TX && j >= j0 && j < j0 + TZ;
+            if (own) f |= 4u;
+            if (r >= 1 && z >= 1) f |= 8u;
+            if (r <= RX - 2 && z <= RZ - 2) f |= 16u;
+            A[k] = sponge_at(i, j, nx, nz, a.fw, a.tab);
+            if (PSV) {
+                c0[k] = G(a.iru)[c];
+                c1[k] = G(a.irw)[c];
+                c2[k] = G(a.lam)[c];
+                c3[k] = G(a.mu)[c];
+                c4[k] = G(a.mxz)[c];
+            }
+            if (SH) {
+                d0[k] = G(a.ir)[c];
+                d1[k] = G(a.myx)[c];
+                d2[k] = G(a.myz)[c];
+            }
+            if (has_src) sg[k] = G(a.src_map)[c];
+            if (has_rec && own) rg[k] = G(a.rec_map)[c];
+        }
+        fl[k] = f;
+    }
+
+    {
+        float* const* in = a.flip ? a.alt : a.st;
+        const gptr<float> in_u = G(in[0]), in_v = G(in[1]), in_w = G(in[2]);
+        const gptr<float> in_xx = G(in[3]), in_zz = G(in[4]), in_xz = G(in[5]);
+        const gptr<float> in_yx = G(in[6]), in_yz = G(in[7]);
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            const int e = t + NT * k;
+            if (e >= RN) continue;
+            const bool g = fl[k] & 1u;
+            const int c = gc[k];
+            if (PSV) {
+                sU[e] = g ? in_u[c] : 0.0f;
+                sW[e] = g ? in_w[c] : 0.0f;
+                sXX[e] = g ? in_xx[c] : 0.0f;
+                sZZ[e] = g ? in_zz[c] : 0.0f;
+                sXZ[e] = g ? in_xz[c] : 0.0f;
+            }
+            if (SH) {
+                sV[e] = g ? in_v[c] : 0.0f;
+                sYX[e] = g ? in_yx[c] : 0.0f;
+                sYZ[e] = g ? in_yz[c] : 0.0f;
+            }
+        }
+    }
+    __syncthreads();
+
+    const size_t n = (size_t)nx * nz;
+    const float idx = a.idx, idz = a.idz, dt = a.dt, factor = a.factor;
+    const int hi = t_b < nt ? t_b : nt;
+    const gptr<const int> sorder = G(a.src_order), sstart = G(a.src_start);
+    const gptr<const int> rorder = G(a.rec_order), rstart = G(a.rec_start);
+    const gptr<const float> wu = G(a.wav_u), wv = G(a.wav_v), ww = G(a.wav_w);
+    const gptr<float> ru = G(a.rec_u), rv = G(a.rec_v), rw = G(a.rec_w);
+
+    bool started = false;  // a step ran: LDS velocities await the sponge
+    for (int s = 0; s < K; s++) {
+        const int it = t_a + s;
+        if (it >= hi) break;
+        gptr<float> hh = nullptr;
+        if (HIST) {
+            const int slot = it - a.hist_base;
+            if (slot >= 0 && slot < a.hist_len) hh = G(a.hist) + (size_t)slot * (H::n * n);
+        }
+        // ---- history: the stresses this step's velocity update reads
+        if (HIST && hh) {
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                if ((fl[k] & 5u) != 5u) continue;
+                const int e = t + NT * k;
+                const int c = gc[k];
+                if (PSV) {
+                    hh[H::sxx * n + c] = sXX[e];
+                    hh[H::szz * n + c] = sZZ[e];
+                    hh[H::sxz * n + c] = sXZ[e];
+                }
+                if (SH) {
+                    hh[H::syx * n + c] = sYX[e];
+                    hh[H::syz * n + c] = sYZ[e];
+                }
+            }
+        }
+        // ---- velocity update + source injection (region)
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            if (!(fl[k] & 1u)) continue;
+            const int e = t + NT * k;
+            const bool upd = (fl[k] & 2u) && (fl[k] & 8u);
+            if (PSV) {
+                float u = sU[e], w = sW[e];
+                if (started) {
+                    u = u * A[k];
+                    w = w * A[k];
+                }
+                if (upd) {
+                    const float sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
+                    const float szz_z = (sZZ[e] - sZZ[e - 1]) * idz;
+                    const float sxz_x = (sXZ[e] - sXZ[e - RZ]) * idx;
+                    const float sxz_z = (sXZ[e] - sXZ[e - 1]) * idz;
+                    u = u + ((sxx_x + sxz_z) * dt) * c0[k];
+                    w = w + ((sxz_x + szz_z) * dt) * c1[k];
+                }
+                if (sg[k] >= 0)
+                    for (int q = sstart[sg[k]]; q < sstart[sg[k] + 1]; q++) {
+                        const size_t o = (size_t)sorder[q] * nt + it;
+                        u = u + (wu[o] * factor) * c0[k];
+                        w = w + (ww[o] * factor) * c1[k];
+                    }
+                sU[e] = u;
+                sW[e] = w;
+                if (HIST && hh && (fl[k] & 4u)) {
+                    hh[H::u * n + gc[k]] = u;
+                    hh[H::w * n + gc[k]] = w;
+                }
+            }
+            if (SH) {
+                float v = sV[e];
+                if (started) v = v * A[k];
+                if (upd) {
+                    const float syx_x = (sYX[e] - sYX[e - RZ]) * idx;
+                    const float syz_z = (sYZ[e] - sYZ[e - 1]) * idz;
+                    v = v + ((syx_x + syz_z) * dt) * d0[k];
+                }
+                if (sg[k] >= 0)
+                    for (int q = sstart[sg[k]]; q < sstart[sg[k] + 1]; q++)
+                        v = v + (wv[(size_t)sorder[q] * nt + it] * factor) * d0[k];
+                sV[e] = v;
+                if (HIST && hh && (fl[k] & 4u)) hh[H::v * n + gc[k]] = v;
+            }
+        }
+        __syncthreads();
+
+        // ---- stress update + sponge (region), records (tile)
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            if (!(fl[k] & 1u)) continue;
+            const int e = t + NT * k;
+            const bool upd = (fl[k] & 2u) && (fl[k] & 16u);
+            if (PSV) {
+                float sxx = sXX[e], szz = sZZ[e], sxz = sXZ[e];
+                if (upd) {
+                    const float u_x = (sU[e + RZ] - sU[e]) * idx;
+                    const float u_z = (sU[e + 1] - sU[e]) * idz;
+                    const float w_x = (sW[e + RZ] - sW[e]) * idx;
+                    const float w_z = (sW[e + 1] - sW[e]) * idz;
+                    const float div_uw = u_x + w_z;
+                    sxx = sxx + dt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
+                    szz = szz + dt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
+                    sxz = sxz + (dt * c4[k]) * (u_z + w_x);
+                }
+                sXX[e] = sxx * A[k];
+                sZZ[e] = szz * A[k];
+                sXZ[e] = sxz * A[k];
+            }
+            if (SH) {
+                float syx = sYX[e], syz = sYZ[e];
+                if (upd) {
+                    const float v_x = (sV[e + RZ] - sV[e]) * idx;
+                    const float v_z = (sV[e + 1] - sV[e]) * idz;
+                    syx = syx + (dt * d1[k]) * v_x;
+                    syz = syz + (dt * d2[k]) * v_z;
+                }
+                sYX[e] = syx * A[k];
+                sYZ[e] = syz * A[k];
+            }
+            if (rg[k] >= 0) {
+                const size_t row = (size_t)it * a.n_rec;
+                for (int q = rstart[rg[k]]; q < rstart[rg[k] + 1]; q++) {
+                    const int l = rorder[q];
+                    if (PSV && a.rec_u) {
+                        ru[row + l] = sU[e] * A[k];
+                        rw[row + l] = sW[e] * A[k];
+                    }
+                    if (SH && a.rec_v) rv[row + l] = sV[e] * A[k];
+                }
+            }
+        }
+        started = true;
+        __syncthreads();
+    }
+
+    // ---- the tile's state (post-sponge velocities)
+    float* const* out = a.flip ? a.st : a.alt;
+    const gptr<float> o_u = G(out[0]), o_v = G(out[1]), o_w = G(out[2]);
+    const gptr<float> o_xx = G(out[3]), o_zz = G(out[4]), o_xz = G(out[5]);
+    const gptr<float> o_yx = G(out[6]), o_yz = G(out[7]);
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        if ((fl[k] & 5u) != 5u) continue;
+        const int e = t + NT * k;
+        const int c = gc[k];
+        if (PSV) {
+            o_u[c] = sU[e] * A[k];
+            o_w[c] = sW[e] * A[k];
+            o_xx[c] = sXX[e];
+            o_zz[c] = sZZ[e];
+            o_xz[c] = sXZ[e];
+        }
+        if (SH) {
+            o_v[c] = sV[e] * A[k];
+            o_yx[c] = sYX[e];
+            o_yz[c] = sYZ[e];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint, steps t_b-1 down to t_a (t_b - t_a <= K); steps at or beyond the
+// shot's record length leave its state unchanged
+
+template <bool PSV, bool SH, int TX, int TZ, int NT, int K, class S>
+__global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int tiles_x,
+                                                    int tiles_z) {
+    constexpr int RX = TX + 2 * K, RZ = TZ + 2 * K, RN = RX * RZ;
+    constexpr int NC = (RN + NT - 1) / NT;
+    constexpr int NP = PSV ? RN : 1, NS = SH ? RN : 1;
+    using H = HF<PSV, SH>;
+    __shared__ float qX[NP], qZ[NP], qS[NP], bU[NP], bW[NP];
+    __shared__ float qY[NS], qV[NS], bV[NS];
+
+    const FB& a = src.get();
+    const int tiles = tiles_x * tiles_z;
+    const int tile = xcd_tile_b(blockIdx.x, tiles);
+    const int nt = a.nt;
+    if (tile >= tiles || t_a >= nt) return;
+    const int i0 = (tile / tiles_z) * TX, j0 = (tile % tiles_z) * TZ;
+    const int ri0 = i0 - K, rj0 = j0 - K;
+    const int nx = a.nx, nz = a.nz;
+    const int t = threadIdx.x;
+    const bool has_src = a.src_map != nullptr, has_rec = a.rec_map != nullptr;
+    float* const* in = a.flip ? a.alt : a.st;
+
+    int gc[NC], sg[NC], rg[NC];
+    unsigned fl[NC];  // bit0 in grid, bit1 interior, bit2 in tile, bit3 phase 1 ok, bit4 phase 2 ok
+    float A[NC], c0[NC], c1[NC], c2[NC], c3[NC], c4[NC], d0[NC], d1[NC], d2[NC];
+    float pu[NC], pw[NC], pxx[NC], pzz[NC], pxz[NC], pv[NC], pyx[NC], pyz[NC];
+    float gl[NC], gm[NC], gx[NC], gu[NC], gw[NC], gyx[NC], gyz[NC], gr[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const int e = t + NT * k;
+        const int r = e / RZ, z = e - r * RZ;
+        const int i = ri0 + r, j = rj0 + z;
+        unsigned f = 0;
+        gc[k] = 0;
+        sg[k] = rg[k] = -1;
+        A[k] = 1.0f;
+        c0[k] = c1[k] = c2[k] = c3[k] = c4[k] = d0[k] = d1[k] = d2[k] = 0.0f;
+        pu[k] = pw[k] = pxx[k] = pzz[k] = pxz[k] = pv[k] = pyx[k] = pyz[k] = 0.0f;
+        gl[k] = gm[k] = gx[k] = gu[k] = gw[k] = gyx[k] = gyz[k] = gr[k] = 0.0f;
+        if (e < RN && i >= 0 && i < nx && j >= 0 && j < nz) {
+            f |= 1u;
+            const int c = i * nz + j;
+            gc[k] = c;
+            if (i >= 1 && i < nx - 1 && j >= 1 && j < nz - 1) f |= 2u;
+            const bool own = i >= i0 && i < i0 + TX && j >= j0 && j < j0 + TZ;
+            if (own) f |= 4u;
+            if (r >= 1 && z >= 1) f |= 8u;
+            if (r <= RX - 2 && z <= RZ - 2) f |= 16u;
+            A[k] = sponge_at(i, j, nx, nz, a.fw, a.tab);
+            if (PSV) {
+                c0[k] = G(a.iru)[c];
+                c1[k] = G(a.irw)[c];
+                c2[k] = G(a.lam)[c];
+                c3[k] = G(a.mu)[c];
+                c4[k] = G(a.mxz)[c];
+                pu[k] = G(in[0])[c];
+                pw[k] = G(in[2])[c];
+                pxx[k] = G(in[3])[c];
+                pzz[k] = G(in[4])[c];
+                pxz[k] = G(in[5])[c];
+                if (own) {
+                    gl[k] = G(a.g_lam)[c];
+                    gm[k] = G(a.g_mu)[c];
+                    gx[k] = G(a.g_mxz)[c];
+                    gu[k] = G(a.g_iru)[c];
+                    gw[k] = G(a.g_irw)[c];
+                }
+            }
+            if (SH) {
+                d0[k] = G(a.ir)[c];
+                d1[k] = G(a.myx)[c];
+                d2[k] = G(a.myz)[c];
+                pv[k] = G(in[1])[c];
+                pyx[k] = G(in[6])[c];
+                pyz[k] = G(in[7])[c];
+                if (own) {
+                    gyx[k] = G(a.g_myx)[c];
+                    gyz[k] = G(a.g_myz)[c];
+                    gr[k] = G(a.g_ir)[c];
+                }
+            }
+            if (has_src && own) sg[k] = G(a.src_map)[c];
+            if (has_rec) rg[k] = G(a.rec_map)[c];
+        } else if (e < RN) {
+            // outside the grid: the transposes read zero here (not interior)
+            if (PSV) qX[e] = qZ[e] = qS[e] = bU[e] = bW[e] = 0.0f;
+            if (SH) qY[e] = qV[e] = bV[e] = 0.0f;
+        }
+        fl[k] = f;
+    }
+
+    const size_t n = (size_t)nx * nz;
+    const float idx = a.idx, idz = a.idz, dt = a.dt, factor = a.factor;
+    const gptr<const int> sorder = G(a.src_order), sstart = G(a.src_start);
+    const gptr<const int> rorder = G(a.rec_order), rstart = G(a.rec_start);
+    const gptr<const float> wu = G(a.wav_u), wv = G(a.wav_v), ww = G(a.wav_w);
+    const gptr<const float> res_u = G(a.res_u), res_v = G(a.res_v), res_w = G(a.res_w);
+
+    for (int s = 0; s < K; s++) {
+        const int it = t_b - 1 - s;
+        if (it < t_a) break;
+        if (it >= nt) continue;
+        const gptr<const float> h =
+            G((const float*)a.hist) + (size_t)(it - a.hist_base) * (H::n * n);
+
+        // ---- phase 1a: data adjoint at the receivers; stress transposes q
+        // of every region cell from its own sponged stress adjoint
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            if (!(fl[k] & 1u)) continue;
+            const int e = t + NT * k;
+            if (rg[k] >= 0) {
+                const size_t row = (size_t)it * a.n_rec;
+                for (int q = rstart[rg[k]]; q < rstart[rg[k] + 1]; q++) {
+                    const int l = rorder[q];
+                    if (PSV) {
+                        pu[k] += res_u[row + l];
+                        pw[k] += res_w[row + l];
+                    }
+                    if (SH) pv[k] += res_v[row + l];
+                }
+            }
+            const bool in_ = fl[k] & 2u;
+            if (PSV) {
+                const float axx = dt * (A[k] * pxx[k]), azz = dt * (A[k] * pzz[k]);
+                qX[e] = in_ ? (c2[k] + 2.0f * c3[k]) * axx + c2[k] * azz : 0.0f;
+                qZ[e] = in_ ? c2[k] * axx + (c2[k] + 2.0f * c3[k]) * azz : 0.0f;
+                qS[e] = in_ ? (c4[k] * dt) * (A[k] * pxz[k]) : 0.0f;
+            }
+            if (SH) {
+                qY[e] = in_ ? (d1[k] * dt) * (A[k] * pyx[k]) : 0.0f;
+                qV[e] = in_ ? (d2[k] * dt) * (A[k] * pyz[k]) : 0.0f;
+            }
+        }
+        __syncthreads();
+
+        // ---- phase 1b: new velocity adjoint; lam / mu / mxz (myx / myz) and
+        // source-term gradients of the tile; b = dt * inv_rho * p for phase 2
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            if (!(fl[k] & 1u)) continue;
+            const int e = t + NT * k;
+            const int c = gc[k];
+            const bool in_ = fl[k] & 2u;
+            const bool own_in = (fl[k] & 6u) == 6u;
+            if (fl[k] & 8u) {
+                if (PSV) {
+                    const float nu =
+                        A[k] * pu[k] + idx * (qX[e - RZ] - qX[e]) + idz * (qS[e - 1] - qS[e]);
+                    const float nw =
+                        A[k] * pw[k] + idx * (qS[e - RZ] - qS[e]) + idz * (qZ[e - 1] - qZ[e]);
+                    if (own_in) {
+                        const float axx = dt * A[k] * pxx[k], azz = dt * A[k] * pzz[k];
+                        const float axz = dt * A[k] * pxz[k];
+                        const gptr<const float> hu = h + H::u * n, hw = h + H::w * n;
+                        const float ux = (hu[c + nz] - hu[c]) * idx, uz = (hu[c + 1] - hu[c]) * idz;
+                        const float wx = (hw[c + nz] - hw[c]) * idx, wz = (hw[c + 1] - hw[c]) * idz;
+                        gl[k] += (axx + azz) * (ux + wz);
+                        gm[k] += 2.0f * (axx * ux + azz * wz);
+                        gx[k] += axz * (uz + wx);
+                    }
+                    pu[k] = nu;
+                    pw[k] = nw;
+                }
+                if (SH) {
+                    const float nv =
+                        A[k] * pv[k] + idx * (qY[e - RZ] - qY[e]) + idz * (qV[e - 1] - qV[e]);
+                    if (own_in) {
+                        const gptr<const float> hv = h + H::v * n;
+                        const float vx = (hv[c + nz] - hv[c]) * idx, vz = (hv[c + 1] - hv[c]) * idz;
+                        gyx[k] += dt * A[k] * pyx[k] * vx;
+                        gyz[k] += dt * A[k] * pyz[k] * vz;
+                    }
+                    pv[k] = nv;
+                }
+                if (sg[k] >= 0)
+                    for (int q = sstart[sg[k]]; q < sstart[sg[k] + 1]; q++) {
+                        const size_t o = (size_t)sorder[q] * nt + it;
+                        if (PSV) {
+                            gu[k] += pu[k] * wu[o] * factor;
+                            gw[k] += pw[k] * ww[o] * factor;
+                        }
+                        if (SH) gr[k] += pv[k] * wv[o] * factor;
+                    }
+            }
+            if (PSV) {
+                bU[e] = in_ ? dt * c0[k] * pu[k] : 0.0f;
+                bW[e] = in_ ? dt * c1[k] * pw[k] : 0.0f;
+            }
+            if (SH) bV[e] = in_ ? dt * d0[k] * pv[k] : 0.0f;
+        }
+        __syncthreads();
+
+        // ---- phase 2: new stress adjoint; inv_rho_u / inv_rho_w (inv_rho)
+        // gradients of the tile
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            if ((fl[k] & 17u) != 17u) continue;
+            const int e = t + NT * k;
+            const int c = gc[k];
+            const bool own_in = (fl[k] & 6u) == 6u;
+            if (PSV) {
+                const float bu0 = bU[e], bw0 = bW[e];
+                pxx[k] = A[k] * pxx[k] + idx * (bu0 - bU[e + RZ]);
+                pxz[k] = A[k] * pxz[k] + idz * (bu0 - bU[e + 1]) + idx * (bw0 - bW[e + RZ]);
+                pzz[k] = A[k] * pzz[k] + idz * (bw0 - bW[e + 1]);
+                if (own_in) {
+                    const gptr<const float> hxx = h + H::sxx * n, hzz = h + H::szz * n;
+                    const gptr<const float> hxz = h + H::sxz * n;
+                    const float sxx_x = (hxx[c] - hxx[c - nz]) * idx;
+                    const float sxz_z = (hxz[c] - hxz[c - 1]) * idz;
+                    const float sxz_x = (hxz[c] - hxz[c - nz]) * idx;
+                    const float szz_z = (hzz[c] - hzz[c - 1]) * idz;
+                    gu[k] += pu[k] * dt * (sxx_x + sxz_z);
+                    gw[k] += pw[k] * dt * (sxz_x + szz_z);
+                }
+            }
+            if (SH) {
+                const float bv0 = bV[e];
+                pyx[k] = A[k] * pyx[k] + idx * (bv0 - bV[e + RZ]);
+                pyz[k] = A[k] * pyz[k] + idz * (bv0 - bV[e + 1]);
+                if (own_in) {
+                    const gptr<const float> hyx = h + H::syx * n, hyz = h + H::syz * n;
+                    const float syx_x = (hyx[c] - hyx[c - nz]) * idx;
+                    const float syz_z = (hyz[c] - hyz[c - 1]) * idz;
+                    gr[k] += pv[k] * dt * (syx_x + syz_z);
+                }
+            }
+        }
+    }
+
+    // ---- the tile's adjoint state and gradients
+    float* const* out = a.flip ? a.st : a.alt;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        if ((fl[k] & 5u) != 5u) continue;
+        const int c = gc[k];
+        if (PSV) {
+            G(out[0])[c] = pu[k];
+            G(out[2])[c] = pw[k];
+            G(out[3])[c] = pxx[k];
+            G(out[4])[c] = pzz[k];
+            G(out[5])[c] = pxz[k];
+            G(a.g_lam)[c] = gl[k];
+            G(a.g_mu)[c] = gm[k];
+            G(a.g_mxz)[c] = gx[k];
+            G(a.g_iru)[c] = gu[k];
+            G(a.g_irw)[c] = gw[k];
+        }
+        if (SH) {
+            G(out[1])[c] = pv[k];
+            G(out[6])[c] = pyx[k];
+            G(out[7])[c] = pyz[k];
+            G(a.g_myx)[c] = gyx[k];
+            G(a.g_myz)[c] = gyz[k];
+            G(a.g_ir)[c] = gr[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+// compiled shapes (tile rows, tile columns, threads, steps per launch)
+static const BlockCfg kFwiCfg[] = {
+    {16, 64, 512, 4}, {16, 64, 256, 2}, {32, 64, 1024, 4}, {16, 64, 512, 8}};
+static const BlockCfg kFwiDefault = {16, 64, 512, 4};
+
+int fwi_block_configs(int systems, int32_t* out, int max_n) {
+    (void)systems;
+    const int n = (int)(sizeof(kFwiCfg) / sizeof(BlockCfg));
+    for (int i = 0; i < n && i < max_n; i++) {
+        out[4 * i] = kFwiCfg[i].tx;
+        out[4 * i + 1] = kFwiCfg[i].tz;
+        out[4 * i + 2] = kFwiCfg[i].nt;
+        out[4 * i + 3] = kFwiCfg[i].k;
+    }
+    return n;
+}
+
+static int fwi_choose(const trtm_fwi_desc* d, BlockCfg* out) {
+    if (d->tile_x <= 0 && d->tile_z <= 0 && d->block_steps <= 0) {
+        *out = kFwiDefault;
+        return TRTM_OK;
+    }
+    for (const BlockCfg& c : kFwiCfg)
+        if ((d->tile_x <= 0 || c.tx == d->tile_x) && (d->tile_z <= 0 || c.tz == d->tile_z) &&
+            (d->block_steps <= 0 || c.k == d->block_steps)) {
+            *out = c;
+            return TRTM_OK;
+        }
+    return fail(TRTM_ERR_UNSUPPORTED, "blocked FWI kernel has no tile_x=%d tile_z=%d block_steps=%d build",
+                d->tile_x, d->tile_z, d->block_steps);
+}
+
+static FB make_fb(const trtm_fwi_desc* d, bool adjoint) {
+    FB a;
+    a.nx = d->nx; a.nz = d->nz; a.nt = d->nt; a.fw = d->fw;
+    a.n_rec = d->n_rec; a.hist_base = d->hist_base; a.hist_len = d->hist_len; a.flip = 0;
+    a.dt = (float)d->dt; a.idx = (float)d->inv_dx; a.idz = (float)d->inv_dz;
+    a.factor = (float)d->src_factor;
+    a.tab = (const float*)d->sponge_table;
+    a.lam = (const float*)d->lam; a.mu = (const float*)d->mu; a.mxz = (const float*)d->mxz;
+    a.myx = (const float*)d->myx; a.myz = (const float*)d->myz; a.ir = (const float*)d->inv_rho;
+    a.iru = (const float*)d->inv_rho_u; a.irw = (const float*)d->inv_rho_w;
+    void* const fwd[8] = {d->u, d->v, d->w, d->sxx, d->szz, d->sxz, d->syx, d->syz};
+    void* const adj[8] = {d->pu, d->pv, d->pw, d->psxx, d->pszz, d->psxz, d->psyx, d->psyz};
+    const size_t n = (size_t)d->nx * d->nz;
+    for (int f = 0; f < 8; f++) {
+        a.st[f] = (float*)(adjoint ? adj[f] : fwd[f]);
+        a.alt[f] = (float*)d->workspace + f * n;
+    }
+    a.src_map = d->n_src_groups > 0 ? d->cell_map : nullptr;
+    const bool records = !adjoint && (d->rec_u || d->rec_v);
+    a.rec_map = d->n_rec_groups > 0 && (adjoint || records) ? d->cell_map + n : nullptr;
+    a.src_order = d->src_order; a.src_start = d->src_group_start;
+    a.rec_order = d->rec_order; a.rec_start = d->rec_group_start;
+    a.wav_u = (const float*)d->wav_u; a.wav_v = (const float*)d->wav_v;
+    a.wav_w = (const float*)d->wav_w;
+    a.rec_u = (float*)d->rec_u; a.rec_v = (float*)d->rec_v; a.rec_w = (float*)d->rec_w;
+    a.res_u = (const float*)d->res_u; a.res_v = (const float*)d->res_v;
+    a.res_w = (const float*)d->res_w;
+    a.hist = (float*)d->hist;
+    a.g_lam = (float*)d->g_lam; a.g_mu = (float*)d->g_mu; a.g_mxz = (float*)d->g_mxz;
+    a.g_myx = (float*)d->g_myx; a.g_myz = (float*)d->g_myz; a.g_ir = (float*)d->g_ir;
+    a.g_iru = (float*)d->g_iru; a.g_irw = (float*)d->g_irw;
+    return a;
+}
+
+template <bool PSV, bool SH, int TX, int TZ, int NT, int K, class S>
+static void launch_fb(bool adjoint, bool hist, const S& src, int shots, int t_a, int t_b,
+                      int tiles_x, int tiles_z, hipStream_t s) {
+    const int tiles = tiles_x * tiles_z;
+    const dim3 grid(8 * ((tiles + 7) / 8), 1, shots);
+    if (adjoint)
+        hipLaunchKernelGGL((k_fwi_adj_blk<PSV, SH, TX, TZ, NT, K, S>), grid, dim3(NT), 0, s, src,
+                           t_a, t_b, tiles_x, tiles_z);
+    else if (hist)
+        hipLaunchKernelGGL((k_fwi_fwd_blk<PSV, SH, TX, TZ, NT, K, true, S>), grid, dim3(NT), 0, s,
+                           src, t_a, t_b, tiles_x, tiles_z);
+    else
+        hipLaunchKernelGGL((k_fwi_fwd_blk<PSV, SH, TX, TZ, NT, K, false, S>), grid, dim3(NT), 0,
+                           s, src, t_a, t_b, tiles_x, tiles_z);
+}
+
+template <bool PSV, bool SH, class S>
+static int launch_cfg(const BlockCfg& c, bool adjoint, bool hist, const S& src, int shots,
+                      int t_a, int t_b, int nx, int nz, hipStream_t s) {
+    const int tx_ = (nx + c.tx - 1) / c.tx, tz_ = (nz + c.tz - 1) / c.tz;
+#define TRTM_FB(TX, TZ, NT, K)                                                                  \
+    if (c.tx == TX && c.tz == TZ && c.nt == NT && c.k == K) {                                   \
+        launch_fb<PSV, SH, TX, TZ, NT, K, S>(adjoint, hist, src, shots, t_a, t_b, tx_, tz_, s); \
+        return TRTM_OK;                                                                         \
+    }
+    TRTM_FB(16, 64, 512, 4)
+    TRTM_FB(16, 64, 256, 2)
+    TRTM_FB(32, 64, 1024, 4)
+    TRTM_FB(16, 64, 512, 8)
+#undef TRTM_FB
+    return fail(TRTM_ERR_UNSUPPORTED, "blocked FWI shape %dx%d/%d/%d not compiled", c.tx, c.tz,
+                c.nt, c.k);
+}
+
+template <class S>
+static int launch_sys(int systems, const BlockCfg& c, bool adjoint, bool hist, const S& src,
+                      int shots, int t_a, int t_b, int nx, int nz, hipStream_t s) {
+    if (systems == (TRTM_PSV | TRTM_SH))
+        return launch_cfg<true, true>(c, adjoint, hist, src, shots, t_a, t_b, nx, nz, s);
+    if (systems == TRTM_PSV)
+        return launch_cfg<true, false>(c, adjoint, hist, src, shots, t_a, t_b, nx, nz, s);
+    return launch_cfg<false, true>(c, adjoint, hist, src, shots, t_a, t_b, nx, nz, s);
+}
+
+// Steps [it_begin, it_end) of n shots, ascending (forward) or descending
+// (adjoint) in launches of K steps; shots in groups of kShotsPerLaunch.  Each
+// shot's state alternates between its buffers and the workspace; a shot takes
+// part in a launch while the launch's first step is below its record length.
+static int fwi_block_run(const trtm_fwi_desc* d, int n, int it_begin, int it_end, bool adjoint,
+                         hipStream_t s) {
+    if (it_begin >= it_end) return TRTM_OK;
+    BlockCfg c;
+    int rc = fwi_choose(d, &c);
+    if (rc) return rc;
+    for (int q = 0; q < n; q++)
+        if (!d[q].workspace)
+            return fail(TRTM_ERR_ARG, "shot %d: blocked FWI kernel needs the workspace", q);
+    const bool hist = !adjoint && d->hist != nullptr;
+    std::vector<FB> fb(n);
+    for (int q = 0; q < n; q++) fb[q] = make_fb(&d[q], adjoint);
+    const int K = c.k, groups = (n + kShotsPerLaunch - 1) / kShotsPerLaunch;
+    const int steps = it_end - it_begin, launches = (steps + K - 1) / K;
+    for (int l = 0; l < launches; l++) {
+        const int t_a = adjoint ? std::max(it_end - (l + 1) * K, it_begin) : it_begin + l * K;
+        const int t_b = adjoint ? it_end - l * K : std::min(t_a + K, it_end);
+        for (int g = 0; g < groups; g++) {
+            const int q0 = g * kShotsPerLaunch, m = std::min(kShotsPerLaunch, n - q0);
+            if (n == 1) {
+                rc = launch_sys(d->systems, c, adjoint, hist, ByValue<FB>{fb[0]}, 1, t_a, t_b,
+                                d->nx, d->nz, s);
+            } else {
+                ByArray<FB> arr;
+                for (int q = 0; q < m; q++) arr.v[q] = fb[q0 + q];
+                rc = launch_sys(d->systems, c, adjoint, hist, arr, m, t_a, t_b, d->nx, d->nz, s);
+            }
+            if (rc) return rc;
+            for (int q = q0; q < q0 + m; q++)
+                if (t_a < fb[q].nt) fb[q].flip ^= 1;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return fail(TRTM_ERR_HIP, "blocked FWI %s launch: %s", adjoint ? "adjoint" : "forward",
+                        hipGetErrorString(e));
+    }
+    // state left in the workspace goes back to the caller's buffers
+    const size_t bytes = (size_t)d->nx * d->nz * sizeof(float);
+    for (int q = 0; q < n; q++) {
+        if (!fb[q].flip) continue;
+        for (int f = 0; f < 8; f++) {
+            const bool psv_f = f == 0 || f == 2 || (f >= 3 && f <= 5);
+            if (!(psv_f ? (d->systems & TRTM_PSV) : (d->systems & TRTM_SH))) continue;
+            const hipError_t e =
+                hipMemcpyAsync(fb[q].st[f], fb[q].alt[f], bytes, hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess)
+                return fail(TRTM_ERR_HIP, "blocked FWI state copy: %s", hipGetErrorString(e));
+        }
+    }
+    return TRTM_OK;
+}
+
+int fwi_block_forward(const trtm_fwi_desc* d, int n, int it_begin, int it_end, hipStream_t s) {
+    return fwi_block_run(d, n, it_begin, it_end, false, s);
+}
+
+int fwi_block_adjoint(const trtm_fwi_desc* d, int n, int it_begin, int it_end, hipStream_t s) {
+    return fwi_block_run(d, n, it_begin, it_end, true, s);
+}
+
+}  // namespace trtm

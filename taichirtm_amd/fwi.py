@@ -20,6 +20,12 @@ back-propagator, not the adjoint of its forward step.  Here:
 
 ``precision="float64"`` runs the same kernels in double, for finite-difference
 and dot-product checks of the float32 gradient.  Flat free surface only.
+
+Step kernels (``kernel``): ``"blocked"`` (float32) advances K steps per launch
+on LDS tiles with a halo, forward and adjoint, the gradient accumulated in
+registers; ``"step"`` runs one step per launch (float32 or float64).  Both
+give the same records and gradients bit for bit; ``"auto"`` is ``"blocked"``
+in float32 and ``"step"`` in float64.
 """
 from __future__ import annotations
 
@@ -36,6 +42,7 @@ from .batch import ShotSpec, checkpoint_interval
 from .field import as_tensor, device, stream_handle
 
 _REAL = {"float32": (torch.float32, 4), "float64": (torch.float64, 8)}
+_KERNELS = {"auto": _lib.KERNEL_AUTO, "step": _lib.KERNEL_SIMPLE, "blocked": _lib.KERNEL_BLOCKED}
 
 
 def _plan(nx, nz, loc):
@@ -64,9 +71,16 @@ class ElasticFWI:
     """Forward records and exact gradients for shots on one (nx, nz) grid."""
 
     def __init__(self, nx, nz, dx, dz, fs, absorbing_frame, shots: Sequence[ShotSpec],
-                 precision="float32", checkpoint_every=None):
+                 precision="float32", checkpoint_every=None, kernel="auto", tile_x=0, tile_z=0,
+                 block_steps=0):
         if precision not in _REAL:
             raise ValueError("precision must be float32 or float64")
+        if kernel not in _KERNELS:
+            raise ValueError("kernel must be one of %s" % sorted(_KERNELS))
+        if kernel == "blocked" and precision != "float32":
+            raise ValueError("the blocked kernels are float32 only")
+        self.kernel = kernel
+        self.shape = (int(tile_x), int(tile_z), int(block_steps))
         self.nx, self.nz = int(nx), int(nz)
         self.dx, self.dz, self.fs = float(dx), float(dz), float(fs)
         self.fw = int(absorbing_frame)
@@ -96,6 +110,10 @@ class ElasticFWI:
             self.geo.append({"src": src, "rec": rec, "wav": wav, "nt": s.nt,
                              "nr": len(rec[0]), "ns": len(src[0]),
                              "map": torch.empty(2 * nx * nz, dtype=torch.int32, device=dev)})
+        self.blocked = kernel == "blocked" or (kernel == "auto" and precision == "float32")
+        for g in self.geo:  # second state buffer of the blocked kernels
+            g["ws"] = (torch.empty(8 * nx * nz, dtype=self.dtype, device=dev) if self.blocked
+                       else None)
         for g in self.geo:  # cell -> source / receiver group
             d = _lib.FwiDesc()
             d.struct_size = ctypes.sizeof(_lib.FwiDesc)
@@ -149,6 +167,9 @@ class ElasticFWI:
         d.n_rec, d.n_rec_groups = len(loc), ng
         d.rec_loc, d.rec_order, d.rec_group_start = _ptr(loc), _ptr(order), _ptr(start)
         d.cell_map = _ptr(g["map"])
+        d.kernel = _KERNELS[self.kernel]
+        d.tile_x, d.tile_z, d.block_steps = self.shape
+        d.workspace = _ptr(g["ws"]) if g["ws"] is not None else None
         return d
 
     def _systems(self):

@@ -153,3 +153,62 @@ def test_fwi_forward_records_equal_forward_modeling():
     rec = p.forward(*[_t(x, torch.float32) for x in (vp, vs, rho)])[0].cpu().numpy()
     for c in range(3):
         assert same(rec[c], h["seis"][c])
+
+
+BLOCK_SHAPES = [(16, 64, 4), (16, 64, 2), (32, 64, 4), (16, 64, 8)]
+
+
+def _uneven(shots, nts, spreads):
+    """Shots with their own record lengths and receiver spreads."""
+    out = []
+    for s, nt, (a, b) in zip(shots, nts, spreads):
+        out.append(ShotSpec(s.src_loc, tuple(np.asarray(w)[:, :nt] for w in s.wavelets),
+                            np.asarray(s.rec_loc)[a:b]))
+    return out
+
+
+def _records_and_gradient(grid, shots, m0, mt, **kw):
+    p = ElasticFWI(*grid, shots, precision="float32", **kw)
+    obs = [o.clone() for o in p.forward(*[_t(x, torch.float32) for x in mt])]
+    recs = [r.clone() for r in p.forward(*[_t(x, torch.float32) for x in m0])]
+    res = [r - o for r, o in zip(recs, obs)]
+    g = p.gradient(res)
+    return recs, g, {k: v.clone() for k, v in p.coefficient_gradient.items()}
+
+
+@pytest.mark.parametrize("shape", BLOCK_SHAPES, ids=lambda s: "%dx%dk%d" % s)
+@pytest.mark.parametrize("system", ["psv", "sh", "both"])
+def test_blocked_kernels_bitwise_equal_step_kernels(system, shape):
+    """K steps per launch (LDS tiles, halo recomputed) against one step per
+    launch: records, coefficient gradients and (vp, vs, rho) gradients bit for
+    bit, for 3 shots of uneven record length and receiver spread, a checkpoint
+    interval that is no multiple of K, and a source next to the grid edge."""
+    _case("FWI blocked %s %s" % (system, shape))
+    grid, shots, m0, mt = problem(system, nt=130, n_shots=3)
+    shots = _uneven(shots, (130, 97, 121), ((0, 52), (5, 30), (20, 45)))
+    shots[2] = ShotSpec(np.array([[1, 1]], np.int32), shots[2].wavelets, shots[2].rec_loc)
+    tx, tz, k = shape
+    ref = _records_and_gradient(grid, shots, m0, mt, kernel="step", checkpoint_every=29)
+    got = _records_and_gradient(grid, shots, m0, mt, kernel="blocked", tile_x=tx, tile_z=tz,
+                                block_steps=k, checkpoint_every=29)
+    for a, b in zip(got[0], ref[0]):
+        assert torch.equal(a, b)
+    for a, b in zip(got[1], ref[1]):
+        assert torch.equal(a, b)
+    for key in ref[2]:
+        assert torch.equal(got[2][key], ref[2][key]), key
+
+
+def test_blocked_kernels_many_shots_bitwise():
+    """Ten shots: two launches of up to eight shots per step block."""
+    _case("FWI blocked 10 shots")
+    grid, shots, m0, mt = problem("psv", nt=90, n_shots=10)
+    shots = [ShotSpec(np.array([[4 + 5 * q, 2]], np.int32), s.wavelets, s.rec_loc)
+             for q, s in enumerate(shots)]
+    shots = _uneven(shots, [90 - 3 * q for q in range(10)], [(q, 40 + q) for q in range(10)])
+    ref = _records_and_gradient(grid, shots, m0, mt, kernel="step", checkpoint_every=20)
+    got = _records_and_gradient(grid, shots, m0, mt, kernel="blocked", checkpoint_every=20)
+    for a, b in zip(got[0], ref[0]):
+        assert torch.equal(a, b)
+    for a, b in zip(got[1], ref[1]):
+        assert torch.equal(a, b)
