@@ -111,15 +111,16 @@ def job_main(args):
         from taichirtm_amd.fwi import ElasticFWI, l2_misfit_and_gradient
         cfg, start = configs.psv_fwi()
         shots = _shot_specs(cfg, args.shots)
+        fwi_kernel = args.kernel if args.kernel in ("tuned", "blocked", "step") else "auto"
         prob = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, shots,
-                          precision="float32")
+                          precision="float32", kernel=fwi_kernel)
         t = lambda x: torch.as_tensor(np.asarray(x, np.float32), device="cuda")
         obs = [o.clone() for o in prob.forward(t(cfg.vp), t(cfg.vs), t(cfg.rho))]
         m0 = [t(x) for x in (start.vp, start.vs, start.rho)]
 
         def step():
             l2_misfit_and_gradient(prob, *m0, obs)
-            return {}
+            return dict(prob.shapes)
         desc = ("P-SV FWI gradient (forward + checkpointed exact adjoint), %d shots of %s"
                 % (len(shots), cfg.name))
     for _ in range(args.warmup):

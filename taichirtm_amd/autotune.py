@@ -37,13 +37,23 @@ def candidates(systems: int) -> List[Shape]:
     return out
 
 
-def tune(key: tuple, systems: int, trial: Callable[[Shape], None], repeats: int = 2) -> Shape:
-    """Fastest shape for ``key``; ``trial(shape)`` runs the timing window once
-    with that shape on the current stream."""
+def fwi_candidates(systems: int) -> List[Shape]:
+    """Every compiled shape of the blocked FWI kernels that fits these systems."""
+    buf = (ctypes.c_int32 * (4 * 64))()
+    n = _lib.call("trtm_fwi_block_configs", systems, buf, 64)
+    return [("fwi", int(buf[4 * i]), int(buf[4 * i + 1]), int(buf[4 * i + 3]))
+            for i in range(min(n, 64))]
+
+
+def tune(key: tuple, systems: int, trial: Callable[[Shape], None], repeats: int = 2,
+         shapes: List[Shape] = None) -> Shape:
+    """Fastest shape for ``key`` among ``shapes`` (default: both blocked
+    step-kernel families); ``trial(shape)`` runs the timing window once with
+    that shape on the current stream."""
     if key in _CACHE:
         return _CACHE[key]
     times = []
-    for shape in candidates(systems):
+    for shape in (shapes if shapes is not None else candidates(systems)):
         trial(shape)  # first call: tile lists, caches
         best = None
         for _ in range(repeats):

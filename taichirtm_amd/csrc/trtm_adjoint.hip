@@ -503,15 +503,17 @@ static int check_fwi(const trtm_fwi_desc* d, int it_begin, int it_end) {
     if (d->kernel != TRTM_KERNEL_AUTO && d->kernel != TRTM_KERNEL_SIMPLE &&
         d->kernel != TRTM_KERNEL_BLOCKED)
         return fail(TRTM_ERR_ARG, "kernel must be TRTM_KERNEL_AUTO, _SIMPLE or _BLOCKED");
-    if (d->kernel == TRTM_KERNEL_BLOCKED && (d->real_bytes != 4 || !d->workspace))
-        return fail(TRTM_ERR_ARG, "the blocked FWI kernel needs float and the workspace");
+    if (d->kernel == TRTM_KERNEL_BLOCKED && (d->real_bytes != 4 || !d->workspace || d->nx < 3 ||
+                                             d->nz < 3))
+        return fail(TRTM_ERR_ARG, "the blocked FWI kernel needs float, the workspace and a grid "
+                                  "of at least 3 x 3");
     return TRTM_OK;
 }
 
 // blocked kernels: float with a workspace, unless one step per launch is asked for
 static bool use_block(const trtm_fwi_desc* d) {
     if (d->kernel == TRTM_KERNEL_SIMPLE) return false;
-    return d->real_bytes == 4 && d->workspace != nullptr;
+    return d->real_bytes == 4 && d->workspace != nullptr && d->nx >= 3 && d->nz >= 3;
 }
 
 static int hist_fields(int systems) {

@@ -23,8 +23,9 @@
 // order as the per-step kernels and written once per launch.
 //
 // State ping-pongs between the caller's buffers and a workspace copy (tiles
-// read the halo that neighbouring tiles write); a shot whose state ends in the
-// workspace is copied back at the end of the call.
+// read the halo that neighbouring tiles write).  A call issues an even number
+// of launches, so the state ends in the caller's buffers; a shot whose record
+// ends inside the range (and so skips launches) is copied back at the end.
 #include <algorithm>
 #include <vector>
 
@@ -60,6 +61,85 @@ struct HF {
     static constexpr int n = (PSV ? 5 : 0) + (SH ? 3 : 0);
 };
 
+// Entry of the sponge table that sponge_at reads for cell (i, j), or -1 for
+// the coefficient 1 (same strip order as sponge_at).
+__device__ __forceinline__ int sponge_slot(int i, int j, int nx, int nz, int fw) {
+    const int jb = nz - 1 - j;
+    if (jb < fw && i >= jb && i < nx - jb) return fw - jb;
+    const int ir = nx - 1 - i;
+    if (ir < fw && j < nz - ir - 1) return fw - ir;
+    if (i < fw && j < nz - i - 1) return fw - i;
+    return -1;
+}
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// Element c of a global array as a uniform base plus a 32-bit byte offset:
+// the saddr form of the global loads and stores, no 64-bit address per lane
+// (grids below 2^30 cells).
+template <class T>
+__device__ __forceinline__ T ld(const T* base, int c) {
+    const gptr<const char> b = (gptr<const char>)(const char*)base;
+    return *(gptr<const T>)(b + (unsigned)c * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st(T* base, int c, T v) {
+    const gptr<char> b = (gptr<char>)(char*)base;
+    *(gptr<T>)(b + (unsigned)c * (unsigned)sizeof(T)) = v;
+}
+
+// Workgroup barrier ordering LDS only: loads still in flight to LDS (the
+// asynchronous history copies) are not waited for here but by vm_wait().
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+// wait for every vector-memory load of this wave (vmcnt 0)
+__device__ __forceinline__ void vm_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Adjoint history window of a tile: rows i0-1 .. i0+TX, columns j0-1 .. j0+TZ,
+// in 64-cell chunks per field, two buffers (the step being taken and the next).
+template <bool PSV, bool SH, int TX, int TZ>
+struct HistWindow {
+    static constexpr int HX = TX + 2, HZ = TZ + 2;
+    static constexpr int chunks = (HX * HZ + 63) / 64;
+    static constexpr int fields = (PSV ? 5 : 0) + (SH ? 3 : 0);
+    static constexpr int floats = 2 * fields * chunks * 64;
+};
+
+template <bool PSV, bool SH, int TX, int TZ, int K>
+constexpr size_t adj_lds_bytes() {
+    constexpr int RN = (TX + 2 * K) * (TZ + 2 * K);
+    return 4 * ((size_t)HistWindow<PSV, SH, TX, TZ>::floats + (PSV ? 5 : 0) * RN + (SH ? 3 : 0) * RN);
+}
+constexpr size_t kLdsLimit = 160 * 1024;
+
+// Region set-up shared by both kernels.  Every global load of the prologue is
+// unconditional (cells outside the grid read cell 0 and are masked by their
+// flags), so the loads of all of a thread's cells are in flight together.
+struct RegionCell {
+    int c;       // global offset (0 outside the grid)
+    unsigned f;  // bit0 in grid, bit1 interior, bit2 in tile, bit3 reads -1 ok, bit4 reads +1 ok
+    int slot;    // sponge table entry, -1 for 1
+};
+
+template <int TX, int TZ, int K>
+__device__ __forceinline__ RegionCell region_cell(int e, int i0, int j0, int nx, int nz, int fw) {
+    constexpr int RX = TX + 2 * K, RZ = TZ + 2 * K, RN = RX * RZ;
+    const int r = e / RZ, z = e - r * RZ;
+    const int i = i0 - K + r, j = j0 - K + z;
+    RegionCell rc{0, 0u, -1};
+    if (e < RN && i >= 0 && i < nx && j >= 0 && j < nz) {
+        rc.c = i * nz + j;
+        rc.f = 1u;
+        if (i >= 1 && i < nx - 1 && j >= 1 && j < nz - 1) rc.f |= 2u;
+        if (i >= i0 && i < i0 + TX && j >= j0 && j < j0 + TZ) rc.f |= 4u;
+        if (r >= 1 && z >= 1) rc.f |= 8u;
+        if (r <= RX - 2 && z <= RZ - 2) rc.f |= 16u;
+        rc.slot = sponge_slot(i, j, nx, nz, fw);
+    }
+    return rc;
+}
+
 // ---------------------------------------------------------------------------
 // forward, steps t_a .. t_b-1 (t_b - t_a <= K)
 
@@ -87,66 +167,54 @@ __global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int
     int gc[NC], sg[NC], rg[NC];
     unsigned fl[NC];  // bit0 in grid, bit1 interior, bit2 in tile, bit3 V ok, bit4 S ok
     float A[NC], c0[NC], c1[NC], c2[NC], c3[NC], c4[NC], d0[NC], d1[NC], d2[NC];
+    int slot[NC];
 #pragma unroll
     for (int k = 0; k < NC; k++) {
-        const int e = t + NT * k;
-        const int r = e / RZ, z = e - r * RZ;
-        const int i = ri0 + r, j = rj0 + z;
-        unsigned f = 0;
-        gc[k] = 0;
-        sg[k] = rg[k] = -1;
-        A[k] = 1.0f;
-        c0[k] = c1[k] = c2[k] = c3[k] = c4[k] = d0[k] = d1[k] = d2[k] = 0.0f;
-        if (e < RN && i >= 0 && i < nx && j >= 0 && j < nz) {
-            f |= 1u;
-            const int c = i * nz + j;
-            gc[k] = c;
-            if (i >= 1 && i < nx - 1 && j >= 1 && j < nz - 1) f |= 2u;
-            const bool own = i >= i0 && i < i0 + TX && j >= j0 && j < j0 + TZ;
-            if (own) f |= 4u;
-            if (r >= 1 && z >= 1) f |= 8u;
-            if (r <= RX - 2 && z <= RZ - 2) f |= 16u;
-            A[k] = sponge_at(i, j, nx, nz, a.fw, a.tab);
-            if (PSV) {
-                c0[k] = G(a.iru)[c];
-                c1[k] = G(a.irw)[c];
-                c2[k] = G(a.lam)[c];
-                c3[k] = G(a.mu)[c];
-                c4[k] = G(a.mxz)[c];
-            }
-            if (SH) {
-                d0[k] = G(a.ir)[c];
-                d1[k] = G(a.myx)[c];
-                d2[k] = G(a.myz)[c];
-            }
-            if (has_src) sg[k] = G(a.src_map)[c];
-            if (has_rec && own) rg[k] = G(a.rec_map)[c];
-        }
-        fl[k] = f;
+        const RegionCell rc = region_cell<TX, TZ, K>(t + NT * k, i0, j0, nx, nz, a.fw);
+        gc[k] = rc.c;
+        fl[k] = rc.f;
+        slot[k] = rc.slot;
     }
-
     {
         float* const* in = a.flip ? a.alt : a.st;
-        const gptr<float> in_u = G(in[0]), in_v = G(in[1]), in_w = G(in[2]);
-        const gptr<float> in_xx = G(in[3]), in_zz = G(in[4]), in_xz = G(in[5]);
-        const gptr<float> in_yx = G(in[6]), in_yz = G(in[7]);
+        const gptr<const float> tab = G(a.tab);
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             const int e = t + NT * k;
-            if (e >= RN) continue;
-            const bool g = fl[k] & 1u;
             const int c = gc[k];
+            const bool g = fl[k] & 1u;
+            const float sa = tab[slot[k] > 0 ? slot[k] : 0];
+            A[k] = slot[k] >= 0 ? sa : 1.0f;
+            sg[k] = has_src ? ld(a.src_map, c) : -1;
+            rg[k] = has_rec ? ld(a.rec_map, c) : -1;
+            if (!g) sg[k] = -1;
+            if ((fl[k] & 5u) != 5u) rg[k] = -1;
             if (PSV) {
-                sU[e] = g ? in_u[c] : 0.0f;
-                sW[e] = g ? in_w[c] : 0.0f;
-                sXX[e] = g ? in_xx[c] : 0.0f;
-                sZZ[e] = g ? in_zz[c] : 0.0f;
-                sXZ[e] = g ? in_xz[c] : 0.0f;
+                c0[k] = ld(a.iru, c);
+                c1[k] = ld(a.irw, c);
+                c2[k] = ld(a.lam, c);
+                c3[k] = ld(a.mu, c);
+                c4[k] = ld(a.mxz, c);
+                const float u = ld(in[0], c), w = ld(in[2], c), xx = ld(in[3], c), zz = ld(in[4], c),
+                            xz = ld(in[5], c);
+                if (e < RN) {
+                    sU[e] = g ? u : 0.0f;
+                    sW[e] = g ? w : 0.0f;
+                    sXX[e] = g ? xx : 0.0f;
+                    sZZ[e] = g ? zz : 0.0f;
+                    sXZ[e] = g ? xz : 0.0f;
+                }
             }
             if (SH) {
-                sV[e] = g ? in_v[c] : 0.0f;
-                sYX[e] = g ? in_yx[c] : 0.0f;
-                sYZ[e] = g ? in_yz[c] : 0.0f;
+                d0[k] = ld(a.ir, c);
+                d1[k] = ld(a.myx, c);
+                d2[k] = ld(a.myz, c);
+                const float v = ld(in[1], c), yx = ld(in[6], c), yz = ld(in[7], c);
+                if (e < RN) {
+                    sV[e] = g ? v : 0.0f;
+                    sYX[e] = g ? yx : 0.0f;
+                    sYZ[e] = g ? yz : 0.0f;
+                }
             }
         }
     }
@@ -164,10 +232,10 @@ __global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int
     for (int s = 0; s < K; s++) {
         const int it = t_a + s;
         if (it >= hi) break;
-        gptr<float> hh = nullptr;
+        float* hh = nullptr;
         if (HIST) {
             const int slot = it - a.hist_base;
-            if (slot >= 0 && slot < a.hist_len) hh = G(a.hist) + (size_t)slot * (H::n * n);
+            if (slot >= 0 && slot < a.hist_len) hh = a.hist + (size_t)slot * (H::n * n);
         }
         // ---- history: the stresses this step's velocity update reads
         if (HIST && hh) {
@@ -177,13 +245,13 @@ __global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int
                 const int e = t + NT * k;
                 const int c = gc[k];
                 if (PSV) {
-                    hh[H::sxx * n + c] = sXX[e];
-                    hh[H::szz * n + c] = sZZ[e];
-                    hh[H::sxz * n + c] = sXZ[e];
+                    st(hh + H::sxx * n, c, sXX[e]);
+                    st(hh + H::szz * n, c, sZZ[e]);
+                    st(hh + H::sxz * n, c, sXZ[e]);
                 }
                 if (SH) {
-                    hh[H::syx * n + c] = sYX[e];
-                    hh[H::syz * n + c] = sYZ[e];
+                    st(hh + H::syx * n, c, sYX[e]);
+                    st(hh + H::syz * n, c, sYZ[e]);
                 }
             }
         }
@@ -216,8 +284,8 @@ __global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int
                 sU[e] = u;
                 sW[e] = w;
                 if (HIST && hh && (fl[k] & 4u)) {
-                    hh[H::u * n + gc[k]] = u;
-                    hh[H::w * n + gc[k]] = w;
+                    st(hh + H::u * n, gc[k], u);
+                    st(hh + H::w * n, gc[k], w);
                 }
             }
             if (SH) {
@@ -232,7 +300,7 @@ __global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int
                     for (int q = sstart[sg[k]]; q < sstart[sg[k] + 1]; q++)
                         v = v + (wv[(size_t)sorder[q] * nt + it] * factor) * d0[k];
                 sV[e] = v;
-                if (HIST && hh && (fl[k] & 4u)) hh[H::v * n + gc[k]] = v;
+                if (HIST && hh && (fl[k] & 4u)) st(hh + H::v * n, gc[k], v);
             }
         }
         __syncthreads();
@@ -288,25 +356,22 @@ __global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int
 
     // ---- the tile's state (post-sponge velocities)
     float* const* out = a.flip ? a.st : a.alt;
-    const gptr<float> o_u = G(out[0]), o_v = G(out[1]), o_w = G(out[2]);
-    const gptr<float> o_xx = G(out[3]), o_zz = G(out[4]), o_xz = G(out[5]);
-    const gptr<float> o_yx = G(out[6]), o_yz = G(out[7]);
 #pragma unroll
     for (int k = 0; k < NC; k++) {
         if ((fl[k] & 5u) != 5u) continue;
         const int e = t + NT * k;
         const int c = gc[k];
         if (PSV) {
-            o_u[c] = sU[e] * A[k];
-            o_w[c] = sW[e] * A[k];
-            o_xx[c] = sXX[e];
-            o_zz[c] = sZZ[e];
-            o_xz[c] = sXZ[e];
+            st(out[0], c, sU[e] * A[k]);
+            st(out[2], c, sW[e] * A[k]);
+            st(out[3], c, sXX[e]);
+            st(out[4], c, sZZ[e]);
+            st(out[5], c, sXZ[e]);
         }
         if (SH) {
-            o_v[c] = sV[e] * A[k];
-            o_yx[c] = sYX[e];
-            o_yz[c] = sYZ[e];
+            st(out[1], c, sV[e] * A[k]);
+            st(out[6], c, sYX[e]);
+            st(out[7], c, sYZ[e]);
         }
     }
 }
@@ -322,8 +387,14 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
     constexpr int NC = (RN + NT - 1) / NT;
     constexpr int NP = PSV ? RN : 1, NS = SH ? RN : 1;
     using H = HF<PSV, SH>;
+    using W = HistWindow<PSV, SH, TX, TZ>;
+    static_assert(adj_lds_bytes<PSV, SH, TX, TZ, K>() <= kLdsLimit, "LDS");
     __shared__ float qX[NP], qZ[NP], qS[NP], bU[NP], bW[NP];
     __shared__ float qY[NS], qV[NS], bV[NS];
+    // history windows [field][chunk * 64 + lane] of alternate steps (two
+    // arrays, so the LDS reads of one are not ordered after the loads into the
+    // other)
+    __shared__ float hwA[W::floats / 2], hwB[W::floats / 2];
 
     const FB& a = src.get();
     const int tiles = tiles_x * tiles_z;
@@ -337,73 +408,78 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
     const bool has_src = a.src_map != nullptr, has_rec = a.rec_map != nullptr;
     float* const* in = a.flip ? a.alt : a.st;
 
+    // The gradient and history loads of a region cell outside the tile (or
+    // off the interior) read a nearby tile cell (an interior one) instead:
+    // always in bounds, and no cache lines beyond the tile's.
+    auto tile_cell = [&](int e, bool interior) {
+        const int r = e / RZ, z = e - r * RZ;
+        int ti = min(clampi(ri0 + r, i0, i0 + TX - 1), nx - 1);
+        int tj = min(clampi(rj0 + z, j0, j0 + TZ - 1), nz - 1);
+        if (interior) {
+            ti = clampi(ti, 1, nx - 2);
+            tj = clampi(tj, 1, nz - 2);
+        }
+        return ti * nz + tj;
+    };
     int gc[NC], sg[NC], rg[NC];
     unsigned fl[NC];  // bit0 in grid, bit1 interior, bit2 in tile, bit3 phase 1 ok, bit4 phase 2 ok
     float A[NC], c0[NC], c1[NC], c2[NC], c3[NC], c4[NC], d0[NC], d1[NC], d2[NC];
     float pu[NC], pw[NC], pxx[NC], pzz[NC], pxz[NC], pv[NC], pyx[NC], pyz[NC];
     float gl[NC], gm[NC], gx[NC], gu[NC], gw[NC], gyx[NC], gyz[NC], gr[NC];
+    int slot[NC];
 #pragma unroll
     for (int k = 0; k < NC; k++) {
         const int e = t + NT * k;
-        const int r = e / RZ, z = e - r * RZ;
-        const int i = ri0 + r, j = rj0 + z;
-        unsigned f = 0;
-        gc[k] = 0;
-        sg[k] = rg[k] = -1;
-        A[k] = 1.0f;
-        c0[k] = c1[k] = c2[k] = c3[k] = c4[k] = d0[k] = d1[k] = d2[k] = 0.0f;
-        pu[k] = pw[k] = pxx[k] = pzz[k] = pxz[k] = pv[k] = pyx[k] = pyz[k] = 0.0f;
-        gl[k] = gm[k] = gx[k] = gu[k] = gw[k] = gyx[k] = gyz[k] = gr[k] = 0.0f;
-        if (e < RN && i >= 0 && i < nx && j >= 0 && j < nz) {
-            f |= 1u;
-            const int c = i * nz + j;
-            gc[k] = c;
-            if (i >= 1 && i < nx - 1 && j >= 1 && j < nz - 1) f |= 2u;
-            const bool own = i >= i0 && i < i0 + TX && j >= j0 && j < j0 + TZ;
-            if (own) f |= 4u;
-            if (r >= 1 && z >= 1) f |= 8u;
-            if (r <= RX - 2 && z <= RZ - 2) f |= 16u;
-            A[k] = sponge_at(i, j, nx, nz, a.fw, a.tab);
-            if (PSV) {
-                c0[k] = G(a.iru)[c];
-                c1[k] = G(a.irw)[c];
-                c2[k] = G(a.lam)[c];
-                c3[k] = G(a.mu)[c];
-                c4[k] = G(a.mxz)[c];
-                pu[k] = G(in[0])[c];
-                pw[k] = G(in[2])[c];
-                pxx[k] = G(in[3])[c];
-                pzz[k] = G(in[4])[c];
-                pxz[k] = G(in[5])[c];
-                if (own) {
-                    gl[k] = G(a.g_lam)[c];
-                    gm[k] = G(a.g_mu)[c];
-                    gx[k] = G(a.g_mxz)[c];
-                    gu[k] = G(a.g_iru)[c];
-                    gw[k] = G(a.g_irw)[c];
-                }
-            }
-            if (SH) {
-                d0[k] = G(a.ir)[c];
-                d1[k] = G(a.myx)[c];
-                d2[k] = G(a.myz)[c];
-                pv[k] = G(in[1])[c];
-                pyx[k] = G(in[6])[c];
-                pyz[k] = G(in[7])[c];
-                if (own) {
-                    gyx[k] = G(a.g_myx)[c];
-                    gyz[k] = G(a.g_myz)[c];
-                    gr[k] = G(a.g_ir)[c];
-                }
-            }
-            if (has_src && own) sg[k] = G(a.src_map)[c];
-            if (has_rec) rg[k] = G(a.rec_map)[c];
-        } else if (e < RN) {
+        const RegionCell rc = region_cell<TX, TZ, K>(e, i0, j0, nx, nz, a.fw);
+        gc[k] = rc.c;
+        fl[k] = rc.f;
+        slot[k] = rc.slot;
+        if (!(rc.f & 1u) && e < RN) {
             // outside the grid: the transposes read zero here (not interior)
             if (PSV) qX[e] = qZ[e] = qS[e] = bU[e] = bW[e] = 0.0f;
             if (SH) qY[e] = qV[e] = bV[e] = 0.0f;
         }
-        fl[k] = f;
+    }
+    {
+        const gptr<const float> tab = G(a.tab);
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            const int c = gc[k], cg = tile_cell(t + NT * k, false);
+            const float sa = tab[slot[k] > 0 ? slot[k] : 0];
+            A[k] = slot[k] >= 0 ? sa : 1.0f;
+            sg[k] = has_src ? ld(a.src_map, cg) : -1;
+            rg[k] = has_rec ? ld(a.rec_map, c) : -1;
+            if ((fl[k] & 5u) != 5u) sg[k] = -1;
+            if (!(fl[k] & 1u)) rg[k] = -1;
+            if (PSV) {
+                c0[k] = ld(a.iru, c);
+                c1[k] = ld(a.irw, c);
+                c2[k] = ld(a.lam, c);
+                c3[k] = ld(a.mu, c);
+                c4[k] = ld(a.mxz, c);
+                pu[k] = ld(in[0], c);
+                pw[k] = ld(in[2], c);
+                pxx[k] = ld(in[3], c);
+                pzz[k] = ld(in[4], c);
+                pxz[k] = ld(in[5], c);
+                gl[k] = ld(a.g_lam, cg);
+                gm[k] = ld(a.g_mu, cg);
+                gx[k] = ld(a.g_mxz, cg);
+                gu[k] = ld(a.g_iru, cg);
+                gw[k] = ld(a.g_irw, cg);
+            }
+            if (SH) {
+                d0[k] = ld(a.ir, c);
+                d1[k] = ld(a.myx, c);
+                d2[k] = ld(a.myz, c);
+                pv[k] = ld(in[1], c);
+                pyx[k] = ld(in[6], c);
+                pyz[k] = ld(in[7], c);
+                gyx[k] = ld(a.g_myx, cg);
+                gyz[k] = ld(a.g_myz, cg);
+                gr[k] = ld(a.g_ir, cg);
+            }
+        }
     }
 
     const size_t n = (size_t)nx * nz;
@@ -413,12 +489,46 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
     const gptr<const float> wu = G(a.wav_u), wv = G(a.wav_v), ww = G(a.wav_w);
     const gptr<const float> res_u = G(a.res_u), res_v = G(a.res_v), res_w = G(a.res_w);
 
+    // The history window of step it goes to buffer b asynchronously (global
+    // to LDS): issued one step ahead, waited for before the step's phase 1b.
+    const int lane = t & 63, wave = t >> 6;
+    auto prefetch = [&](int it, float* dst) {
+        const float* base = a.hist + (size_t)(it - a.hist_base) * (H::n * n);
+        for (int q = wave; q < W::fields * W::chunks; q += NT / 64) {
+            const int f = q / W::chunks, ch = q - f * W::chunks;
+            const int w = ch * 64 + lane;
+            const int r = w / W::HZ, z = w - r * W::HZ;
+            const int i = clampi(i0 - 1 + r, 0, nx - 1), j = clampi(j0 - 1 + z, 0, nz - 1);
+            const unsigned off = (unsigned)(f * (int)n + i * nz + j) * 4u;
+            __builtin_amdgcn_global_load_lds(
+                (const void*)((const char*)base + off),
+                (__attribute__((address_space(3))) void*)&dst[f * W::chunks * 64 + ch * 64],
+                4, 0, 0);
+        }
+    };
+    // window offset of each region cell (own interior cells only use it)
+    int wo[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const int e = t + NT * k;
+        const int r = e / RZ, z = e - r * RZ;
+        wo[k] = (r - K + 1) * W::HZ + (z - K + 1);
+    }
+    auto hv = [&](const float* hb, int f, int w) { return hb[f * W::chunks * 64 + w]; };
+
+    // step t_b-1-s uses window s & 1; the first step taken is below nt
+    const int first = (t_b < nt ? t_b : nt) - 1;
+    if ((t_b - 1 - first) & 1)
+        prefetch(first, hwB);
+    else
+        prefetch(first, hwA);
+#pragma unroll
     for (int s = 0; s < K; s++) {
         const int it = t_b - 1 - s;
         if (it < t_a) break;
         if (it >= nt) continue;
-        const gptr<const float> h =
-            G((const float*)a.hist) + (size_t)(it - a.hist_base) * (H::n * n);
+        float* const buf = (s & 1) ? hwB : hwA;
+        float* const nxt = (s & 1) ? hwA : hwB;
 
         // ---- phase 1a: data adjoint at the receivers; stress transposes q
         // of every region cell from its own sponged stress adjoint
@@ -449,7 +559,9 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                 qV[e] = in_ ? (d2[k] * dt) * (A[k] * pyz[k]) : 0.0f;
             }
         }
-        __syncthreads();
+        vm_wait();  // this step's history window (and the residuals) arrived
+        lds_barrier();
+        if (it - 1 >= t_a) prefetch(it - 1, nxt);  // window last read a step ago
 
         // ---- phase 1b: new velocity adjoint; lam / mu / mxz (myx / myz) and
         // source-term gradients of the tile; b = dt * inv_rho * p for phase 2
@@ -469,9 +581,12 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                     if (own_in) {
                         const float axx = dt * A[k] * pxx[k], azz = dt * A[k] * pzz[k];
                         const float axz = dt * A[k] * pxz[k];
-                        const gptr<const float> hu = h + H::u * n, hw = h + H::w * n;
-                        const float ux = (hu[c + nz] - hu[c]) * idx, uz = (hu[c + 1] - hu[c]) * idz;
-                        const float wx = (hw[c + nz] - hw[c]) * idx, wz = (hw[c + 1] - hw[c]) * idz;
+                        const int w = wo[k];
+                        const float u0 = hv(buf, H::u, w), w0 = hv(buf, H::w, w);
+                        const float ux = (hv(buf, H::u, w + W::HZ) - u0) * idx;
+                        const float uz = (hv(buf, H::u, w + 1) - u0) * idz;
+                        const float wx = (hv(buf, H::w, w + W::HZ) - w0) * idx;
+                        const float wz = (hv(buf, H::w, w + 1) - w0) * idz;
                         gl[k] += (axx + azz) * (ux + wz);
                         gm[k] += 2.0f * (axx * ux + azz * wz);
                         gx[k] += axz * (uz + wx);
@@ -483,8 +598,10 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                     const float nv =
                         A[k] * pv[k] + idx * (qY[e - RZ] - qY[e]) + idz * (qV[e - 1] - qV[e]);
                     if (own_in) {
-                        const gptr<const float> hv = h + H::v * n;
-                        const float vx = (hv[c + nz] - hv[c]) * idx, vz = (hv[c + 1] - hv[c]) * idz;
+                        const int w = wo[k];
+                        const float v0 = hv(buf, H::v, w);
+                        const float vx = (hv(buf, H::v, w + W::HZ) - v0) * idx;
+                        const float vz = (hv(buf, H::v, w + 1) - v0) * idz;
                         gyx[k] += dt * A[k] * pyx[k] * vx;
                         gyz[k] += dt * A[k] * pyz[k] * vz;
                     }
@@ -506,7 +623,7 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
             }
             if (SH) bV[e] = in_ ? dt * d0[k] * pv[k] : 0.0f;
         }
-        __syncthreads();
+        lds_barrier();
 
         // ---- phase 2: new stress adjoint; inv_rho_u / inv_rho_w (inv_rho)
         // gradients of the tile
@@ -522,12 +639,13 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                 pxz[k] = A[k] * pxz[k] + idz * (bu0 - bU[e + 1]) + idx * (bw0 - bW[e + RZ]);
                 pzz[k] = A[k] * pzz[k] + idz * (bw0 - bW[e + 1]);
                 if (own_in) {
-                    const gptr<const float> hxx = h + H::sxx * n, hzz = h + H::szz * n;
-                    const gptr<const float> hxz = h + H::sxz * n;
-                    const float sxx_x = (hxx[c] - hxx[c - nz]) * idx;
-                    const float sxz_z = (hxz[c] - hxz[c - 1]) * idz;
-                    const float sxz_x = (hxz[c] - hxz[c - nz]) * idx;
-                    const float szz_z = (hzz[c] - hzz[c - 1]) * idz;
+                    const int w = wo[k];
+                    const float xx = hv(buf, H::sxx, w), xz = hv(buf, H::sxz, w);
+                    const float zz = hv(buf, H::szz, w);
+                    const float sxx_x = (xx - hv(buf, H::sxx, w - W::HZ)) * idx;
+                    const float sxz_z = (xz - hv(buf, H::sxz, w - 1)) * idz;
+                    const float sxz_x = (xz - hv(buf, H::sxz, w - W::HZ)) * idx;
+                    const float szz_z = (zz - hv(buf, H::szz, w - 1)) * idz;
                     gu[k] += pu[k] * dt * (sxx_x + sxz_z);
                     gw[k] += pw[k] * dt * (sxz_x + szz_z);
                 }
@@ -537,9 +655,10 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                 pyx[k] = A[k] * pyx[k] + idx * (bv0 - bV[e + RZ]);
                 pyz[k] = A[k] * pyz[k] + idz * (bv0 - bV[e + 1]);
                 if (own_in) {
-                    const gptr<const float> hyx = h + H::syx * n, hyz = h + H::syz * n;
-                    const float syx_x = (hyx[c] - hyx[c - nz]) * idx;
-                    const float syz_z = (hyz[c] - hyz[c - 1]) * idz;
+                    const int w = wo[k];
+                    const float yx = hv(buf, H::syx, w), yz = hv(buf, H::syz, w);
+                    const float syx_x = (yx - hv(buf, H::syx, w - W::HZ)) * idx;
+                    const float syz_z = (yz - hv(buf, H::syz, w - 1)) * idz;
                     gr[k] += pv[k] * dt * (syx_x + syz_z);
                 }
             }
@@ -553,24 +672,24 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
         if ((fl[k] & 5u) != 5u) continue;
         const int c = gc[k];
         if (PSV) {
-            G(out[0])[c] = pu[k];
-            G(out[2])[c] = pw[k];
-            G(out[3])[c] = pxx[k];
-            G(out[4])[c] = pzz[k];
-            G(out[5])[c] = pxz[k];
-            G(a.g_lam)[c] = gl[k];
-            G(a.g_mu)[c] = gm[k];
-            G(a.g_mxz)[c] = gx[k];
-            G(a.g_iru)[c] = gu[k];
-            G(a.g_irw)[c] = gw[k];
+            st(out[0], c, pu[k]);
+            st(out[2], c, pw[k]);
+            st(out[3], c, pxx[k]);
+            st(out[4], c, pzz[k]);
+            st(out[5], c, pxz[k]);
+            st(a.g_lam, c, gl[k]);
+            st(a.g_mu, c, gm[k]);
+            st(a.g_mxz, c, gx[k]);
+            st(a.g_iru, c, gu[k]);
+            st(a.g_irw, c, gw[k]);
         }
         if (SH) {
-            G(out[1])[c] = pv[k];
-            G(out[6])[c] = pyx[k];
-            G(out[7])[c] = pyz[k];
-            G(a.g_myx)[c] = gyx[k];
-            G(a.g_myz)[c] = gyz[k];
-            G(a.g_ir)[c] = gr[k];
+            st(out[1], c, pv[k]);
+            st(out[6], c, pyx[k]);
+            st(out[7], c, pyz[k]);
+            st(a.g_myx, c, gyx[k]);
+            st(a.g_myz, c, gyz[k]);
+            st(a.g_ir, c, gr[k]);
         }
     }
 }
@@ -578,30 +697,54 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
 // ---------------------------------------------------------------------------
 // host side
 
-// compiled shapes (tile rows, tile columns, threads, steps per launch)
-static const BlockCfg kFwiCfg[] = {
-    {16, 64, 512, 4}, {16, 64, 256, 2}, {32, 64, 1024, 4}, {16, 64, 512, 8}};
-static const BlockCfg kFwiDefault = {16, 64, 512, 4};
+template <bool PSV, bool SH, int TX, int TZ, int K>
+constexpr bool fwi_fits() {
+    constexpr size_t fwd = 4 * (size_t)(TX + 2 * K) * (TZ + 2 * K) * ((PSV ? 5 : 0) + (SH ? 3 : 0));
+    return fwd <= kLdsLimit && adj_lds_bytes<PSV, SH, TX, TZ, K>() <= kLdsLimit;
+}
 
+static bool fwi_cfg_fits(int systems, const BlockCfg& c) {
+    const int nf = ((systems & TRTM_PSV) ? 5 : 0) + ((systems & TRTM_SH) ? 3 : 0);
+    const size_t rn = (size_t)(c.tx + 2 * c.k) * (c.tz + 2 * c.k);
+    const size_t hist = 2 * (size_t)nf * (((c.tx + 2) * (c.tz + 2) + 63) / 64) * 64;
+    return 4 * rn * nf <= kLdsLimit && 4 * (hist + rn * nf) <= kLdsLimit;
+}
+
+// compiled shapes (tile rows, tile columns, threads, steps per launch)
+static const BlockCfg kFwiCfg[] = {{24, 64, 768, 4},  {32, 64, 1024, 4}, {16, 64, 512, 4},
+                                   {16, 64, 512, 8},  {32, 64, 1024, 2}, {16, 64, 256, 2}};
+// defaults, the fastest measured on MI355X for 8 P-SV shots of 600 x 300: the
+// forward passes {24, 64, 768, 4}, the adjoint {32, 64, 1024, 4}; the first
+// listed shape that fits otherwise
+static const BlockCfg kFwiDefaultFwd = {24, 64, 768, 4}, kFwiDefaultAdj = {32, 64, 1024, 4};
+
+// shapes whose forward and adjoint LDS fit the workgroup limit for these systems
 int fwi_block_configs(int systems, int32_t* out, int max_n) {
-    (void)systems;
-    const int n = (int)(sizeof(kFwiCfg) / sizeof(BlockCfg));
-    for (int i = 0; i < n && i < max_n; i++) {
-        out[4 * i] = kFwiCfg[i].tx;
-        out[4 * i + 1] = kFwiCfg[i].tz;
-        out[4 * i + 2] = kFwiCfg[i].nt;
-        out[4 * i + 3] = kFwiCfg[i].k;
+    int n = 0;
+    for (const BlockCfg& c : kFwiCfg) {
+        if (!fwi_cfg_fits(systems, c)) continue;
+        if (n < max_n) {
+            out[4 * n] = c.tx;
+            out[4 * n + 1] = c.tz;
+            out[4 * n + 2] = c.nt;
+            out[4 * n + 3] = c.k;
+        }
+        n++;
     }
     return n;
 }
 
-static int fwi_choose(const trtm_fwi_desc* d, BlockCfg* out) {
-    if (d->tile_x <= 0 && d->tile_z <= 0 && d->block_steps <= 0) {
-        *out = kFwiDefault;
+// the pass's default, or the first listed shape that fits the systems and
+// matches every nonzero request
+static int fwi_choose(const trtm_fwi_desc* d, bool adjoint, BlockCfg* out) {
+    const BlockCfg& def = adjoint ? kFwiDefaultAdj : kFwiDefaultFwd;
+    if (d->tile_x <= 0 && d->tile_z <= 0 && d->block_steps <= 0 && fwi_cfg_fits(d->systems, def)) {
+        *out = def;
         return TRTM_OK;
     }
     for (const BlockCfg& c : kFwiCfg)
-        if ((d->tile_x <= 0 || c.tx == d->tile_x) && (d->tile_z <= 0 || c.tz == d->tile_z) &&
+        if (fwi_cfg_fits(d->systems, c) && (d->tile_x <= 0 || c.tx == d->tile_x) &&
+            (d->tile_z <= 0 || c.tz == d->tile_z) &&
             (d->block_steps <= 0 || c.k == d->block_steps)) {
             *out = c;
             return TRTM_OK;
@@ -665,14 +808,19 @@ static int launch_cfg(const BlockCfg& c, bool adjoint, bool hist, const S& src, 
                       int t_a, int t_b, int nx, int nz, hipStream_t s) {
     const int tx_ = (nx + c.tx - 1) / c.tx, tz_ = (nz + c.tz - 1) / c.tz;
 #define TRTM_FB(TX, TZ, NT, K)                                                                  \
-    if (c.tx == TX && c.tz == TZ && c.nt == NT && c.k == K) {                                   \
-        launch_fb<PSV, SH, TX, TZ, NT, K, S>(adjoint, hist, src, shots, t_a, t_b, tx_, tz_, s); \
-        return TRTM_OK;                                                                         \
+    if constexpr (fwi_fits<PSV, SH, TX, TZ, K>()) {                                             \
+        if (c.tx == TX && c.tz == TZ && c.nt == NT && c.k == K) {                               \
+            launch_fb<PSV, SH, TX, TZ, NT, K, S>(adjoint, hist, src, shots, t_a, t_b, tx_, tz_, \
+                                                 s);                                            \
+            return TRTM_OK;                                                                     \
+        }                                                                                       \
     }
     TRTM_FB(16, 64, 512, 4)
     TRTM_FB(16, 64, 256, 2)
     TRTM_FB(32, 64, 1024, 4)
     TRTM_FB(16, 64, 512, 8)
+    TRTM_FB(32, 64, 1024, 2)
+    TRTM_FB(24, 64, 768, 4)
 #undef TRTM_FB
     return fail(TRTM_ERR_UNSUPPORTED, "blocked FWI shape %dx%d/%d/%d not compiled", c.tx, c.tz,
                 c.nt, c.k);
@@ -696,7 +844,7 @@ static int fwi_block_run(const trtm_fwi_desc* d, int n, int it_begin, int it_end
                          hipStream_t s) {
     if (it_begin >= it_end) return TRTM_OK;
     BlockCfg c;
-    int rc = fwi_choose(d, &c);
+    int rc = fwi_choose(d, adjoint, &c);
     if (rc) return rc;
     for (int q = 0; q < n; q++)
         if (!d[q].workspace)
@@ -705,10 +853,24 @@ static int fwi_block_run(const trtm_fwi_desc* d, int n, int it_begin, int it_end
     std::vector<FB> fb(n);
     for (int q = 0; q < n; q++) fb[q] = make_fb(&d[q], adjoint);
     const int K = c.k, groups = (n + kShotsPerLaunch - 1) / kShotsPerLaunch;
-    const int steps = it_end - it_begin, launches = (steps + K - 1) / K;
-    for (int l = 0; l < launches; l++) {
+    // launches of up to K steps in stepping order; an even number of them (the
+    // first one split in two when needed) leaves the state in the caller's
+    // buffers for every shot that runs to the end of the range
+    std::vector<std::pair<int, int>> chunks;
+    for (int l = 0; l * K < it_end - it_begin; l++) {
         const int t_a = adjoint ? std::max(it_end - (l + 1) * K, it_begin) : it_begin + l * K;
         const int t_b = adjoint ? it_end - l * K : std::min(t_a + K, it_end);
+        chunks.push_back({t_a, t_b});
+    }
+    if (chunks.size() % 2 == 1 && chunks[0].second - chunks[0].first > 1) {
+        const auto first = chunks[0];
+        const int cut = adjoint ? first.second - 1 : first.first + 1;
+        chunks[0] = adjoint ? std::make_pair(cut, first.second) : std::make_pair(first.first, cut);
+        chunks.insert(chunks.begin() + 1, adjoint ? std::make_pair(first.first, cut)
+                                                  : std::make_pair(cut, first.second));
+    }
+    for (const auto& ch : chunks) {
+        const int t_a = ch.first, t_b = ch.second;
         for (int g = 0; g < groups; g++) {
             const int q0 = g * kShotsPerLaunch, m = std::min(kShotsPerLaunch, n - q0);
             if (n == 1) {

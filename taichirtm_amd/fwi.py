@@ -25,7 +25,10 @@ Step kernels (``kernel``): ``"blocked"`` (float32) advances K steps per launch
 on LDS tiles with a halo, forward and adjoint, the gradient accumulated in
 registers; ``"step"`` runs one step per launch (float32 or float64).  Both
 give the same records and gradients bit for bit; ``"auto"`` is ``"blocked"``
-in float32 and ``"step"`` in float64.
+with the library's shape per pass in float32 and ``"step"`` in float64;
+``"tuned"`` times every compiled blocked shape on a window of this problem's
+own steps, per pass (checkpointing forward, history forward, adjoint), and
+keeps the fastest.
 """
 from __future__ import annotations
 
@@ -42,7 +45,9 @@ from .batch import ShotSpec, checkpoint_interval
 from .field import as_tensor, device, stream_handle
 
 _REAL = {"float32": (torch.float32, 4), "float64": (torch.float64, 8)}
-_KERNELS = {"auto": _lib.KERNEL_AUTO, "step": _lib.KERNEL_SIMPLE, "blocked": _lib.KERNEL_BLOCKED}
+_KERNELS = {"auto": _lib.KERNEL_AUTO, "step": _lib.KERNEL_SIMPLE, "blocked": _lib.KERNEL_BLOCKED,
+            "tuned": _lib.KERNEL_BLOCKED}
+_WINDOW = 64  # steps per autotuning trial
 
 
 def _plan(nx, nz, loc):
@@ -77,10 +82,14 @@ class ElasticFWI:
             raise ValueError("precision must be float32 or float64")
         if kernel not in _KERNELS:
             raise ValueError("kernel must be one of %s" % sorted(_KERNELS))
-        if kernel == "blocked" and precision != "float32":
+        if kernel in ("blocked", "tuned") and precision != "float32":
             raise ValueError("the blocked kernels are float32 only")
         self.kernel = kernel
-        self.shape = (int(tile_x), int(tile_z), int(block_steps))
+        # blocked kernel shape (tile_x, tile_z, block_steps) per pass: the
+        # checkpointing forward, the forward that records the history, the
+        # adjoint (0 = library default)
+        shape = (int(tile_x), int(tile_z), int(block_steps))
+        self.shapes = {"forward": shape, "history": shape, "adjoint": shape}
         self.nx, self.nz = int(nx), int(nz)
         self.dx, self.dz, self.fs = float(dx), float(dz), float(fs)
         self.fw = int(absorbing_frame)
@@ -110,7 +119,9 @@ class ElasticFWI:
             self.geo.append({"src": src, "rec": rec, "wav": wav, "nt": s.nt,
                              "nr": len(rec[0]), "ns": len(src[0]),
                              "map": torch.empty(2 * nx * nz, dtype=torch.int32, device=dev)})
-        self.blocked = kernel == "blocked" or (kernel == "auto" and precision == "float32")
+        self.blocked = kernel in ("blocked", "tuned") or (kernel == "auto" and
+                                                          precision == "float32")
+        self.tuned = {}  # pass -> (tile_x, tile_z, block_steps) chosen by kernel="tuned"
         for g in self.geo:  # second state buffer of the blocked kernels
             g["ws"] = (torch.empty(8 * nx * nz, dtype=self.dtype, device=dev) if self.blocked
                        else None)
@@ -168,7 +179,6 @@ class ElasticFWI:
         d.rec_loc, d.rec_order, d.rec_group_start = _ptr(loc), _ptr(order), _ptr(start)
         d.cell_map = _ptr(g["map"])
         d.kernel = _KERNELS[self.kernel]
-        d.tile_x, d.tile_z, d.block_steps = self.shape
         d.workspace = _ptr(g["ws"]) if g["ws"] is not None else None
         return d
 
@@ -188,7 +198,32 @@ class ElasticFWI:
 
     # -- forward --------------------------------------------------------------
 
-    def _call_batch(self, fn, descs, a, b):
+    def _tune(self, name, fn, descs, a, b, reset):
+        """kernel="tuned": time every compiled shape on steps [a, b) of this
+        problem (at most a 64-step window), keep the fastest for pass
+        ``name``, then undo the trial steps with reset()."""
+        if self.kernel != "tuned" or name in self.tuned:
+            return
+        from . import autotune
+        if name == "adjoint":
+            a = max(a, b - _WINDOW)
+        else:
+            b = min(b, a + _WINDOW)
+        key = ("fwi", name, self.nx, self.nz, self.fw, self.systems, len(descs))
+
+        def trial(shape):
+            self._call_batch(fn, descs, a, b, shape[1:])
+            reset()
+
+        shape = autotune.tune(key, self.systems, trial,
+                              shapes=autotune.fwi_candidates(self.systems))
+        self.tuned[name] = shape[1:]
+        self.shapes[name] = shape[1:]
+        reset()
+
+    def _call_batch(self, fn, descs, a, b, shape=(0, 0, 0)):
+        for d in descs:
+            d.tile_x, d.tile_z, d.block_steps = shape
         arr = (_lib.FwiDesc * len(descs))(*descs)
         _lib.call(fn, arr, len(descs), a, b, stream_handle())
 
@@ -206,10 +241,13 @@ class ElasticFWI:
         shape = (self.nx, self.nz)
         self.edges = list(range(0, nt_max, self.L)) + [nt_max]
         nseg = len(self.edges) - 1
-        self.state = [{k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in names}
-                      for _ in self.geo]
-        self._ck = [torch.empty((nseg, len(names)) + shape, dtype=self.dtype, device=dev)
-                    for _ in self.geo]
+        # every shot's state in one tensor, so a checkpoint is one copy
+        self.state_all = torch.zeros((len(self.geo), len(names)) + shape, dtype=self.dtype,
+                                     device=dev)
+        self.state = [{k: self.state_all[q, m] for m, k in enumerate(names)}
+                      for q in range(len(self.geo))]
+        self._ck = torch.empty((nseg,) + tuple(self.state_all.shape), dtype=self.dtype,
+                               device=dev)
         recs, descs = [], []
         for g, st in zip(self.geo, self.state):
             d = self._desc(self.med, g)
@@ -220,11 +258,12 @@ class ElasticFWI:
             d.rec_u, d.rec_v, d.rec_w = (_ptr(rec[c]) for c in ("u", "v", "w"))
             recs.append(rec)
             descs.append(d)
+        self._tune("forward", "trtm_fwi_forward_batch", descs, self.edges[0], self.edges[1],
+                   self.state_all.zero_)
         for k in range(nseg):
-            for st, ck in zip(self.state, self._ck):
-                for m, name in enumerate(names):
-                    ck[k, m].copy_(st[name])
-            self._call_batch("trtm_fwi_forward_batch", descs, self.edges[k], self.edges[k + 1])
+            self._ck[k].copy_(self.state_all)
+            self._call_batch("trtm_fwi_forward_batch", descs, self.edges[k], self.edges[k + 1],
+                             self.shapes["forward"])
         return [torch.stack([rec[c].t() for c in ("u", "v", "w")]).contiguous() for rec in recs]
 
     # -- gradient ---------------------------------------------------------------
@@ -244,9 +283,12 @@ class ElasticFWI:
         gnames = ("g_lam", "g_mu", "g_mxz", "g_myx", "g_myz", "g_ir", "g_iru", "g_irw")
         n_hist = (5 if self.systems & _lib.PSV else 0) + (3 if self.systems & _lib.SH else 0)
         keep, descs = [], []
-        for g, st, da in zip(self.geo, self.state, data_adjoint):
-            adj = {"p" + k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in names}
-            grads = {k: torch.zeros(shape, dtype=self.dtype, device=dev) for k in gnames}
+        S = len(self.geo)
+        adj_all = torch.zeros((S, len(names)) + shape, dtype=self.dtype, device=dev)
+        grad_all = torch.zeros((S, len(gnames)) + shape, dtype=self.dtype, device=dev)
+        for q, (g, st, da) in enumerate(zip(self.geo, self.state, data_adjoint)):
+            adj = {"p" + k: adj_all[q, m] for m, k in enumerate(names)}
+            grads = {k: grad_all[q, m] for m, k in enumerate(gnames)}
             hist = torch.empty((self.L, n_hist) + shape, dtype=self.dtype, device=dev)
             da = torch.as_tensor(da, device=dev).to(self.dtype)
             res = [da[c].t().contiguous() for c in range(3)]  # (nt, nr) each
@@ -262,14 +304,21 @@ class ElasticFWI:
             descs.append(d)
         for k in range(len(self.edges) - 2, -1, -1):
             a, b = self.edges[k], self.edges[k + 1]
-            for g, st, ck in zip(self.geo, self.state, self._ck):
-                if a < g["nt"]:
-                    for m, name in enumerate(names):
-                        st[name].copy_(ck[k, m])
+            # a shot whose record ends before a skips these steps; its state is unused
+            self.state_all.copy_(self._ck[k])
             for d in descs:
                 d.hist_base, d.hist_len = a, b - a
-            self._call_batch("trtm_fwi_forward_batch", descs, a, b)
-            self._call_batch("trtm_fwi_adjoint_batch", descs, a, b)
+            ck = self._ck[k]
+            self._tune("history", "trtm_fwi_forward_batch", descs, a, b,
+                       lambda: self.state_all.copy_(ck))
+            self._call_batch("trtm_fwi_forward_batch", descs, a, b, self.shapes["history"])
+
+            def reset_adjoint():
+                adj_all.zero_()
+                grad_all.zero_()
+            if k == len(self.edges) - 2:
+                self._tune("adjoint", "trtm_fwi_adjoint_batch", descs, a, b, reset_adjoint)
+            self._call_batch("trtm_fwi_adjoint_batch", descs, a, b, self.shapes["adjoint"])
         grads = {k: keep[0][1][k].clone() for k in gnames}
         for _, gq, _, _ in keep[1:]:
             for k in gnames:
