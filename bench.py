@@ -20,11 +20,12 @@ cpu_baseline: the oracle (C/OpenMP restatement of the reference's unfused
 kernel:  "tuned" (default) lets the autotuner pick the blocked kernel family
          and shape per direction during the warmup step.
 
-Other workloads (--workload psv_batch | fwi, --shots N): the batched
-checkpointed P-SV RTM of the 2000 x 1000 faulted model and the P-SV FWI
-gradient of the 600 x 300 x 2500 problem; one step is the whole job, the
-roofline is job-level (useful cell-updates x algorithmic bytes / wall).
-Prints one JSON line.
+Other workloads (--workload psv_batch | fwi | salt, --shots N): the batched
+checkpointed P-SV RTM of the 2000 x 1000 faulted model, the P-SV FWI
+gradient of the 600 x 300 x 2500 problem, and the FWI gradient plus RTM image
+of the 4000 x 2000 x 8000 salt model under a device-memory budget
+(--budget-gb); one step is the whole job, the roofline is job-level (useful
+cell-updates x algorithmic bytes / wall).  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -72,7 +73,8 @@ def build_workload(name):
 # algorithmic bytes per useful cell-update of a whole job (forward and
 # backward/adjoint updates averaged): P-SV RTM with imaging every step
 # (68 + 84) / 2; P-SV FWI forward 60 and adjoint-with-gradient 120
-JOB_BYTES = {"psv_batch": (68 + 84) / 2.0, "fwi": (60 + 120) / 2.0}
+JOB_BYTES = {"psv_batch": (68 + 84) / 2.0, "fwi": (60 + 120) / 2.0,
+             "salt": (60 + 120 + 68 + 84) / 4.0}
 
 
 def _shot_specs(cfg, n):
@@ -87,8 +89,9 @@ def _shot_specs(cfg, n):
 
 
 def job_main(args):
-    """Batched checkpointed P-SV RTM (psv_batch) or the P-SV FWI gradient
-    (fwi): one step = the whole job over --shots shots."""
+    """Batched checkpointed P-SV RTM (psv_batch), the P-SV FWI gradient (fwi)
+    or the budgeted gradient + image of the salt model (salt): one step = the
+    whole job over --shots shots."""
     import torch
     from taichirtm_amd import autotune, configs
     if args.workload == "psv_batch":
@@ -107,7 +110,7 @@ def job_main(args):
             return b.tuned
         desc = ("batched checkpointed P-SV RTM, %d shots of %s (uneven record lengths and "
                 "receiver spreads), isnap=%d, stacked image" % (len(shots), cfg.name, args.isnap))
-    else:
+    elif args.workload == "fwi":
         from taichirtm_amd.fwi import ElasticFWI, l2_misfit_and_gradient
         cfg, start = configs.psv_fwi()
         shots = _shot_specs(cfg, args.shots)
@@ -123,6 +126,27 @@ def job_main(args):
             return dict(prob.shapes)
         desc = ("P-SV FWI gradient (forward + checkpointed exact adjoint), %d shots of %s"
                 % (len(shots), cfg.name))
+    elif args.workload == "salt":
+        from taichirtm_amd import budget
+        true, cfg, shot0 = budget.salt_survey(n_shots=32)
+        pick = np.linspace(0, len(shot0) - 1, args.shots).round().astype(int)
+        shots = [shot0[q] for q in pick]
+        obs = budget.observed_data(true, shots)  # input data, made before the timed region
+        with_obs = [budget.ShotSpec(s.src_loc, s.wavelets, s.rec_loc, tuple(o))
+                    for s, o in zip(shots, obs)]
+        del obs
+        job = budget.GradientAndImage(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, with_obs,
+                                      args.budget_gb * 1e9, kernel=args.kernel)
+
+        def step():
+            job.run(cfg.vp, cfg.vs, cfg.rho)
+            p = job.plan
+            return {"group": [p.group], "L_fwi": [p.L_fwi], "L_rtm": [p.L_rtm]}
+        desc = ("P-SV FWI gradient + RTM image together under a %.0f GB budget, %d of the 32 "
+                "shots of %s (receiver spreads 1000-3000), layered start model"
+                % (args.budget_gb, len(shots), true.name))
+    else:
+        raise SystemExit("unknown workload %r" % args.workload)
     for _ in range(args.warmup):
         tuned = step()
     torch.cuda.synchronize()
@@ -131,7 +155,8 @@ def job_main(args):
         tuned = step()
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
-    useful = 2.0 * cfg.nx * cfg.nz * sum(s.nt for s in shots)
+    passes = 4.0 if args.workload == "salt" else 2.0  # salt: FWI and RTM, forward + backward each
+    useful = passes * cfg.nx * cfg.nz * sum(s.nt for s in shots)
     g = useful / wall / 1e9
     achieved = useful * JOB_BYTES[args.workload] / wall / 1e9
     out = {
@@ -258,6 +283,7 @@ def main():
     ap.add_argument("--block-steps", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--isnap", type=int, default=1)
+    ap.add_argument("--budget-gb", type=float, default=128.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -1,0 +1,171 @@
+"""FWI gradient and RTM image of a survey together, under an HBM budget.
+
+A survey's gradient (exact adjoint, ``fwi.ElasticFWI``) and image (the
+reference's back-propagation of the observed data with zero-lag imaging,
+``batch.BatchRTM``, backward_modeling.py:539-576 and imaging.py:81-93) are
+computed shot group by shot group.  The group size and the checkpoint spacing
+of each pass come from the budget: per shot, the FWI pass holds its state,
+adjoint state, gradient accumulators, blocked-kernel workspace, checkpoints
+every L steps and an L-step history; the RTM pass its forward and backward
+propagators, checkpoints and an L-step snapshot buffer.  L minimises that
+per-shot memory (about sqrt(nt)); the group is the largest number of shots
+whose memory fits the budget next to the shared medium.
+
+The per-shot coefficient gradients are accumulated in shot order in float32
+and the per-shot images in float64, so the result does not depend on the
+grouping: any budget gives the bits of one group holding every shot.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List, Sequence
+
+import numpy as np
+import torch
+
+from . import configs
+from .batch import BatchRTM, ShotSpec, checkpoint_interval
+from .field import device
+from .fwi import ElasticFWI, accumulate_in_shot_order
+
+_F32 = 4
+
+
+@dataclass
+class Plan:
+    group: int        # shots per group
+    groups: int
+    L_fwi: int        # checkpoint spacing of the FWI pass
+    L_rtm: int        # of the RTM pass
+    shot_bytes_fwi: int
+    shot_bytes_rtm: int
+    shared_bytes: int
+
+
+def _fwi_shot_bytes(cells, nt, nr, L):
+    # state 5 + adjoint 5 + gradients 8 + workspace 8 + cell map 2 fields,
+    # checkpoints ceil(nt/L) x 5, history L x 5, records / residuals / data
+    fields = 28 + math.ceil(nt / L) * 5 + L * 5
+    return fields * cells * _F32 + 12 * nr * nt * _F32
+
+
+def _rtm_shot_bytes(cells, nt, nr, L, isnap):
+    # forward and backward: state 5 + workspace 8 each, image 3; checkpoints
+    # ceil(nt/L) x 5; snapshot buffer (L/isnap + 1) x 2; seismograms / data
+    fields = 29 + math.ceil(nt / L) * 5 + (L // isnap + 1) * 2
+    return fields * cells * _F32 + 12 * nr * nt * _F32
+
+
+def plan(nx, nz, shots: Sequence[ShotSpec], budget_bytes, isnap=1) -> Plan:
+    """Group size and checkpoint spacings that fit ``budget_bytes``."""
+    cells = nx * nz
+    nt = max(s.nt for s in shots)
+    nr = max(len(np.asarray(s.rec_loc).reshape(-1, 2)) for s in shots)
+    # both media (27 fields), image accumulators (6), gradient sum (8), the
+    # float64 chain rule (30), margin
+    shared = 80 * cells * _F32
+    L_fwi = checkpoint_interval(nt, 5, 5)
+    L_rtm = checkpoint_interval(nt, 5, 2, isnap, align=isnap)
+    per_fwi = _fwi_shot_bytes(cells, nt, nr, L_fwi)
+    per_rtm = _rtm_shot_bytes(cells, nt, nr, L_rtm, isnap)
+    free = budget_bytes - shared
+    group = min(len(shots), int(free // max(per_fwi, per_rtm))) if free > 0 else 0
+    if group < 1:
+        raise MemoryError("one shot needs %.2f GB (FWI) / %.2f GB (RTM) next to %.2f GB shared; "
+                          "budget %.2f GB" % (per_fwi / 1e9, per_rtm / 1e9, shared / 1e9,
+                                              budget_bytes / 1e9))
+    return Plan(group, math.ceil(len(shots) / group), L_fwi, L_rtm, per_fwi, per_rtm, shared)
+
+
+class GradientAndImage:
+    """FWI gradient of J = 1/2 sum |d(m) - d_obs|^2 and the stacked RTM image of
+    the observed data, for shots on one (nx, nz) grid, within budget_bytes of
+    device memory.  Every shot carries its observed data (u, v, w), each
+    (nr, nt)."""
+
+    def __init__(self, nx, nz, dx, dz, fs, absorbing_frame, shots: List[ShotSpec],
+                 budget_bytes, isnap=1, kernel="tuned"):
+        if any(s.observed is None for s in shots):
+            raise ValueError("every shot needs its observed data")
+        self.nx, self.nz, self.dx, self.dz, self.fs = int(nx), int(nz), dx, dz, fs
+        self.fw = int(absorbing_frame)
+        self.shots = list(shots)
+        self.isnap = int(isnap)
+        self.kernel = kernel
+        self.plan = plan(self.nx, self.nz, self.shots, budget_bytes, self.isnap)
+
+    def run(self, vp, vs, rho):
+        """(J, (g_vp, g_vs, g_rho) float64, image {"u", "v", "w"} float64)."""
+        dev = device()
+        p = self.plan
+        vp_n, vs_n, rho_n = (np.asarray(torch.as_tensor(x).cpu(), np.float32) for x in (vp, vs, rho))
+        mu, lam = configs.moduli(vp_n, vs_n, rho_n)
+        m = [torch.as_tensor(x, dtype=torch.float32, device=dev) for x in (vp_n, vs_n, rho_n)]
+        J = 0.0
+        gsum = None
+        image = {c: torch.zeros((self.nx, self.nz), dtype=torch.float64, device=dev)
+                 for c in ("u", "v", "w")}
+        last = None
+        for g0 in range(0, len(self.shots), p.group):
+            group = self.shots[g0:g0 + p.group]
+            fwi = ElasticFWI(self.nx, self.nz, self.dx, self.dz, self.fs, self.fw,
+                             [ShotSpec(s.src_loc, s.wavelets, s.rec_loc) for s in group],
+                             precision="float32", checkpoint_every=p.L_fwi,
+                             kernel=self.kernel if self.kernel in ("tuned", "blocked") else "auto")
+            recs = fwi.forward(*m)
+            res = []
+            for r, s in zip(recs, group):
+                obs = torch.stack([torch.as_tensor(np.asarray(o, np.float32), device=dev)
+                                   for o in s.observed])
+                d = r - obs
+                J += 0.5 * float((d.double() ** 2).sum())
+                res.append(d)
+            del recs
+            fwi.gradient(res)
+            gsum = accumulate_in_shot_order(gsum, fwi.shot_coefficient_gradients)
+            fwi.release()
+            last = fwi
+            rtm = BatchRTM(self.nx, self.nz, self.dx, self.dz, self.fs, mu, lam, rho_n, self.fw,
+                           group, isnap=self.isnap, imaging="checkpoint",
+                           checkpoint_every=p.L_rtm,
+                           kernel=self.kernel if self.kernel in ("tuned", "blocked", "wave")
+                           else "blocked")
+            codes = rtm.run()
+            if any(codes):
+                raise RuntimeError("non-finite wavefield, codes %s" % codes)
+            for imgs in rtm.image_tensors():
+                for c in ("u", "v", "w"):
+                    if imgs.get(c) is not None:
+                        image[c] += imgs[c].to(torch.float64)
+            del rtm, res, fwi
+        self.J = J
+        self.coefficient_gradient = gsum
+        gradient = last.chain(gsum)
+        return J, gradient, image
+
+
+def salt_survey(n_shots=32, nt=8000, nx=4000, nz=2000, seed=0):
+    """The large budgeted workload: the salt model's observed data are made
+    by the caller from ``true``; ``start`` is the layered background the
+    gradient and image are computed in.  Returns (true, start, shots without
+    observed data)."""
+    true, start = configs.psv_salt(nx=nx, nz=nz, nt=nt, n_shots=n_shots, seed=seed)
+    shots = []
+    for s in true.shots:
+        shots.append(ShotSpec(np.array([s.src], np.int32), true.wavelets(s),
+                              np.asarray(s.rec, np.int32)))
+    return true, start, shots
+
+
+def observed_data(cfg, shots: List[ShotSpec], group=4):
+    """Records of ``cfg``'s model for every shot (forward only, one
+    checkpoint), as numpy (3, nr, nt) per shot."""
+    out = []
+    m = [torch.as_tensor(np.asarray(x, np.float32), device=device()) for x in (cfg.vp, cfg.vs, cfg.rho)]
+    for g0 in range(0, len(shots), group):
+        sub = shots[g0:g0 + group]
+        p = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, sub, precision="float32",
+                       checkpoint_every=max(s.nt for s in sub))
+        out += [r.cpu().numpy() for r in p.forward(*m)]
+    return out

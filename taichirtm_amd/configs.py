@@ -204,7 +204,9 @@ def psv_fwi(nx=600, nz=300, nt=2500, dx=10.0, dt=8e-4, n_shots=16, fw=30, f0=15.
 def psv_salt(nx=4000, nz=2000, nt=8000, n_shots=32, nr_range=(1000, 3000), dx=5.0, dt=4e-4,
              fw=40, f0=15.0, seed=0, n_layers=12):
     """Large budgeted adjoint: layered background 1500-4000 m/s with a salt
-    body (Vp 4500, Vs 2600, rho 2.15) from thresholded smoothed noise."""
+    body (Vp 4500, Vs 2600, rho 2.15) from thresholded smoothed noise; shots
+    with uneven receiver spreads centred on their sources.  Returns (true,
+    start) configs: with the salt, and the layered background alone."""
     rng = np.random.default_rng(seed)
     tops = np.sort(rng.choice(np.arange(20, nz - 20), size=n_layers - 1, replace=False))
     vp_layer = np.linspace(1500.0, 4000.0, n_layers)
@@ -217,6 +219,7 @@ def psv_salt(nx=4000, nz=2000, nt=8000, n_shots=32, nr_range=(1000, 3000), dx=5.
     ii, jj = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
     window = np.exp(-(((ii - nx / 2) / (0.25 * nx)) ** 2 + ((jj - 0.45 * nz) / (0.18 * nz)) ** 2))
     salt = noise * window > 0.6
+    background = (vp.copy(), vs.copy(), rho.copy())
     vp[salt], vs[salt], rho[salt] = 4500.0, 2600.0, 2.15
     xs = np.linspace(fw + 10, nx - fw - 10, n_shots).astype(int)
     shots = []
@@ -225,5 +228,6 @@ def psv_salt(nx=4000, nz=2000, nt=8000, n_shots=32, nr_range=(1000, 3000), dx=5.
         lo = max(0, x - nr // 2)
         hi = min(nx, lo + nr)
         shots.append(Shot(src=(int(x), 1), rec=_surface_receivers(lo, hi), nt=nt, component="w"))
-    return Config("psv_salt_%dx%dx%d" % (nx, nz, nt), nx, nz, dx, dx, dt, nt, fw, f0,
-                  vp, vs, rho, shots)
+    name = "psv_salt_%dx%dx%d" % (nx, nz, nt)
+    return (Config(name, nx, nz, dx, dx, dt, nt, fw, f0, vp, vs, rho, shots),
+            Config(name, nx, nz, dx, dx, dt, nt, fw, f0, *background, list(shots)))

@@ -48,6 +48,20 @@ _REAL = {"float32": (torch.float32, 4), "float64": (torch.float64, 8)}
 _KERNELS = {"auto": _lib.KERNEL_AUTO, "step": _lib.KERNEL_SIMPLE, "blocked": _lib.KERNEL_BLOCKED,
             "tuned": _lib.KERNEL_BLOCKED}
 _WINDOW = 64  # steps per autotuning trial
+GRAD_NAMES = ("g_lam", "g_mu", "g_mxz", "g_myx", "g_myz", "g_ir", "g_iru", "g_irw")
+
+
+def accumulate_in_shot_order(total, per_shot):
+    """total + per_shot[0] + per_shot[1] + ... per coefficient, in that order
+    (total None: start from the first shot), so shots split over several calls
+    sum to the same bits as one call."""
+    for g in per_shot:
+        if total is None:
+            total = {k: g[k].clone() for k in GRAD_NAMES}
+        else:
+            for k in GRAD_NAMES:
+                total[k] += g[k]
+    return total
 
 
 def _plan(nx, nz, loc):
@@ -280,7 +294,7 @@ class ElasticFWI:
         dev = device()
         names = self._fields()
         shape = (self.nx, self.nz)
-        gnames = ("g_lam", "g_mu", "g_mxz", "g_myx", "g_myz", "g_ir", "g_iru", "g_irw")
+        gnames = GRAD_NAMES
         n_hist = (5 if self.systems & _lib.PSV else 0) + (3 if self.systems & _lib.SH else 0)
         keep, descs = [], []
         S = len(self.geo)
@@ -319,19 +333,31 @@ class ElasticFWI:
             if k == len(self.edges) - 2:
                 self._tune("adjoint", "trtm_fwi_adjoint_batch", descs, a, b, reset_adjoint)
             self._call_batch("trtm_fwi_adjoint_batch", descs, a, b, self.shapes["adjoint"])
-        grads = {k: keep[0][1][k].clone() for k in gnames}
-        for _, gq, _, _ in keep[1:]:
-            for k in gnames:
-                grads[k] += gq[k]
-        s = stream_handle()
+        self.shot_coefficient_gradients = [gq for _, gq, _, _ in keep]
+        grads = accumulate_in_shot_order(None, self.shot_coefficient_gradients)
+        self.coefficient_gradient = grads
+        return self.chain(grads)
+
+    def release(self):
+        """Drop the checkpoints, state, workspaces and per-shot gradients;
+        the medium stays for ``chain``."""
+        self._ck = self.state_all = self.state = None
+        self.shot_coefficient_gradients = None
+        for g in self.geo:
+            g["ws"] = None
+
+    def chain(self, grads):
+        """Coefficient-space gradient (dict over lam, mu, mxz, myx, myz, ir,
+        iru, irw) to (vp, vs, rho) in float64 for the medium of the last
+        forward."""
+        shape = (self.nx, self.nz)
         m = self.med
         dbl = lambda t: t.to(torch.float64).contiguous()
-        out = [torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(3)]
+        out = [torch.empty(shape, dtype=torch.float64, device=device()) for _ in range(3)]
         args = [dbl(m["mu"]), dbl(m["rho"]), dbl(m["vp"]), dbl(m["vs"])] + \
-               [dbl(grads[k]) for k in gnames]
+               [dbl(grads[k]) for k in GRAD_NAMES]
         _lib.call("trtm_fwi_chain_gradient", self.nx, self.nz, *[_ptr(t) for t in args],
-                  *[_ptr(t) for t in out], s)
-        self.coefficient_gradient = grads
+                  *[_ptr(t) for t in out], stream_handle())
         return tuple(out)
 
 
