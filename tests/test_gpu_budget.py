@@ -69,13 +69,6 @@ def test_grouping_does_not_change_gradient_or_image():
         assert torch.equal(st[c], i1[c])
 
 
-def test_plan_raises_below_one_shot():
-    shots = [ShotSpec(np.array([[50, 1]], np.int32), (np.zeros((1, 100)),) * 3,
-                      np.array([[k, 1] for k in range(10)], np.int32))]
-    with pytest.raises(MemoryError):
-        budget.plan(400, 200, shots, 1 << 20)
-
-
 def test_salt_full_grid_within_budget():
     """4000 x 2000 salt model, 2 shots of 240 steps, budget for one shot per
     group: peak device memory of the job stays within the budget."""
