@@ -379,6 +379,9 @@ class ReverseTimeMigration:
         self.offset = 2 * fw_frame * dx
         self.CFL = CFL
         self.src_loc_step = src_step
+        # not kept by the reference: the receiver grid cells and snapshot interval used
+        self.receiver_loc_step = rec_step
+        self.isnap_used = isnap
         return self
 
     def save_result(self, directory: str, savename: str):

@@ -206,7 +206,7 @@ def test_sh_layered_reduced_parity_and_full_size_kernels_agree():
 
 BLOCK_CFGS = {"psv": [(16, 64, 2), (16, 64, 4), (16, 128, 4), (16, 128, 8), (32, 128, 4)],
               "sh": [(16, 64, 4), (16, 64, 8), (16, 128, 4), (16, 128, 8), (32, 128, 4),
-                     (32, 128, 8)],
+                     (32, 128, 8), (24, 128, 8), (32, 96, 8)],
               "both": [(16, 64, 2), (16, 64, 4), (16, 128, 4)]}
 
 
