@@ -21,6 +21,7 @@
 // The grid's second dimension is the shot: shots share the medium and differ
 // in state, record length, injection and recording geometry (ShotDev).
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "trtm_block.hpp"
@@ -34,9 +35,24 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                                               int tiles_x, int tiles_z) {
     constexpr int RX = TX + 2 * K, RZ = TZ + 2 * K, RN = RX * RZ;
     constexpr int NC = (RN + NT - 1) / NT;
-    constexpr int NP = PSV ? RN : 1, NS = SH ? RN : 1;
-    __shared__ float sU[NP], sW[NP], sXX[NP], sZZ[NP], sXZ[NP];
-    __shared__ float sV[NS], sYX[NS], sYZ[NS];
+    // Region arrays with RZ + 1 cells of padding on each side and room for
+    // every thread slot: without a surface matrix the updates run branch-free
+    // on every slot (neighbour reads of edge and spare slots stay inside the
+    // array) and the result is selected per cell.
+    constexpr int PAD = PSV ? 0 : RZ + 1;  // the branch-free updates run for SH only
+    constexpr int NA = PSV ? RN : (RN > NT * NC ? RN : NT * NC) + 2 * PAD;
+    constexpr int NP = PSV ? NA : 1, NS = SH ? NA : 1;
+    __shared__ float sU_[NP], sW_[NP], sXX_[NP], sZZ_[NP], sXZ_[NP];
+    __shared__ float sV_[NS], sYX_[NS], sYZ_[NS];
+    constexpr int OP = PSV ? PAD : 0, OS = SH ? PAD : 0;
+    float* const sU = sU_ + OP;
+    float* const sW = sW_ + OP;
+    float* const sXX = sXX_ + OP;
+    float* const sZZ = sZZ_ + OP;
+    float* const sXZ = sXZ_ + OP;
+    float* const sV = sV_ + OS;
+    float* const sYX = sYX_ + OS;
+    float* const sYZ = sYZ_ + OS;
 
     const ShotDev* __restrict__ sh = shots + blockIdx.y;
     const int it_end = sh->it_end, it_lo = sh->it_lo, it_off = sh->it_off;
@@ -152,56 +168,129 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         const int tb = nt - 1 - it;  // backward time index
 
         // ---- velocity update (region)
+        auto velocity_phase = [&](auto mode) {
+            // mode bit 0: stresses read through the surface matrix; bit 1:
+            // branch-free over every slot (SH: fewer instructions; P-SV keeps
+            // the per-cell branches and one copy of the loop, its registers
+            // are the tighter limit); bit 2: surface matrix decided per launch
+            constexpr bool SM = decltype(mode)::value & 1, BF = decltype(mode)::value & 2;
+            constexpr bool SMR = decltype(mode)::value & 4;
+            auto mk = [&](float x, int c) {
+                if constexpr (SM) return x * sm[c];
+                else if constexpr (SMR) return has_sm ? x * sm[c] : x;
+                else return x;
+            };
 #pragma unroll
-        for (int k = 0; k < NC; k++) {
-            if (!(fl[k] & 1u)) continue;
-            const int e = t + NT * k;
-            const int c = gc[k];
-            const bool upd = (fl[k] & 2u) && (fl[k] & 8u);
-            if (PSV) {
-                float u = sU[e], w = sW[e];
-                if (started) {
-                    u = u * A[k];
-                    w = w * A[k];
-                }
-                if (upd) {
-                    float sxx_x, szz_z, sxz_x, sxz_z;
-                    if (FWD) {
-                        sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
-                        szz_z = (msk(sZZ[e], c) - msk(sZZ[e - 1], c - 1)) * idz;
-                        sxz_x = (msk(sXZ[e], c) - msk(sXZ[e - RZ], c - nz)) * idx;
-                        sxz_z = (msk(sXZ[e], c) - msk(sXZ[e - 1], c - 1)) * idz;
-                        u = u + ((sxx_x + sxz_z) * dt) * c0[k];
-                        w = w + ((sxz_x + szz_z) * dt) * c1[k];
-                    } else {
-                        sxx_x = (sXX[e + RZ] - sXX[e]) * idx;
-                        szz_z = (msk(sZZ[e + 1], c + 1) - msk(sZZ[e], c)) * idz;
-                        sxz_x = (msk(sXZ[e + RZ], c + nz) - msk(sXZ[e], c)) * idx;
-                        sxz_z = (msk(sXZ[e + 1], c + 1) - msk(sXZ[e], c)) * idz;
-                        u = u + ((-(sxx_x + sxz_z)) * dt) * c0[k];
-                        w = w + ((-(sxz_x + szz_z)) * dt) * c1[k];
+            for (int k = 0; k < NC; k++) {
+                if (!BF && !(fl[k] & 1u)) continue;
+                const int e = t + NT * k;
+                const int c = gc[k];
+                const bool upd = (fl[k] & 10u) == 10u;
+                if (PSV) {
+                    float u = sU[e], w = sW[e];
+                    if (started) {
+                        u = u * A[k];
+                        w = w * A[k];
                     }
-                }
-                sU[e] = u;
-                sW[e] = w;
-            }
-            if (SH) {
-                float v = sV[e];
-                if (started) v = v * A[k];
-                if (upd) {
-                    float syx_x, syz_z;
-                    if (FWD) {
-                        syx_x = (sYX[e] - sYX[e - RZ]) * idx;
-                        syz_z = (msk(sYZ[e], c) - msk(sYZ[e - 1], c - 1)) * idz;
-                        v = v + ((syx_x + syz_z) * dt) * d0[k];
-                    } else {
-                        syx_x = (sYX[e + RZ] - sYX[e]) * idx;
-                        syz_z = (msk(sYZ[e + 1], c + 1) - msk(sYZ[e], c)) * idz;
-                        v = v + ((-(syx_x + syz_z)) * dt) * d0[k];
+                    if (BF || upd) {
+                        float sxx_x, szz_z, sxz_x, sxz_z, un, wn;
+                        if (FWD) {
+                            sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
+                            szz_z = (mk(sZZ[e], c) - mk(sZZ[e - 1], c - 1)) * idz;
+                            sxz_x = (mk(sXZ[e], c) - mk(sXZ[e - RZ], c - nz)) * idx;
+                            sxz_z = (mk(sXZ[e], c) - mk(sXZ[e - 1], c - 1)) * idz;
+                            un = u + ((sxx_x + sxz_z) * dt) * c0[k];
+                            wn = w + ((sxz_x + szz_z) * dt) * c1[k];
+                        } else {
+                            sxx_x = (sXX[e + RZ] - sXX[e]) * idx;
+                            szz_z = (mk(sZZ[e + 1], c + 1) - mk(sZZ[e], c)) * idz;
+                            sxz_x = (mk(sXZ[e + RZ], c + nz) - mk(sXZ[e], c)) * idx;
+                            sxz_z = (mk(sXZ[e + 1], c + 1) - mk(sXZ[e], c)) * idz;
+                            un = u + ((-(sxx_x + sxz_z)) * dt) * c0[k];
+                            wn = w + ((-(sxz_x + szz_z)) * dt) * c1[k];
+                        }
+                        u = upd ? un : u;
+                        w = upd ? wn : w;
                     }
+                    sU[e] = u;
+                    sW[e] = w;
                 }
-                sV[e] = v;
+                if (SH) {
+                    float v = sV[e];
+                    if (started) v = v * A[k];
+                    if (BF || upd) {
+                        float syx_x, syz_z, vn;
+                        if (FWD) {
+                            syx_x = (sYX[e] - sYX[e - RZ]) * idx;
+                            syz_z = (mk(sYZ[e], c) - mk(sYZ[e - 1], c - 1)) * idz;
+                            vn = v + ((syx_x + syz_z) * dt) * d0[k];
+                        } else {
+                            syx_x = (sYX[e + RZ] - sYX[e]) * idx;
+                            syz_z = (mk(sYZ[e + 1], c + 1) - mk(sYZ[e], c)) * idz;
+                            vn = v + ((-(syx_x + syz_z)) * dt) * d0[k];
+                        }
+                        v = upd ? vn : v;
+                    }
+                    sV[e] = v;
+                }
             }
+        };
+        if constexpr (PSV) {  // per-cell branches, the original loop
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                if (!(fl[k] & 1u)) continue;
+                const int e = t + NT * k;
+                const int c = gc[k];
+                const bool upd = (fl[k] & 2u) && (fl[k] & 8u);
+                if (PSV) {
+                    float u = sU[e], w = sW[e];
+                    if (started) {
+                        u = u * A[k];
+                        w = w * A[k];
+                    }
+                    if (upd) {
+                        float sxx_x, szz_z, sxz_x, sxz_z;
+                        if (FWD) {
+                            sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
+                            szz_z = (msk(sZZ[e], c) - msk(sZZ[e - 1], c - 1)) * idz;
+                            sxz_x = (msk(sXZ[e], c) - msk(sXZ[e - RZ], c - nz)) * idx;
+                            sxz_z = (msk(sXZ[e], c) - msk(sXZ[e - 1], c - 1)) * idz;
+                            u = u + ((sxx_x + sxz_z) * dt) * c0[k];
+                            w = w + ((sxz_x + szz_z) * dt) * c1[k];
+                        } else {
+                            sxx_x = (sXX[e + RZ] - sXX[e]) * idx;
+                            szz_z = (msk(sZZ[e + 1], c + 1) - msk(sZZ[e], c)) * idz;
+                            sxz_x = (msk(sXZ[e + RZ], c + nz) - msk(sXZ[e], c)) * idx;
+                            sxz_z = (msk(sXZ[e + 1], c + 1) - msk(sXZ[e], c)) * idz;
+                            u = u + ((-(sxx_x + sxz_z)) * dt) * c0[k];
+                            w = w + ((-(sxz_x + szz_z)) * dt) * c1[k];
+                        }
+                    }
+                    sU[e] = u;
+                    sW[e] = w;
+                }
+                if (SH) {
+                    float v = sV[e];
+                    if (started) v = v * A[k];
+                    if (upd) {
+                        float syx_x, syz_z;
+                        if (FWD) {
+                            syx_x = (sYX[e] - sYX[e - RZ]) * idx;
+                            syz_z = (msk(sYZ[e], c) - msk(sYZ[e - 1], c - 1)) * idz;
+                            v = v + ((syx_x + syz_z) * dt) * d0[k];
+                        } else {
+                            syx_x = (sYX[e + RZ] - sYX[e]) * idx;
+                            syz_z = (msk(sYZ[e + 1], c + 1) - msk(sYZ[e], c)) * idz;
+                            v = v + ((-(syx_x + syz_z)) * dt) * d0[k];
+                        }
+                    }
+                    sV[e] = v;
+                }
+            }
+        } else if (has_sm) {
+            velocity_phase(std::integral_constant<int, 1>{});
+        } else {
+            velocity_phase(std::integral_constant<int, 2>{});
         }
         __syncthreads();
 
@@ -262,75 +351,169 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         }
         if (cint > 0 && it % cint == 0) chk = it / cint;
         int code = 0;
+        auto stress_phase = [&](auto mode) {
+            // mode bit 0: stresses read through the surface matrix; bit 1:
+            // branch-free over every slot (SH: fewer instructions; P-SV keeps
+            // the per-cell branches and one copy of the loop, its registers
+            // are the tighter limit); bit 2: surface matrix decided per launch
+            constexpr bool SM = decltype(mode)::value & 1, BF = decltype(mode)::value & 2;
+            constexpr bool SMR = decltype(mode)::value & 4;
+            auto mk = [&](float x, int c) {
+                if constexpr (SM) return x * sm[c];
+                else if constexpr (SMR) return has_sm ? x * sm[c] : x;
+                else return x;
+            };
 #pragma unroll
-        for (int k = 0; k < NC; k++) {
-            if (!(fl[k] & 1u)) continue;
-            const int e = t + NT * k;
-            const int c = gc[k];
-            const bool upd = (fl[k] & 2u) && (fl[k] & 16u);
-            const bool own = fl[k] & 4u;
-            if (PSV) {
-                float sxx = sXX[e], szz = msk(sZZ[e], c), sxz = msk(sXZ[e], c);
-                if (upd) {
-                    float u_x, u_z, w_x, w_z;
-                    if (FWD) {
-                        u_x = (sU[e + RZ] - sU[e]) * idx;
-                        u_z = (sU[e + 1] - sU[e]) * idz;
-                        w_x = (sW[e + RZ] - sW[e]) * idx;
-                        w_z = (sW[e + 1] - sW[e]) * idz;
-                    } else {
-                        u_x = (sU[e] - sU[e - RZ]) * idx;
-                        u_z = (sU[e] - sU[e - 1]) * idz;
-                        w_x = (sW[e] - sW[e - RZ]) * idx;
-                        w_z = (sW[e] - sW[e - 1]) * idz;
+            for (int k = 0; k < NC; k++) {
+                if (!BF && !(fl[k] & 1u)) continue;
+                const int e = t + NT * k;
+                const int c = gc[k];
+                const bool upd = (fl[k] & 18u) == 18u;
+                const bool own = fl[k] & 4u;
+                if (PSV) {
+                    float sxx = sXX[e], szz = mk(sZZ[e], c), sxz = mk(sXZ[e], c);
+                    if (BF || upd) {
+                        float u_x, u_z, w_x, w_z;
+                        if (FWD) {
+                            u_x = (sU[e + RZ] - sU[e]) * idx;
+                            u_z = (sU[e + 1] - sU[e]) * idz;
+                            w_x = (sW[e + RZ] - sW[e]) * idx;
+                            w_z = (sW[e + 1] - sW[e]) * idz;
+                        } else {
+                            u_x = (sU[e] - sU[e - RZ]) * idx;
+                            u_z = (sU[e] - sU[e - 1]) * idz;
+                            w_x = (sW[e] - sW[e - RZ]) * idx;
+                            w_z = (sW[e] - sW[e - 1]) * idz;
+                        }
+                        const float div_uw = u_x + w_z;
+                        const float xxn = sxx + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
+                        const float zzn = szz + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
+                        const float xzn = sxz + (sdt * c4[k]) * (u_z + w_x);
+                        sxx = upd ? xxn : sxx;
+                        szz = upd ? zzn : szz;
+                        sxz = upd ? xzn : sxz;
                     }
-                    const float div_uw = u_x + w_z;
-                    sxx = sxx + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
-                    szz = szz + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
-                    sxz = sxz + (sdt * c4[k]) * (u_z + w_x);
+                    sXX[e] = sxx * A[k];
+                    sZZ[e] = szz * A[k];
+                    sXZ[e] = sxz * A[k];
+                    if (own) {
+                        const float u = sU[e] * A[k], w = sW[e] * A[k];
+                        if (snap >= 0) {
+                            snap_u[(size_t)snap * n + c] = u;
+                            snap_w[(size_t)snap * n + c] = w;
+                        }
+                        if (img >= 0) {
+                            aU[k] = aU[k] + snap_u[(size_t)img * n + c] * u;
+                            aW[k] = aW[k] + snap_w[(size_t)img * n + c] * w;
+                        }
+                        if (chk >= 0) {
+                            if (not_finite_or_big(u) && code < 1) code = 1;
+                            if (not_finite_or_big(w)) code = 3;
+                        }
+                    }
                 }
-                sXX[e] = sxx * A[k];
-                sZZ[e] = szz * A[k];
-                sXZ[e] = sxz * A[k];
-                if (own) {
-                    const float u = sU[e] * A[k], w = sW[e] * A[k];
-                    if (snap >= 0) {
-                        snap_u[(size_t)snap * n + c] = u;
-                        snap_w[(size_t)snap * n + c] = w;
+                if (SH) {
+                    float syx = sYX[e], syz = mk(sYZ[e], c);
+                    if (BF || upd) {
+                        float v_x, v_z;
+                        if (FWD) {
+                            v_x = (sV[e + RZ] - sV[e]) * idx;
+                            v_z = (sV[e + 1] - sV[e]) * idz;
+                        } else {
+                            v_x = (sV[e] - sV[e - RZ]) * idx;
+                            v_z = (sV[e] - sV[e - 1]) * idz;
+                        }
+                        const float yxn = syx + (sdt * d1[k]) * v_x;
+                        const float yzn = syz + (sdt * d2[k]) * v_z;
+                        syx = upd ? yxn : syx;
+                        syz = upd ? yzn : syz;
                     }
-                    if (img >= 0) {
-                        aU[k] = aU[k] + snap_u[(size_t)img * n + c] * u;
-                        aW[k] = aW[k] + snap_w[(size_t)img * n + c] * w;
-                    }
-                    if (chk >= 0) {
-                        if (not_finite_or_big(u) && code < 1) code = 1;
-                        if (not_finite_or_big(w)) code = 3;
+                    sYX[e] = syx * A[k];
+                    sYZ[e] = syz * A[k];
+                    if (own) {
+                        const float v = sV[e] * A[k];
+                        if (snap >= 0) snap_v[(size_t)snap * n + c] = v;
+                        if (img >= 0) aV[k] = aV[k] + snap_v[(size_t)img * n + c] * v;
+                        if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
                     }
                 }
             }
-            if (SH) {
-                float syx = sYX[e], syz = msk(sYZ[e], c);
-                if (upd) {
-                    float v_x, v_z;
-                    if (FWD) {
-                        v_x = (sV[e + RZ] - sV[e]) * idx;
-                        v_z = (sV[e + 1] - sV[e]) * idz;
-                    } else {
-                        v_x = (sV[e] - sV[e - RZ]) * idx;
-                        v_z = (sV[e] - sV[e - 1]) * idz;
+        };
+        if constexpr (PSV) {  // per-cell branches, the original loop
+    #pragma unroll
+            for (int k = 0; k < NC; k++) {
+                if (!(fl[k] & 1u)) continue;
+                const int e = t + NT * k;
+                const int c = gc[k];
+                const bool upd = (fl[k] & 2u) && (fl[k] & 16u);
+                const bool own = fl[k] & 4u;
+                if (PSV) {
+                    float sxx = sXX[e], szz = msk(sZZ[e], c), sxz = msk(sXZ[e], c);
+                    if (upd) {
+                        float u_x, u_z, w_x, w_z;
+                        if (FWD) {
+                            u_x = (sU[e + RZ] - sU[e]) * idx;
+                            u_z = (sU[e + 1] - sU[e]) * idz;
+                            w_x = (sW[e + RZ] - sW[e]) * idx;
+                            w_z = (sW[e + 1] - sW[e]) * idz;
+                        } else {
+                            u_x = (sU[e] - sU[e - RZ]) * idx;
+                            u_z = (sU[e] - sU[e - 1]) * idz;
+                            w_x = (sW[e] - sW[e - RZ]) * idx;
+                            w_z = (sW[e] - sW[e - 1]) * idz;
+                        }
+                        const float div_uw = u_x + w_z;
+                        sxx = sxx + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
+                        szz = szz + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
+                        sxz = sxz + (sdt * c4[k]) * (u_z + w_x);
                     }
-                    syx = syx + (sdt * d1[k]) * v_x;
-                    syz = syz + (sdt * d2[k]) * v_z;
+                    sXX[e] = sxx * A[k];
+                    sZZ[e] = szz * A[k];
+                    sXZ[e] = sxz * A[k];
+                    if (own) {
+                        const float u = sU[e] * A[k], w = sW[e] * A[k];
+                        if (snap >= 0) {
+                            snap_u[(size_t)snap * n + c] = u;
+                            snap_w[(size_t)snap * n + c] = w;
+                        }
+                        if (img >= 0) {
+                            aU[k] = aU[k] + snap_u[(size_t)img * n + c] * u;
+                            aW[k] = aW[k] + snap_w[(size_t)img * n + c] * w;
+                        }
+                        if (chk >= 0) {
+                            if (not_finite_or_big(u) && code < 1) code = 1;
+                            if (not_finite_or_big(w)) code = 3;
+                        }
+                    }
                 }
-                sYX[e] = syx * A[k];
-                sYZ[e] = syz * A[k];
-                if (own) {
-                    const float v = sV[e] * A[k];
-                    if (snap >= 0) snap_v[(size_t)snap * n + c] = v;
-                    if (img >= 0) aV[k] = aV[k] + snap_v[(size_t)img * n + c] * v;
-                    if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
+                if (SH) {
+                    float syx = sYX[e], syz = msk(sYZ[e], c);
+                    if (upd) {
+                        float v_x, v_z;
+                        if (FWD) {
+                            v_x = (sV[e + RZ] - sV[e]) * idx;
+                            v_z = (sV[e + 1] - sV[e]) * idz;
+                        } else {
+                            v_x = (sV[e] - sV[e - RZ]) * idx;
+                            v_z = (sV[e] - sV[e - 1]) * idz;
+                        }
+                        syx = syx + (sdt * d1[k]) * v_x;
+                        syz = syz + (sdt * d2[k]) * v_z;
+                    }
+                    sYX[e] = syx * A[k];
+                    sYZ[e] = syz * A[k];
+                    if (own) {
+                        const float v = sV[e] * A[k];
+                        if (snap >= 0) snap_v[(size_t)snap * n + c] = v;
+                        if (img >= 0) aV[k] = aV[k] + snap_v[(size_t)img * n + c] * v;
+                        if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
+                    }
                 }
             }
+        } else if (has_sm) {
+            stress_phase(std::integral_constant<int, 1>{});
+        } else {
+            stress_phase(std::integral_constant<int, 2>{});
         }
         if (code) atomicMax(sh->check_flags + chk, code + (FWD ? 0 : 3));
 
