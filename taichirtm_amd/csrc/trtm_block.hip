@@ -440,7 +440,7 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
             }
         };
         if constexpr (PSV) {  // per-cell branches, the original loop
-    #pragma unroll
+#pragma unroll
             for (int k = 0; k < NC; k++) {
                 if (!(fl[k] & 1u)) continue;
                 const int e = t + NT * k;
