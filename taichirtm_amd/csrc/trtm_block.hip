@@ -166,6 +166,28 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         if (it < it_lo) continue;
         if (it >= it_end) break;
         const int tb = nt - 1 - it;  // backward time index
+        // this step's snapshot slot to write (forward) or correlate (backward),
+        // finite-check slot; -1 none
+        int snap = -1, img = -1, chk = -1;
+        if (FWD && save && it % isnap == 0 && it != 0) {
+            const int sl = it / isnap - 1 - snap_base;
+            if (sl >= 0 && sl < num_snaps) snap = sl;
+        }
+        if (imaging && tb > 0 && tb % isnap == 0) {
+            const int sl = tb / isnap - 1 - snap_base;
+            if (sl >= 0 && sl < num_snaps && G(sh->isnaps)[sl] == tb) img = sl;
+        }
+        if (cint > 0 && it % cint == 0) chk = it / cint;
+        // SH imaging: the forward wavefield of every slot is loaded here, a
+        // whole phase before the image update uses it (cells outside the grid
+        // read cell 0; only the tile's cells use the value)
+        float fimg[SH && !PSV ? NC : 1];
+        if constexpr (SH && !PSV) {
+            if (img >= 0) {
+#pragma unroll
+                for (int k = 0; k < NC; k++) fimg[k] = snap_v[(size_t)img * n + gc[k]];
+            }
+        }
 
         // ---- velocity update (region)
         auto velocity_phase = [&](auto mode) {
@@ -340,16 +362,6 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         }
 
         // ---- stress update + sponge (region), outputs (tile)
-        int snap = -1, img = -1, chk = -1;
-        if (FWD && save && it % isnap == 0 && it != 0) {
-            const int sl = it / isnap - 1 - snap_base;
-            if (sl >= 0 && sl < num_snaps) snap = sl;
-        }
-        if (imaging && tb > 0 && tb % isnap == 0) {
-            const int sl = tb / isnap - 1 - snap_base;
-            if (sl >= 0 && sl < num_snaps && G(sh->isnaps)[sl] == tb) img = sl;
-        }
-        if (cint > 0 && it % cint == 0) chk = it / cint;
         int code = 0;
         auto stress_phase = [&](auto mode) {
             // mode bit 0: stresses read through the surface matrix; bit 1:
@@ -433,7 +445,11 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                     if (own) {
                         const float v = sV[e] * A[k];
                         if (snap >= 0) snap_v[(size_t)snap * n + c] = v;
-                        if (img >= 0) aV[k] = aV[k] + snap_v[(size_t)img * n + c] * v;
+                        if constexpr (!PSV) {
+                            if (img >= 0) aV[k] = aV[k] + fimg[k] * v;
+                        } else {
+                            if (img >= 0) aV[k] = aV[k] + snap_v[(size_t)img * n + c] * v;
+                        }
                         if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
                     }
                 }
