@@ -181,11 +181,20 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         // SH imaging: the forward wavefield of every slot is loaded here, a
         // whole phase before the image update uses it (cells outside the grid
         // read cell 0; only the tile's cells use the value)
-        float fimg[SH && !PSV ? NC : 1];
+        float fimg[SH && !PSV ? NC : 1], fimu[PSV && !SH ? NC : 1], fimw[PSV && !SH ? NC : 1];
         if constexpr (SH && !PSV) {
             if (img >= 0) {
 #pragma unroll
                 for (int k = 0; k < NC; k++) fimg[k] = snap_v[(size_t)img * n + gc[k]];
+            }
+        }
+        if constexpr (PSV && !SH) {
+            if (img >= 0) {
+#pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    fimu[k] = snap_u[(size_t)img * n + gc[k]];
+                    fimw[k] = snap_w[(size_t)img * n + gc[k]];
+                }
             }
         }
 
@@ -493,8 +502,13 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                             snap_w[(size_t)snap * n + c] = w;
                         }
                         if (img >= 0) {
-                            aU[k] = aU[k] + snap_u[(size_t)img * n + c] * u;
-                            aW[k] = aW[k] + snap_w[(size_t)img * n + c] * w;
+                            if constexpr (!SH) {
+                                aU[k] = aU[k] + fimu[k] * u;
+                                aW[k] = aW[k] + fimw[k] * w;
+                            } else {
+                                aU[k] = aU[k] + snap_u[(size_t)img * n + c] * u;
+                                aW[k] = aW[k] + snap_w[(size_t)img * n + c] * w;
+                            }
                         }
                         if (chk >= 0) {
                             if (not_finite_or_big(u) && code < 1) code = 1;
