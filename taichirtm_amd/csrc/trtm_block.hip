@@ -199,71 +199,37 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         }
 
         // ---- velocity update (region)
-        auto velocity_phase = [&](auto mode) {
-            // mode bit 0: stresses read through the surface matrix; bit 1:
-            // branch-free over every slot (SH: fewer instructions; P-SV keeps
-            // the per-cell branches and one copy of the loop, its registers
-            // are the tighter limit); bit 2: surface matrix decided per launch
-            constexpr bool SM = decltype(mode)::value & 1, BF = decltype(mode)::value & 2;
-            constexpr bool SMR = decltype(mode)::value & 4;
+        // SH-only kernels, without a surface matrix branch-free over every
+        // slot (the result selected per cell); P-SV keeps per-cell branches
+        // (its registers are the tighter limit)
+        auto velocity_phase = [&](auto sm_tag) {
+            constexpr bool SM = decltype(sm_tag)::value;  // stresses read through the surface matrix
             auto mk = [&](float x, int c) {
                 if constexpr (SM) return x * sm[c];
-                else if constexpr (SMR) return has_sm ? x * sm[c] : x;
                 else return x;
             };
 #pragma unroll
             for (int k = 0; k < NC; k++) {
-                if (!BF && !(fl[k] & 1u)) continue;
+                if (SM && !(fl[k] & 1u)) continue;
                 const int e = t + NT * k;
                 const int c = gc[k];
                 const bool upd = (fl[k] & 10u) == 10u;
-                if (PSV) {
-                    float u = sU[e], w = sW[e];
-                    if (started) {
-                        u = u * A[k];
-                        w = w * A[k];
+                float v = sV[e];
+                if (started) v = v * A[k];
+                if (!SM || upd) {
+                    float syx_x, syz_z, vn;
+                    if (FWD) {
+                        syx_x = (sYX[e] - sYX[e - RZ]) * idx;
+                        syz_z = (mk(sYZ[e], c) - mk(sYZ[e - 1], c - 1)) * idz;
+                        vn = v + ((syx_x + syz_z) * dt) * d0[k];
+                    } else {
+                        syx_x = (sYX[e + RZ] - sYX[e]) * idx;
+                        syz_z = (mk(sYZ[e + 1], c + 1) - mk(sYZ[e], c)) * idz;
+                        vn = v + ((-(syx_x + syz_z)) * dt) * d0[k];
                     }
-                    if (BF || upd) {
-                        float sxx_x, szz_z, sxz_x, sxz_z, un, wn;
-                        if (FWD) {
-                            sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
-                            szz_z = (mk(sZZ[e], c) - mk(sZZ[e - 1], c - 1)) * idz;
-                            sxz_x = (mk(sXZ[e], c) - mk(sXZ[e - RZ], c - nz)) * idx;
-                            sxz_z = (mk(sXZ[e], c) - mk(sXZ[e - 1], c - 1)) * idz;
-                            un = u + ((sxx_x + sxz_z) * dt) * c0[k];
-                            wn = w + ((sxz_x + szz_z) * dt) * c1[k];
-                        } else {
-                            sxx_x = (sXX[e + RZ] - sXX[e]) * idx;
-                            szz_z = (mk(sZZ[e + 1], c + 1) - mk(sZZ[e], c)) * idz;
-                            sxz_x = (mk(sXZ[e + RZ], c + nz) - mk(sXZ[e], c)) * idx;
-                            sxz_z = (mk(sXZ[e + 1], c + 1) - mk(sXZ[e], c)) * idz;
-                            un = u + ((-(sxx_x + sxz_z)) * dt) * c0[k];
-                            wn = w + ((-(sxz_x + szz_z)) * dt) * c1[k];
-                        }
-                        u = upd ? un : u;
-                        w = upd ? wn : w;
-                    }
-                    sU[e] = u;
-                    sW[e] = w;
+                    v = upd ? vn : v;
                 }
-                if (SH) {
-                    float v = sV[e];
-                    if (started) v = v * A[k];
-                    if (BF || upd) {
-                        float syx_x, syz_z, vn;
-                        if (FWD) {
-                            syx_x = (sYX[e] - sYX[e - RZ]) * idx;
-                            syz_z = (mk(sYZ[e], c) - mk(sYZ[e - 1], c - 1)) * idz;
-                            vn = v + ((syx_x + syz_z) * dt) * d0[k];
-                        } else {
-                            syx_x = (sYX[e + RZ] - sYX[e]) * idx;
-                            syz_z = (mk(sYZ[e + 1], c + 1) - mk(sYZ[e], c)) * idz;
-                            vn = v + ((-(syx_x + syz_z)) * dt) * d0[k];
-                        }
-                        v = upd ? vn : v;
-                    }
-                    sV[e] = v;
-                }
+                sV[e] = v;
             }
         };
         if constexpr (PSV) {  // per-cell branches, the original loop
@@ -319,9 +285,9 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                 }
             }
         } else if (has_sm) {
-            velocity_phase(std::integral_constant<int, 1>{});
+            velocity_phase(std::true_type{});
         } else {
-            velocity_phase(std::integral_constant<int, 2>{});
+            velocity_phase(std::false_type{});
         }
         __syncthreads();
 
@@ -372,95 +338,40 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
 
         // ---- stress update + sponge (region), outputs (tile)
         int code = 0;
-        auto stress_phase = [&](auto mode) {
-            // mode bit 0: stresses read through the surface matrix; bit 1:
-            // branch-free over every slot (SH: fewer instructions; P-SV keeps
-            // the per-cell branches and one copy of the loop, its registers
-            // are the tighter limit); bit 2: surface matrix decided per launch
-            constexpr bool SM = decltype(mode)::value & 1, BF = decltype(mode)::value & 2;
-            constexpr bool SMR = decltype(mode)::value & 4;
+        auto stress_phase = [&](auto sm_tag) {
+            constexpr bool SM = decltype(sm_tag)::value;
             auto mk = [&](float x, int c) {
                 if constexpr (SM) return x * sm[c];
-                else if constexpr (SMR) return has_sm ? x * sm[c] : x;
                 else return x;
             };
 #pragma unroll
             for (int k = 0; k < NC; k++) {
-                if (!BF && !(fl[k] & 1u)) continue;
+                if (SM && !(fl[k] & 1u)) continue;
                 const int e = t + NT * k;
                 const int c = gc[k];
                 const bool upd = (fl[k] & 18u) == 18u;
-                const bool own = fl[k] & 4u;
-                if (PSV) {
-                    float sxx = sXX[e], szz = mk(sZZ[e], c), sxz = mk(sXZ[e], c);
-                    if (BF || upd) {
-                        float u_x, u_z, w_x, w_z;
-                        if (FWD) {
-                            u_x = (sU[e + RZ] - sU[e]) * idx;
-                            u_z = (sU[e + 1] - sU[e]) * idz;
-                            w_x = (sW[e + RZ] - sW[e]) * idx;
-                            w_z = (sW[e + 1] - sW[e]) * idz;
-                        } else {
-                            u_x = (sU[e] - sU[e - RZ]) * idx;
-                            u_z = (sU[e] - sU[e - 1]) * idz;
-                            w_x = (sW[e] - sW[e - RZ]) * idx;
-                            w_z = (sW[e] - sW[e - 1]) * idz;
-                        }
-                        const float div_uw = u_x + w_z;
-                        const float xxn = sxx + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
-                        const float zzn = szz + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
-                        const float xzn = sxz + (sdt * c4[k]) * (u_z + w_x);
-                        sxx = upd ? xxn : sxx;
-                        szz = upd ? zzn : szz;
-                        sxz = upd ? xzn : sxz;
+                float syx = sYX[e], syz = mk(sYZ[e], c);
+                if (!SM || upd) {
+                    float v_x, v_z;
+                    if (FWD) {
+                        v_x = (sV[e + RZ] - sV[e]) * idx;
+                        v_z = (sV[e + 1] - sV[e]) * idz;
+                    } else {
+                        v_x = (sV[e] - sV[e - RZ]) * idx;
+                        v_z = (sV[e] - sV[e - 1]) * idz;
                     }
-                    sXX[e] = sxx * A[k];
-                    sZZ[e] = szz * A[k];
-                    sXZ[e] = sxz * A[k];
-                    if (own) {
-                        const float u = sU[e] * A[k], w = sW[e] * A[k];
-                        if (snap >= 0) {
-                            snap_u[(size_t)snap * n + c] = u;
-                            snap_w[(size_t)snap * n + c] = w;
-                        }
-                        if (img >= 0) {
-                            aU[k] = aU[k] + snap_u[(size_t)img * n + c] * u;
-                            aW[k] = aW[k] + snap_w[(size_t)img * n + c] * w;
-                        }
-                        if (chk >= 0) {
-                            if (not_finite_or_big(u) && code < 1) code = 1;
-                            if (not_finite_or_big(w)) code = 3;
-                        }
-                    }
+                    const float yxn = syx + (sdt * d1[k]) * v_x;
+                    const float yzn = syz + (sdt * d2[k]) * v_z;
+                    syx = upd ? yxn : syx;
+                    syz = upd ? yzn : syz;
                 }
-                if (SH) {
-                    float syx = sYX[e], syz = mk(sYZ[e], c);
-                    if (BF || upd) {
-                        float v_x, v_z;
-                        if (FWD) {
-                            v_x = (sV[e + RZ] - sV[e]) * idx;
-                            v_z = (sV[e + 1] - sV[e]) * idz;
-                        } else {
-                            v_x = (sV[e] - sV[e - RZ]) * idx;
-                            v_z = (sV[e] - sV[e - 1]) * idz;
-                        }
-                        const float yxn = syx + (sdt * d1[k]) * v_x;
-                        const float yzn = syz + (sdt * d2[k]) * v_z;
-                        syx = upd ? yxn : syx;
-                        syz = upd ? yzn : syz;
-                    }
-                    sYX[e] = syx * A[k];
-                    sYZ[e] = syz * A[k];
-                    if (own) {
-                        const float v = sV[e] * A[k];
-                        if (snap >= 0) snap_v[(size_t)snap * n + c] = v;
-                        if constexpr (!PSV) {
-                            if (img >= 0) aV[k] = aV[k] + fimg[k] * v;
-                        } else {
-                            if (img >= 0) aV[k] = aV[k] + snap_v[(size_t)img * n + c] * v;
-                        }
-                        if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
-                    }
+                sYX[e] = syx * A[k];
+                sYZ[e] = syz * A[k];
+                if (fl[k] & 4u) {  // the tile's cell: outputs
+                    const float v = sV[e] * A[k];
+                    if (snap >= 0) snap_v[(size_t)snap * n + c] = v;
+                    if (img >= 0) aV[k] = aV[k] + fimg[k] * v;
+                    if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
                 }
             }
         };
@@ -541,9 +452,9 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                 }
             }
         } else if (has_sm) {
-            stress_phase(std::integral_constant<int, 1>{});
+            stress_phase(std::true_type{});
         } else {
-            stress_phase(std::integral_constant<int, 2>{});
+            stress_phase(std::false_type{});
         }
         if (code) atomicMax(sh->check_flags + chk, code + (FWD ? 0 : 3));
 
