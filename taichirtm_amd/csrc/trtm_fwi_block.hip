@@ -491,19 +491,30 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
 
     // The history window of step it goes to buffer b asynchronously (global
     // to LDS): issued one step ahead, waited for before the step's phase 1b.
-    const int lane = t & 63, wave = t >> 6;
+    // Wave w copies chunks q = w, w + NW, ... of the window (q = field *
+    // chunks + chunk); the source offset of this lane in each of them is the
+    // same every step, so it is computed once.
+    constexpr int NW = NT / 64, NQ = W::fields * W::chunks, QPW = (NQ + NW - 1) / NW;
+    const int lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    unsigned poff[QPW];  // byte offset from the step's history slot
+#pragma unroll
+    for (int m = 0; m < QPW; m++) {
+        const int q = wave + NW * m;
+        const int f = q / W::chunks, ch = q - f * W::chunks;
+        const int w = ch * 64 + lane;
+        const int r = w / W::HZ, z = w - r * W::HZ;
+        const int i = clampi(i0 - 1 + r, 0, nx - 1), j = clampi(j0 - 1 + z, 0, nz - 1);
+        poff[m] = (unsigned)((q < NQ ? f : 0) * (int)n + i * nz + j) * 4u;
+    }
     auto prefetch = [&](int it, float* dst) {
-        const float* base = a.hist + (size_t)(it - a.hist_base) * (H::n * n);
-        for (int q = wave; q < W::fields * W::chunks; q += NT / 64) {
-            const int f = q / W::chunks, ch = q - f * W::chunks;
-            const int w = ch * 64 + lane;
-            const int r = w / W::HZ, z = w - r * W::HZ;
-            const int i = clampi(i0 - 1 + r, 0, nx - 1), j = clampi(j0 - 1 + z, 0, nz - 1);
-            const unsigned off = (unsigned)(f * (int)n + i * nz + j) * 4u;
-            __builtin_amdgcn_global_load_lds(
-                (const void*)((const char*)base + off),
-                (__attribute__((address_space(3))) void*)&dst[f * W::chunks * 64 + ch * 64],
-                4, 0, 0);
+        const char* base = (const char*)(a.hist + (size_t)(it - a.hist_base) * (H::n * n));
+#pragma unroll
+        for (int m = 0; m < QPW; m++) {
+            const int q = wave + NW * m;  // wave-uniform
+            if (q < NQ)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(base + poff[m]),
+                    (__attribute__((address_space(3))) void*)&dst[q * 64], 4, 0, 0);
         }
     };
     // window offset of each region cell (own interior cells only use it)
