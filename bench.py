@@ -59,6 +59,7 @@ ALGO_BYTES = {
 MEASURED_TRAFFIC = {
     ("sh_layered_1200x600x4000", "blocked", (32, 128, 1024, 8)): 57.4e6,
     ("sh_layered_1200x600x4000", "blocked", (24, 128, 768, 8)): 59.5e6,
+    ("sh_layered_1200x600x4000", "blocked", (24, 128, 1024, 8)): 59.5e6,
 }
 
 

@@ -518,7 +518,7 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
 // configurations compiled per wave system
 static const BlockCfg kCfgSH[] = {{16, 64, 256, 4},   {16, 64, 256, 8},    {16, 128, 512, 4},
                                   {16, 128, 512, 8},  {32, 128, 1024, 4},  {32, 128, 1024, 8},
-                                  {24, 128, 768, 8},  {32, 96, 1024, 8}};
+                                  {24, 128, 1024, 8}, {32, 96, 1024, 8}};
 static const BlockCfg kCfgPSV[] = {{16, 64, 256, 2},  {16, 64, 256, 4},    {16, 128, 512, 4},
                                    {16, 128, 512, 8}, {32, 128, 1024, 4}};
 static const BlockCfg kCfgBoth[] = {{16, 64, 256, 2}, {16, 64, 256, 4}, {16, 128, 512, 4}};
@@ -787,7 +787,7 @@ static bool launch_sys(BlockCfg c, const KArgs& m, const ShotDev* shots, int n_s
     } else {
         TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 64, 256, 8) TRTM_CFG(16, 128, 512, 4)
         TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4) TRTM_CFG(32, 128, 1024, 8)
-        TRTM_CFG(24, 128, 768, 8) TRTM_CFG(32, 96, 1024, 8)
+        TRTM_CFG(24, 128, 1024, 8) TRTM_CFG(32, 96, 1024, 8)
     }
 #undef TRTM_CFG
     return false;
