@@ -539,7 +539,7 @@ static void cfg_list(int systems, const BlockCfg** list, int* n) {
 // defaults: the fastest shapes measured on MI355X for the 1200 x 600 SH and
 // 200 x 200 / 2000 x 1000 P-SV workloads
 static BlockCfg default_cfg(int systems) {
-    if (systems == TRTM_SH) return {32, 128, 1024, 8};
+    if (systems == TRTM_SH) return {24, 128, 1024, 8};
     if (systems == TRTM_PSV) return {32, 128, 1024, 4};
     return {16, 128, 512, 4};
 }
