@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TRTM_ABI_VERSION 3
+#define TRTM_ABI_VERSION 4
 
 #define TRTM_OK 0
 #define TRTM_ERR_ARG (-1)         /* invalid argument or shape */
@@ -161,6 +161,12 @@ typedef struct trtm_step_desc {
     /* scratch of trtm_workspace_bytes() bytes (fused kernel), may be NULL otherwise */
     void *workspace;
     int64_t workspace_bytes;
+    /* snapshots in another layout (blocked kernel only): slot stride in floats
+     * (0: nx*nz), and snap_sponge = 1 when each stored value still lacks the
+     * cell's sponge factor, which the backward image then applies (snapshots
+     * taken from an FWI history: velocities before the sponge) */
+    int64_t snap_stride;
+    int32_t snap_sponge, reserved_snap;
 } trtm_step_desc;
 
 /* Scratch bytes trtm_forward_steps / trtm_backward_steps need for desc. */

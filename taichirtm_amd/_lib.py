@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtaichirtm_hip.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 OK = 0
 ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3
 PSV, SH = 1, 2
@@ -47,6 +47,7 @@ class StepDesc(ctypes.Structure):
         ("isnaps_host", _p),
         ("image_u", _p), ("image_v", _p), ("image_w", _p),
         ("workspace", _p), ("workspace_bytes", ctypes.c_int64),
+        ("snap_stride", ctypes.c_int64), ("snap_sponge", _i32), ("reserved_snap", _i32),
     ]
 
 

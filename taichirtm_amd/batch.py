@@ -330,6 +330,115 @@ class BatchRTM:
                 self._tune("backward", "trtm_backward_steps_batch", descs, -b, -a, (), reset_bw)
             self._batch("trtm_backward_steps_batch", descs, -b, -a)
 
+    # -- backward fed with forward wavefields from elsewhere ------------------
+
+    def external_begin(self, fwd_systems, L):
+        """Prepare the backward passes for ``external_segment``: the forward
+        wavefield comes segment by segment from an FWI history (the FWI
+        forward is the reference's forward step, forward_modeling.py:539-551,
+        so A times its post-injection velocity is the forward state after each
+        step).  No forward propagators, checkpoints or snapshot buffers are
+        held here; imaging every step (isnap 1), blocked kernel."""
+        if self.isnap != 1:
+            raise ValueError("external forward wavefields need isnap = 1")
+        if any(s.observed is None for s in self.shots):
+            raise ValueError("external forward wavefields need every shot's observed data")
+        self.block["kernel"] = "blocked" if self.block["kernel"] == "wave" else self.block["kernel"]
+        self._ext_fields = {"u": 3, "w": 4, "v": 7 if fwd_systems & _lib.PSV else 2}
+        self._ext_nhist = (5 if fwd_systems & _lib.PSV else 0) + (3 if fwd_systems & _lib.SH else 0)
+        self.L = int(L)
+        dev = device()
+        self.bw, self.flags_b, descs, keep = [], [], [], []
+        for s in self.shots:
+            obs = [np.asarray(o, np.float32) for o in s.observed]
+            common = self._common(s)
+            if common["kernel"] == "tuned":
+                common["kernel"] = "blocked"
+            b = BackwardModeling(obsdata_u_field=obs[0], obsdata_v_field=obs[1],
+                                 obsdata_w_field=obs[2], **common)
+            if b.systems & ~fwd_systems:
+                raise ValueError("observed data drive a wave system the forward history lacks")
+            d = b._desc()
+            d.kernel = _lib.KERNEL_BLOCKED
+            fl = torch.zeros(b.nt // self.sci + 1, dtype=torch.int32, device=dev)
+            d.check_interval, d.check_flags = self.sci, _ptr(fl)
+            d.isnap = 1
+            d.image_u, d.image_v, d.image_w = (_ptr(b._img.get(c)) for c in ("u", "v", "w"))
+            d.num_snaps = self.L  # workspace sizing
+            keep.append(b._workspace(d))
+            d.snap_sponge = 1
+            self.bw.append(b)
+            self.flags_b.append(fl)
+            descs.append(d)
+        self.systems = {b.systems for b in self.bw}.pop() if len({b.systems for b in self.bw}) == 1 \
+            else None
+        if self.systems is None:
+            raise ValueError("all shots of a batch must excite the same wave systems")
+        self._ext_descs, self._bw_ws, self._ext_isn = descs, keep, []
+        self._ext_tuned = False
+
+    def external_segment(self, a, b, hists):
+        """Backward steps of forward times b-1 down to a for every shot, the
+        image taking the forward wavefield F(t) = sponge x hist[t - a] of the
+        shot's history (fields in the FWI order sxx, szz, sxz, u, w, syx, syz,
+        v of the active systems)."""
+        n = self.nx * self.nz
+        nslot = b - a
+        if nslot > self.L:
+            raise ValueError("segment longer than the history")
+        self._ext_isn = []
+        descs = self._ext_descs
+        for q, (s, d, h) in enumerate(zip(self.shots, descs, hists)):
+            isn = np.full(self.L, -1, np.int32)
+            for j in range(nslot):
+                t = a + j
+                if 1 <= t < s.nt:
+                    isn[j] = t
+            self._ext_isn.append(isn)
+            base = h.data_ptr()
+            d.snap_base = a - 1
+            d.num_snaps = self.L
+            d.snap_stride = self._ext_nhist * n
+            d.snap_u, d.snap_v, d.snap_w = (
+                base + 4 * self._ext_fields[c] * n if self.systems & bit else None
+                for c, bit in (("u", _lib.PSV), ("v", _lib.SH), ("w", _lib.PSV)))
+            d.isnaps_host = isn.ctypes.data_as(ctypes.c_void_p)
+            d.step_offset = s.nt
+        if not self._ext_tuned and self.block.get("kernel") == "tuned":
+            from . import autotune
+            from ._propagator import set_shape
+            key = ("batch-external", self.nx, self.nz, self.medium.fw, self.systems, len(descs))
+            hi_ = min(b, a + 64)
+
+            def reset():  # the first segment taken starts from rest
+                for bw, fl in zip(self.bw, self.flags_b):
+                    bw.reset_state()
+                    fl.zero_()
+
+            def trial(shape):
+                for d in descs:
+                    set_shape(d, shape)
+                self._batch("trtm_backward_steps_batch", descs, -hi_, -a)
+                reset()
+
+            shape = autotune.tune(key, self.systems, trial,
+                                  shapes=[c for c in autotune.candidates(self.systems)
+                                          if c[0] == "blocked"])
+            for d in descs:
+                set_shape(d, shape)
+            self.tuned["backward"] = shape
+            self._ext_tuned = True
+        self._batch("trtm_backward_steps_batch", descs, -b, -a)
+
+    def external_codes(self) -> List[int]:
+        """Per-shot backward codes (0, or 4-6 with the first failing check step)."""
+        out = []
+        for fl in self.flags_b:
+            h = fl.cpu().numpy()
+            nz_ = np.nonzero(h)[0]
+            out.append(int(h[nz_[0]]) if len(nz_) else 0)
+        return out
+
     # -- driver -------------------------------------------------------------
 
     def run(self) -> List[int]:

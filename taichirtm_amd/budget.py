@@ -3,13 +3,17 @@
 A survey's gradient (exact adjoint, ``fwi.ElasticFWI``) and image (the
 reference's back-propagation of the observed data with zero-lag imaging,
 ``batch.BatchRTM``, backward_modeling.py:539-576 and imaging.py:81-93) are
-computed shot group by shot group.  The group size and the checkpoint spacing
-of each pass come from the budget: per shot, the FWI pass holds its state,
-adjoint state, gradient accumulators, blocked-kernel workspace, checkpoints
-every L steps and an L-step history; the RTM pass its forward and backward
-propagators, checkpoints and an L-step snapshot buffer.  L minimises that
-per-shot memory (about sqrt(nt)); the group is the largest number of shots
-whose memory fits the budget next to the shared medium.
+computed shot group by shot group.  One forward pass keeps checkpoints every L
+steps; walking the segments last to first, each segment's forward history is
+recomputed once and feeds both backward passes: the FWI adjoint reads it as
+its history, and the RTM backward takes the forward wavefield of each step as
+the sponge factor times the history's post-injection velocities (exactly the
+forward state after the step).  Per shot the job holds the FWI state, adjoint
+state, gradient accumulators, blocked-kernel workspace, checkpoints and the
+L-step history, plus the RTM backward propagator; L minimises that memory
+(about sqrt(nt)) and the group is the largest number of shots whose memory
+fits the budget next to the shared medium.  (``share_forward=False`` runs the
+two passes one after the other instead, each with its own checkpoints.)
 
 The per-shot coefficient gradients are accumulated in shot order in float32
 and the per-shot images in float64, so the result does not depend on the
@@ -57,8 +61,17 @@ def _rtm_shot_bytes(cells, nt, nr, L, isnap):
     return fields * cells * _F32 + 12 * nr * nt * _F32
 
 
-def plan(nx, nz, shots: Sequence[ShotSpec], budget_bytes, isnap=1) -> Plan:
-    """Group size and checkpoint spacings that fit ``budget_bytes``."""
+def _rtm_backward_shot_bytes(cells, nt, nr):
+    # backward only (forward wavefield from the FWI history): state 5 +
+    # workspace 8 + image 3; observed data / synthetic source
+    return 16 * cells * _F32 + 6 * nr * nt * _F32
+
+
+def plan(nx, nz, shots: Sequence[ShotSpec], budget_bytes, isnap=1, share_forward=True) -> Plan:
+    """Group size and checkpoint spacings that fit ``budget_bytes``.  With
+    ``share_forward`` the RTM backward pass runs beside the FWI adjoint on the
+    same recomputed forward history (one forward recompute for both); else
+    the two passes run one after the other, each with its own checkpoints."""
     cells = nx * nz
     nt = max(s.nt for s in shots)
     nr = max(len(np.asarray(s.rec_loc).reshape(-1, 2)) for s in shots)
@@ -66,11 +79,17 @@ def plan(nx, nz, shots: Sequence[ShotSpec], budget_bytes, isnap=1) -> Plan:
     # float64 chain rule (30), margin
     shared = 80 * cells * _F32
     L_fwi = checkpoint_interval(nt, 5, 5)
-    L_rtm = checkpoint_interval(nt, 5, 2, isnap, align=isnap)
     per_fwi = _fwi_shot_bytes(cells, nt, nr, L_fwi)
-    per_rtm = _rtm_shot_bytes(cells, nt, nr, L_rtm, isnap)
+    if share_forward:
+        L_rtm = L_fwi
+        per_rtm = _rtm_backward_shot_bytes(cells, nt, nr)
+        per_shot = per_fwi + per_rtm
+    else:
+        L_rtm = checkpoint_interval(nt, 5, 2, isnap, align=isnap)
+        per_rtm = _rtm_shot_bytes(cells, nt, nr, L_rtm, isnap)
+        per_shot = max(per_fwi, per_rtm)
     free = budget_bytes - shared
-    group = min(len(shots), int(free // max(per_fwi, per_rtm))) if free > 0 else 0
+    group = min(len(shots), int(free // per_shot)) if free > 0 else 0
     if group < 1:
         raise MemoryError("one shot needs %.2f GB (FWI) / %.2f GB (RTM) next to %.2f GB shared; "
                           "budget %.2f GB" % (per_fwi / 1e9, per_rtm / 1e9, shared / 1e9,
@@ -85,15 +104,18 @@ class GradientAndImage:
     (nr, nt)."""
 
     def __init__(self, nx, nz, dx, dz, fs, absorbing_frame, shots: List[ShotSpec],
-                 budget_bytes, isnap=1, kernel="tuned"):
+                 budget_bytes, isnap=1, kernel="tuned", share_forward=True):
         if any(s.observed is None for s in shots):
             raise ValueError("every shot needs its observed data")
+        if share_forward and int(isnap) != 1:
+            raise ValueError("sharing the forward history images every step (isnap = 1)")
         self.nx, self.nz, self.dx, self.dz, self.fs = int(nx), int(nz), dx, dz, fs
         self.fw = int(absorbing_frame)
         self.shots = list(shots)
         self.isnap = int(isnap)
         self.kernel = kernel
-        self.plan = plan(self.nx, self.nz, self.shots, budget_bytes, self.isnap)
+        self.share_forward = share_forward
+        self.plan = plan(self.nx, self.nz, self.shots, budget_bytes, self.isnap, share_forward)
 
     def run(self, vp, vs, rho):
         """(J, (g_vp, g_vs, g_rho) float64, image {"u", "v", "w"} float64)."""
@@ -122,16 +144,25 @@ class GradientAndImage:
                 J += 0.5 * float((d.double() ** 2).sum())
                 res.append(d)
             del recs
-            fwi.gradient(res)
+            rkernel = self.kernel if self.kernel in ("tuned", "blocked", "wave") else "blocked"
+            if self.share_forward:
+                # the RTM backward runs beside the adjoint on each recomputed
+                # segment of the forward history
+                rtm = BatchRTM(self.nx, self.nz, self.dx, self.dz, self.fs, mu, lam, rho_n,
+                               self.fw, group, isnap=1, imaging="checkpoint", kernel=rkernel)
+                rtm.external_begin(fwi.systems, fwi.L)
+                fwi.gradient(res, on_segment=rtm.external_segment)
+                codes = rtm.external_codes()
+            else:
+                fwi.gradient(res)
             gsum = accumulate_in_shot_order(gsum, fwi.shot_coefficient_gradients)
             fwi.release()
             last = fwi
-            rtm = BatchRTM(self.nx, self.nz, self.dx, self.dz, self.fs, mu, lam, rho_n, self.fw,
-                           group, isnap=self.isnap, imaging="checkpoint",
-                           checkpoint_every=p.L_rtm,
-                           kernel=self.kernel if self.kernel in ("tuned", "blocked", "wave")
-                           else "blocked")
-            codes = rtm.run()
+            if not self.share_forward:
+                rtm = BatchRTM(self.nx, self.nz, self.dx, self.dz, self.fs, mu, lam, rho_n,
+                               self.fw, group, isnap=self.isnap, imaging="checkpoint",
+                               checkpoint_every=p.L_rtm, kernel=rkernel)
+                codes = rtm.run()
             if any(codes):
                 raise RuntimeError("non-finite wavefield, codes %s" % codes)
             for imgs in rtm.image_tensors():

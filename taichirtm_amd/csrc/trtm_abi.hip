@@ -168,6 +168,12 @@ static int check_desc(const trtm_step_desc* d, int32_t it_begin, int32_t it_end)
     if (d->check_interval > 0 && !d->check_flags) return fail(TRTM_ERR_ARG, "check_flags is NULL");
     if (d->kernel < TRTM_KERNEL_AUTO || d->kernel > TRTM_KERNEL_WAVE)
         return fail(TRTM_ERR_ARG, "bad kernel selector %d", d->kernel);
+    if (d->snap_stride < 0 || (d->snap_stride > 0 && d->snap_stride < (int64_t)d->nx * d->nz) ||
+        (d->snap_sponge != 0 && d->snap_sponge != 1))
+        return fail(TRTM_ERR_ARG, "snap_stride must be 0 or >= nx*nz, snap_sponge 0 or 1");
+    if ((d->snap_stride > 0 || d->snap_sponge) && d->kernel != TRTM_KERNEL_BLOCKED)
+        return fail(TRTM_ERR_UNSUPPORTED,
+                    "a snapshot stride or sponge needs the blocked kernel (TRTM_KERNEL_BLOCKED)");
     return TRTM_OK;
 }
 

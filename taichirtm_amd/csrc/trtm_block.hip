@@ -159,6 +159,8 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
     const int snap_base = sh->snap_base, cint = sh->check_interval;
     const int n_groups = sh->n_groups, n_rec = sh->n_rec;
     const gptr<float> snap_u = G(sh->snap_u), snap_v = G(sh->snap_v), snap_w = G(sh->snap_w);
+    const size_t sstride = (size_t)sh->snap_stride;
+    const bool ssponge = sh->snap_sponge != 0;
 
     bool started = false;  // a step ran in this launch: LDS velocities await the sponge
     for (int s = 0; s < K; s++) {
@@ -185,15 +187,22 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         if constexpr (SH && !PSV) {
             if (img >= 0) {
 #pragma unroll
-                for (int k = 0; k < NC; k++) fimg[k] = snap_v[(size_t)img * n + gc[k]];
+                for (int k = 0; k < NC; k++) {
+                    fimg[k] = snap_v[(size_t)img * sstride + gc[k]];
+                    if (ssponge) fimg[k] = fimg[k] * A[k];
+                }
             }
         }
         if constexpr (PSV && !SH) {
             if (img >= 0) {
 #pragma unroll
                 for (int k = 0; k < NC; k++) {
-                    fimu[k] = snap_u[(size_t)img * n + gc[k]];
-                    fimw[k] = snap_w[(size_t)img * n + gc[k]];
+                    fimu[k] = snap_u[(size_t)img * sstride + gc[k]];
+                    fimw[k] = snap_w[(size_t)img * sstride + gc[k]];
+                    if (ssponge) {
+                        fimu[k] = fimu[k] * A[k];
+                        fimw[k] = fimw[k] * A[k];
+                    }
                 }
             }
         }
@@ -369,7 +378,7 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                 sYZ[e] = syz * A[k];
                 if (fl[k] & 4u) {  // the tile's cell: outputs
                     const float v = sV[e] * A[k];
-                    if (snap >= 0) snap_v[(size_t)snap * n + c] = v;
+                    if (snap >= 0) snap_v[(size_t)snap * sstride + c] = v;
                     if (img >= 0) aV[k] = aV[k] + fimg[k] * v;
                     if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
                 }
@@ -409,16 +418,22 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                     if (own) {
                         const float u = sU[e] * A[k], w = sW[e] * A[k];
                         if (snap >= 0) {
-                            snap_u[(size_t)snap * n + c] = u;
-                            snap_w[(size_t)snap * n + c] = w;
+                            snap_u[(size_t)snap * sstride + c] = u;
+                            snap_w[(size_t)snap * sstride + c] = w;
                         }
                         if (img >= 0) {
                             if constexpr (!SH) {
                                 aU[k] = aU[k] + fimu[k] * u;
                                 aW[k] = aW[k] + fimw[k] * w;
                             } else {
-                                aU[k] = aU[k] + snap_u[(size_t)img * n + c] * u;
-                                aW[k] = aW[k] + snap_w[(size_t)img * n + c] * w;
+                                float fu = snap_u[(size_t)img * sstride + c];
+                                float fw = snap_w[(size_t)img * sstride + c];
+                                if (ssponge) {
+                                    fu = fu * A[k];
+                                    fw = fw * A[k];
+                                }
+                                aU[k] = aU[k] + fu * u;
+                                aW[k] = aW[k] + fw * w;
                             }
                         }
                         if (chk >= 0) {
@@ -445,8 +460,12 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                     sYZ[e] = syz * A[k];
                     if (own) {
                         const float v = sV[e] * A[k];
-                        if (snap >= 0) snap_v[(size_t)snap * n + c] = v;
-                        if (img >= 0) aV[k] = aV[k] + snap_v[(size_t)img * n + c] * v;
+                        if (snap >= 0) snap_v[(size_t)snap * sstride + c] = v;
+                        if (img >= 0) {
+                            float fv = snap_v[(size_t)img * sstride + c];
+                            if (ssponge) fv = fv * A[k];
+                            aV[k] = aV[k] + fv * v;
+                        }
                         if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
                     }
                 }
@@ -744,6 +763,8 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
         x.rmap_slot = wm.rmap_slot;
         x.rmap = wm.rmap;
     }
+    x.snap_stride = d->snap_stride > 0 ? d->snap_stride : (long long)d->nx * d->nz;
+    x.snap_sponge = d->snap_sponge;
     x.snap_u = d->snap_u;
     x.snap_v = d->snap_v;
     x.snap_w = d->snap_w;

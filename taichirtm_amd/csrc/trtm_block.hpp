@@ -22,6 +22,8 @@ struct ShotDev {
     // single-entry injection cells (wavelet row) and of the receivers (index)
     const int *imap_slot, *imap, *rmap_slot, *rmap;
     float *snap_u, *snap_v, *snap_w;
+    long long snap_stride;  // floats per snapshot slot
+    int snap_sponge, pad3;  // 1: stored values lack the sponge factor (backward applies it)
     const int* isnaps;
     float *image_u, *image_v, *image_w;
     int* check_flags;

@@ -282,13 +282,17 @@ class ElasticFWI:
 
     # -- gradient ---------------------------------------------------------------
 
-    def gradient(self, data_adjoint: List[torch.Tensor]):
+    def gradient(self, data_adjoint: List[torch.Tensor], on_segment=None):
         """Gradient of J with respect to (vp, vs, rho), float64 (nx, nz) each,
         for dJ/d(records) given per shot as (3, nr, nt).  Segments last to
         first: every shot's history is recomputed from its checkpoint and the
         adjoint steps run over it, all shots in one batched call each; each
         shot accumulates its own coefficient-space gradient, summed in shot
-        order at the end."""
+        order at the end.  ``on_segment(a, b, hists)``, when given, is called
+        after the adjoint of steps [a, b) with each shot's history of those
+        steps, (b - a, fields, nx, nz) with the fields sxx, szz, sxz, u, w, syx,
+        syz, v of the active systems: the post-injection velocities there
+        times the sponge are the forward wavefield after each step."""
         if self._ck is None:
             raise RuntimeError("call forward first")
         dev = device()
@@ -333,6 +337,8 @@ class ElasticFWI:
             if k == len(self.edges) - 2:
                 self._tune("adjoint", "trtm_fwi_adjoint_batch", descs, a, b, reset_adjoint)
             self._call_batch("trtm_fwi_adjoint_batch", descs, a, b, self.shapes["adjoint"])
+            if on_segment is not None:  # the segment's history is still in place
+                on_segment(a, b, [h for _, _, h, _ in keep])
         self.shot_coefficient_gradients = [gq for _, gq, _, _ in keep]
         grads = accumulate_in_shot_order(None, self.shot_coefficient_gradients)
         self.coefficient_gradient = grads

@@ -16,18 +16,27 @@ def test_plan_raises_below_one_shot():
         budget.plan(400, 200, _shots(1, 100, 10), 1 << 20)
 
 
-def test_salt_plan_fits_and_scales_with_budget():
+@pytest.mark.parametrize("share", [True, False])
+def test_salt_plan_fits_and_scales_with_budget(share):
     """4000 x 2000 x 8000, 32 shots of 3000 receivers: checkpoints every
-    ~sqrt(nt) steps; the group grows with the budget and always fits it."""
+    ~sqrt(nt) steps; the group grows with the budget and always fits it.
+    Sharing the forward history, the RTM pass holds only its backward
+    propagator next to the FWI pass."""
     shots = _shots(32, 8000, 3000)
     cells = 4000 * 2000
     last = 0
     for gb in (40, 64, 128, 256):
-        p = budget.plan(4000, 2000, shots, gb * 1e9)
-        assert 80 <= p.L_fwi <= 100 and 120 <= p.L_rtm <= 160
-        assert p.shared_bytes + p.group * max(p.shot_bytes_fwi, p.shot_bytes_rtm) <= gb * 1e9
+        p = budget.plan(4000, 2000, shots, gb * 1e9, share_forward=share)
+        assert 80 <= p.L_fwi <= 100
+        if share:
+            assert p.L_rtm == p.L_fwi and p.shot_bytes_rtm < 20 * cells * 4 + 6 * 3000 * 8000 * 4
+            per = p.shot_bytes_fwi + p.shot_bytes_rtm
+        else:
+            assert 120 <= p.L_rtm <= 160
+            per = max(p.shot_bytes_fwi, p.shot_bytes_rtm)
+        assert p.shared_bytes + p.group * per <= gb * 1e9
         assert p.groups * p.group >= 32 and p.group >= last
         # FWI per shot: state, adjoint, gradients, workspace, checkpoints, history
         assert p.shot_bytes_fwi >= (28 + 5 * (8000 // p.L_fwi) + 5 * p.L_fwi) * cells * 4
         last = p.group
-    assert last == 8
+    assert last == (7 if share else 8)
