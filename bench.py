@@ -58,8 +58,7 @@ ALGO_BYTES = {
 # measured HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, copy-calibrated)
 MEASURED_TRAFFIC = {
     ("sh_layered_1200x600x4000", "blocked", (32, 128, 1024, 8)): 57.4e6,
-    ("sh_layered_1200x600x4000", "blocked", (24, 128, 768, 8)): 59.5e6,
-    ("sh_layered_1200x600x4000", "blocked", (24, 128, 1024, 8)): 59.5e6,
+    ("sh_layered_1200x600x4000", "blocked", (24, 128, 1024, 8)): 59.4e6,
 }
 
 
