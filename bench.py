@@ -101,16 +101,27 @@ def job_main(args):
         shots = _shot_specs(cfg, args.shots)
         mu, lam = cfg.moduli()
 
+        group = args.group or len(shots)
+
         def step():
-            b = BatchRTM(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, mu, lam, cfg.rho, cfg.fw, shots,
-                         isnap=args.isnap, imaging="checkpoint", kernel=args.kernel,
-                         tile_x=args.tile_x, tile_z=args.tile_z, block_steps=args.block_steps)
-            if b.run() != [0] * len(shots):
-                raise RuntimeError("non-finite wavefield")
-            b.stacked_image()
-            return b.tuned
+            # shots in groups of --group (each group one batch), images
+            # stacked in float64 in shot order
+            stack = None
+            for g0 in range(0, len(shots), group):
+                sub = shots[g0:g0 + group]
+                b = BatchRTM(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, mu, lam, cfg.rho, cfg.fw, sub,
+                             isnap=args.isnap, imaging="checkpoint", kernel=args.kernel,
+                             tile_x=args.tile_x, tile_z=args.tile_z, block_steps=args.block_steps)
+                if b.run() != [0] * len(sub):
+                    raise RuntimeError("non-finite wavefield")
+                img = b.stacked_image()
+                stack = img if stack is None else {c: stack[c].add_(img[c]) for c in stack}
+                tuned = b.tuned
+                del b, img
+            return tuned
         desc = ("batched checkpointed P-SV RTM, %d shots of %s (uneven record lengths and "
-                "receiver spreads), isnap=%d, stacked image" % (len(shots), cfg.name, args.isnap))
+                "receiver spreads) in batches of %d, isnap=%d, stacked image"
+                % (len(shots), cfg.name, group, args.isnap))
     elif args.workload == "fwi":
         from taichirtm_amd.fwi import ElasticFWI, l2_misfit_and_gradient
         cfg, start = configs.psv_fwi()
@@ -148,12 +159,18 @@ def job_main(args):
                 % (args.budget_gb, len(shots), true.name))
     else:
         raise SystemExit("unknown workload %r" % args.workload)
-    for _ in range(args.warmup):
+    for k in range(args.warmup):
+        t1 = time.perf_counter()
         tuned = step()
+        torch.cuda.synchronize()
+        print("warmup step %d done, %.1f s" % (k, time.perf_counter() - t1), file=sys.stderr,
+              flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.steps):
         tuned = step()
+        print("timed step %d launched, %.1f s since start" % (k, time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
     passes = 4.0 if args.workload == "salt" else 2.0  # salt: FWI and RTM, forward + backward each
@@ -280,6 +297,8 @@ def main():
     ap.add_argument("--workload", default="sh_layered")
     ap.add_argument("--kernel", default="tuned")
     ap.add_argument("--shots", type=int, default=8)
+    ap.add_argument("--group", type=int, default=0,
+                    help="psv_batch: shots per batch (0 = all in one batch)")
     ap.add_argument("--tile-x", type=int, default=0)
     ap.add_argument("--block-steps", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
