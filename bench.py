@@ -127,17 +127,22 @@ def job_main(args):
         cfg, start = configs.psv_fwi()
         shots = _shot_specs(cfg, args.shots)
         fwi_kernel = args.kernel if args.kernel in ("tuned", "blocked", "step") else "auto"
+        # the forward pass keeps the history of as many trailing checkpoint
+        # segments as --budget-gb holds; the adjoint recomputes the rest
         prob = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, shots,
-                          precision="float32", kernel=fwi_kernel)
+                          precision="float32", kernel=fwi_kernel,
+                          history_bytes=int(args.budget_gb * 1e9))
         t = lambda x: torch.as_tensor(np.asarray(x, np.float32), device="cuda")
         obs = [o.clone() for o in prob.forward(t(cfg.vp), t(cfg.vs), t(cfg.rho))]
         m0 = [t(x) for x in (start.vp, start.vs, start.rho)]
 
         def step():
             l2_misfit_and_gradient(prob, *m0, obs)
-            return dict(prob.shapes)
-        desc = ("P-SV FWI gradient (forward + checkpointed exact adjoint), %d shots of %s"
-                % (len(shots), cfg.name))
+            out = dict(prob.shapes)
+            out["kept_segments"] = [prob.kept_segments, len(prob.edges) - 1]
+            return out
+        desc = ("P-SV FWI gradient (forward + checkpointed exact adjoint), %d shots of %s, "
+                "forward history of up to %.0f GB kept" % (len(shots), cfg.name, args.budget_gb))
     elif args.workload == "salt":
         from taichirtm_amd import budget
         true, cfg, shot0 = budget.salt_survey(n_shots=32)
@@ -303,7 +308,8 @@ def main():
     ap.add_argument("--block-steps", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--isnap", type=int, default=1)
-    ap.add_argument("--budget-gb", type=float, default=128.0)
+    ap.add_argument("--budget-gb", type=float, default=128.0,
+                    help="salt: device memory of the job; fwi: forward history kept")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

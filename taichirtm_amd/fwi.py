@@ -6,10 +6,12 @@ back-propagator, not the adjoint of its forward step.  Here:
 
 - ``ElasticFWI.forward(vp, vs, rho)`` runs every shot with the forward scheme
   of forward_modeling.py:539-551 (same expressions) and keeps the full state
-  every ``checkpoint_every`` steps;
-- ``ElasticFWI.gradient(data_adjoint)`` walks the checkpoint segments last to
-  first, recomputes each segment's history (the stresses and velocities the
-  updates read), runs the adjoint steps over it and accumulates the gradient in
+  every ``checkpoint_every`` steps; with ``history_bytes`` it also writes the
+  history (the stresses and velocities the updates read) of as many trailing
+  segments as fit that many bytes, which then need no checkpoint;
+- ``ElasticFWI.gradient(data_adjoint)`` walks the segments last to first,
+  takes each segment's history from the forward pass or recomputes it from the
+  segment's checkpoint, runs the adjoint steps over it and accumulates the gradient in
   coefficient space (lam, mu, mxz, myx, myz, inv_rho, inv_rho_u, inv_rho_w),
   then maps it through the harmonic / arithmetic averages to (lam, mu, rho) and
   to (vp, vs, rho) with mu = rho vs^2, lam = rho (vp^2 - 2 vs^2)
@@ -91,7 +93,7 @@ class ElasticFWI:
 
     def __init__(self, nx, nz, dx, dz, fs, absorbing_frame, shots: Sequence[ShotSpec],
                  precision="float32", checkpoint_every=None, kernel="auto", tile_x=0, tile_z=0,
-                 block_steps=0):
+                 block_steps=0, history_bytes=0):
         if precision not in _REAL:
             raise ValueError("precision must be float32 or float64")
         if kernel not in _KERNELS:
@@ -150,7 +152,12 @@ class ElasticFWI:
             d.cell_map = _ptr(g["map"])
             _lib.call("trtm_fwi_prepare", d, stream_handle())
         self.checkpoint_every = checkpoint_every
+        # device memory for history written by the forward pass (the trailing
+        # segments that fit skip their recompute in ``gradient``)
+        self.history_bytes = int(history_bytes)
         self._ck = None
+        self._kept = []
+        self.kept_segments = 0
 
     # -- medium ---------------------------------------------------------------
 
@@ -255,13 +262,20 @@ class ElasticFWI:
         shape = (self.nx, self.nz)
         self.edges = list(range(0, nt_max, self.L)) + [nt_max]
         nseg = len(self.edges) - 1
+        S = len(self.geo)
+        # trailing segments whose history the forward pass keeps
+        seg_bytes = self.L * n_hist * self.nx * self.nz * self.rb * S
+        self.kept_segments = min(nseg, self.history_bytes // seg_bytes)
+        first_kept = nseg - self.kept_segments
+        self._kept = []
+        self._ck = None
         # every shot's state in one tensor, so a checkpoint is one copy
-        self.state_all = torch.zeros((len(self.geo), len(names)) + shape, dtype=self.dtype,
-                                     device=dev)
-        self.state = [{k: self.state_all[q, m] for m, k in enumerate(names)}
-                      for q in range(len(self.geo))]
-        self._ck = torch.empty((nseg,) + tuple(self.state_all.shape), dtype=self.dtype,
+        self.state_all = torch.zeros((S, len(names)) + shape, dtype=self.dtype, device=dev)
+        self.state = [{k: self.state_all[q, m] for m, k in enumerate(names)} for q in range(S)]
+        self._ck = torch.empty((first_kept,) + tuple(self.state_all.shape), dtype=self.dtype,
                                device=dev)
+        self._kept = [torch.empty((S, self.L, n_hist) + shape, dtype=self.dtype, device=dev)
+                      for _ in range(self.kept_segments)]
         recs, descs = [], []
         for g, st in zip(self.geo, self.state):
             d = self._desc(self.med, g)
@@ -272,12 +286,24 @@ class ElasticFWI:
             d.rec_u, d.rec_v, d.rec_w = (_ptr(rec[c]) for c in ("u", "v", "w"))
             recs.append(rec)
             descs.append(d)
-        self._tune("forward", "trtm_fwi_forward_batch", descs, self.edges[0], self.edges[1],
-                   self.state_all.zero_)
+        if first_kept > 0:
+            self._tune("forward", "trtm_fwi_forward_batch", descs, self.edges[0], self.edges[1],
+                       self.state_all.zero_)
         for k in range(nseg):
-            self._ck[k].copy_(self.state_all)
-            self._call_batch("trtm_fwi_forward_batch", descs, self.edges[k], self.edges[k + 1],
-                             self.shapes["forward"])
+            a, b = self.edges[k], self.edges[k + 1]
+            if k < first_kept:
+                self._ck[k].copy_(self.state_all)
+                self._call_batch("trtm_fwi_forward_batch", descs, a, b, self.shapes["forward"])
+                continue
+            hist = self._kept[k - first_kept]
+            for q, d in enumerate(descs):
+                d.hist, d.hist_base, d.hist_len = _ptr(hist[q]), a, b - a
+            if "history" not in self.tuned and self.kernel == "tuned":
+                start = self.state_all.clone()
+                self._tune("history", "trtm_fwi_forward_batch", descs, a, b,
+                           lambda: self.state_all.copy_(start))
+                del start
+            self._call_batch("trtm_fwi_forward_batch", descs, a, b, self.shapes["history"])
         return [torch.stack([rec[c].t() for c in ("u", "v", "w")]).contiguous() for rec in recs]
 
     # -- gradient ---------------------------------------------------------------
@@ -285,8 +311,9 @@ class ElasticFWI:
     def gradient(self, data_adjoint: List[torch.Tensor], on_segment=None):
         """Gradient of J with respect to (vp, vs, rho), float64 (nx, nz) each,
         for dJ/d(records) given per shot as (3, nr, nt).  Segments last to
-        first: every shot's history is recomputed from its checkpoint and the
-        adjoint steps run over it, all shots in one batched call each; each
+        first: every shot's history is taken from the forward pass or
+        recomputed from its checkpoint, and the adjoint steps run over it, all
+        shots in one batched call each; each
         shot accumulates its own coefficient-space gradient, summed in shot
         order at the end.  ``on_segment(a, b, hists)``, when given, is called
         after the adjoint of steps [a, b) with each shot's history of those
@@ -295,6 +322,9 @@ class ElasticFWI:
         times the sponge are the forward wavefield after each step."""
         if self._ck is None:
             raise RuntimeError("call forward first")
+        if self.kept_segments and not self._kept:
+            raise RuntimeError("the history kept by the forward pass was used by an earlier "
+                               "gradient; run forward again")
         dev = device()
         names = self._fields()
         shape = (self.nx, self.nz)
@@ -302,12 +332,17 @@ class ElasticFWI:
         n_hist = (5 if self.systems & _lib.PSV else 0) + (3 if self.systems & _lib.SH else 0)
         keep, descs = [], []
         S = len(self.geo)
+        nseg = len(self.edges) - 1
+        first_kept = nseg - self.kept_segments
         adj_all = torch.zeros((S, len(names)) + shape, dtype=self.dtype, device=dev)
         grad_all = torch.zeros((S, len(gnames)) + shape, dtype=self.dtype, device=dev)
+        # recompute buffer, for the segments the forward pass did not keep
+        rebuf = (torch.empty((S, self.L, n_hist) + shape, dtype=self.dtype, device=dev)
+                 if first_kept > 0 else None)
         for q, (g, st, da) in enumerate(zip(self.geo, self.state, data_adjoint)):
             adj = {"p" + k: adj_all[q, m] for m, k in enumerate(names)}
             grads = {k: grad_all[q, m] for m, k in enumerate(gnames)}
-            hist = torch.empty((self.L, n_hist) + shape, dtype=self.dtype, device=dev)
+            hist = rebuf[q] if rebuf is not None else None
             da = torch.as_tensor(da, device=dev).to(self.dtype)
             res = [da[c].t().contiguous() for c in range(3)]  # (nt, nr) each
             d = self._desc(self.med, g)
@@ -317,29 +352,36 @@ class ElasticFWI:
             for k in gnames:
                 setattr(d, k, _ptr(grads[k]))
             d.res_u, d.res_v, d.res_w = (_ptr(r) for r in res)
-            d.hist = _ptr(hist)
-            keep.append((adj, grads, hist, res))
+            keep.append((adj, grads, res))
             descs.append(d)
-        for k in range(len(self.edges) - 2, -1, -1):
+        for k in range(nseg - 1, -1, -1):
             a, b = self.edges[k], self.edges[k + 1]
-            # a shot whose record ends before a skips these steps; its state is unused
-            self.state_all.copy_(self._ck[k])
-            for d in descs:
-                d.hist_base, d.hist_len = a, b - a
-            ck = self._ck[k]
-            self._tune("history", "trtm_fwi_forward_batch", descs, a, b,
-                       lambda: self.state_all.copy_(ck))
-            self._call_batch("trtm_fwi_forward_batch", descs, a, b, self.shapes["history"])
+            if k >= first_kept:
+                hists = list(self._kept[k - first_kept])
+            else:
+                hists = list(rebuf)
+            for d, h in zip(descs, hists):
+                d.hist, d.hist_base, d.hist_len = _ptr(h), a, b - a
+            if k < first_kept:
+                # a shot whose record ends before a skips these steps; its state is unused
+                self.state_all.copy_(self._ck[k])
+                ck = self._ck[k]
+                self._tune("history", "trtm_fwi_forward_batch", descs, a, b,
+                           lambda: self.state_all.copy_(ck))
+                self._call_batch("trtm_fwi_forward_batch", descs, a, b, self.shapes["history"])
 
             def reset_adjoint():
                 adj_all.zero_()
                 grad_all.zero_()
-            if k == len(self.edges) - 2:
+            if k == nseg - 1:
                 self._tune("adjoint", "trtm_fwi_adjoint_batch", descs, a, b, reset_adjoint)
             self._call_batch("trtm_fwi_adjoint_batch", descs, a, b, self.shapes["adjoint"])
             if on_segment is not None:  # the segment's history is still in place
-                on_segment(a, b, [h for _, _, h, _ in keep])
-        self.shot_coefficient_gradients = [gq for _, gq, _, _ in keep]
+                on_segment(a, b, hists)
+            if k >= first_kept:
+                self._kept[k - first_kept] = None  # used: free it
+        self._kept = []
+        self.shot_coefficient_gradients = [gq for _, gq, _ in keep]
         grads = accumulate_in_shot_order(None, self.shot_coefficient_gradients)
         self.coefficient_gradient = grads
         return self.chain(grads)
@@ -348,6 +390,7 @@ class ElasticFWI:
         """Drop the checkpoints, state, workspaces and per-shot gradients;
         the medium stays for ``chain``."""
         self._ck = self.state_all = self.state = None
+        self._kept = []
         self.shot_coefficient_gradients = None
         for g in self.geo:
             g["ws"] = None

@@ -231,6 +231,36 @@ def test_tuned_kernels_bitwise_equal_step_kernels():
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("kernel", ["blocked", "tuned", "step"])
+def test_history_kept_by_forward_bitwise(kernel):
+    """history_bytes: the trailing segments' history written by the forward
+    pass (no checkpoint, no recompute) gives the recomputing run's records and
+    gradients bit for bit, with every segment kept and with some (3 shots of
+    uneven record length, the last segment shorter than the interval)."""
+    _case("FWI kept history %s" % kernel)
+    grid, shots, m0, mt = problem("psv", nt=150, n_shots=3)
+    shots = _uneven(shots, (150, 111, 137), ((0, 52), (5, 30), (20, 45)))
+    shots[2] = ShotSpec(np.array([[45, 2]], np.int32), shots[2].wavelets, shots[2].rec_loc)
+    nx, nz = grid[0], grid[1]
+    ref = _records_and_gradient(grid, shots, m0, mt, kernel=kernel, checkpoint_every=40)
+    seg = 40 * 5 * nx * nz * 4 * 3
+    for kept, nbytes in ((4, 10 * seg), (2, 2 * seg + seg // 2)):
+        p = ElasticFWI(*grid, shots, precision="float32", kernel=kernel, checkpoint_every=40,
+                       history_bytes=nbytes)
+        obs = [o.clone() for o in p.forward(*[_t(x, torch.float32) for x in mt])]
+        recs = [r.clone() for r in p.forward(*[_t(x, torch.float32) for x in m0])]
+        assert p.kept_segments == kept
+        g = p.gradient([r - o for r, o in zip(recs, obs)])
+        for a, b in zip(recs, ref[0]):
+            assert torch.equal(a, b)
+        for a, b in zip(g, ref[1]):
+            assert torch.equal(a, b)
+        for key in ref[2]:
+            assert torch.equal(p.coefficient_gradient[key], ref[2][key]), key
+        with pytest.raises(RuntimeError):
+            p.gradient([r - o for r, o in zip(recs, obs)])
+
+
 @pytest.mark.parametrize("system", ["psv", "sh"])
 def test_gradient_correlates_with_rtm_image(system):
     """The exact-adjoint gradient against the reference's RTM: back-propagating
