@@ -111,17 +111,20 @@ def job_main(args):
                 sub = shots[g0:g0 + group]
                 b = BatchRTM(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, mu, lam, cfg.rho, cfg.fw, sub,
                              isnap=args.isnap, imaging="checkpoint", kernel=args.kernel,
-                             tile_x=args.tile_x, tile_z=args.tile_z, block_steps=args.block_steps)
+                             tile_x=args.tile_x, tile_z=args.tile_z, block_steps=args.block_steps,
+                             snapshot_bytes=int(args.budget_gb * 1e9))
                 if b.run() != [0] * len(sub):
                     raise RuntimeError("non-finite wavefield")
                 img = b.stacked_image()
                 stack = img if stack is None else {c: stack[c].add_(img[c]) for c in stack}
-                tuned = b.tuned
+                tuned = dict(b.tuned)
+                tuned["kept_segments"] = [len(b.edges) - 1 - b.first_kept, len(b.edges) - 1]
                 del b, img
             return tuned
         desc = ("batched checkpointed P-SV RTM, %d shots of %s (uneven record lengths and "
-                "receiver spreads) in batches of %d, isnap=%d, stacked image"
-                % (len(shots), cfg.name, group, args.isnap))
+                "receiver spreads) in batches of %d, isnap=%d, stacked image, segment "
+                "wavefields of up to %.0f GB kept from the forward pass"
+                % (len(shots), cfg.name, group, args.isnap, args.budget_gb))
     elif args.workload == "fwi":
         from taichirtm_amd.fwi import ElasticFWI, l2_misfit_and_gradient
         cfg, start = configs.psv_fwi()
@@ -309,7 +312,8 @@ def main():
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--isnap", type=int, default=1)
     ap.add_argument("--budget-gb", type=float, default=128.0,
-                    help="salt: device memory of the job; fwi: forward history kept")
+                    help="salt: device memory of the job; fwi: forward history kept; "
+                         "psv_batch: segment wavefields kept")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -19,7 +19,10 @@ Two imaging modes give the same image bit for bit:
   the step kernel.  Memory: nt/L states + L/isnap wavefields per shot, with L
   from a memory budget (about sqrt(nt) by default); cost: one extra forward
   sweep.  Recomputation is deterministic, so the wavefields, and the image,
-  are the snapshot mode's exactly.
+  are the snapshot mode's exactly.  With ``snapshot_bytes`` the forward pass
+  writes the segment buffers of as many trailing segments as fit that many
+  bytes (only shots still running in a segment count); those segments need
+  neither checkpoint nor recompute.
 """
 from __future__ import annotations
 
@@ -89,13 +92,15 @@ class BatchRTM:
     stability_check_interval, the temporally blocked kernel ("blocked": LDS
     tiles, "wave": one wavefront per tile with the state in registers, or
     "tuned": family and shape chosen by the autotuner on this batch) and its
-    tile_x / tile_z / block_steps (0 = library default).
+    tile_x / tile_z / block_steps (0 = library default), and snapshot_bytes
+    (checkpoint mode: device memory for segment wavefields kept from the
+    forward pass).
     """
 
     def __init__(self, nx, nz, dx, dz, fs, mu_field, lam_field, rho_field, absorbing_frame,
                  shots: List[ShotSpec], isnap=1, imaging="snapshot", checkpoint_every=None,
                  stability_check_interval=100, surface_matrix_field=None, tile_x=0, tile_z=0,
-                 block_steps=0, kernel="blocked"):
+                 block_steps=0, kernel="blocked", snapshot_bytes=0):
         if imaging not in ("snapshot", "checkpoint"):
             raise ValueError("imaging must be 'snapshot' or 'checkpoint'")
         if kernel not in ("blocked", "wave", "tuned"):
@@ -111,6 +116,11 @@ class BatchRTM:
         self.checkpoint_every = checkpoint_every
         self.sci = int(stability_check_interval)
         self.block = dict(tile_x=tile_x, tile_z=tile_z, block_steps=block_steps, kernel=kernel)
+        # checkpoint mode: device memory for segment snapshots written by the
+        # forward pass (the trailing segments that fit need no recompute)
+        self.snapshot_bytes = int(snapshot_bytes)
+        self.kept = {}
+        self.first_kept = None
         self.tuned = {}
         self.codes = []
         self.fw: List[ForwardModeling] = []
@@ -216,21 +226,57 @@ class BatchRTM:
         if snapshot:
             self._batch("trtm_forward_steps_batch", descs, 0, nt_max, 1)
             return
-        # checkpoint mode: keep the state at every multiple of L
+        # checkpoint mode: keep the state at every multiple of L, or, for the
+        # trailing segments that fit snapshot_bytes, the segment's snapshots
         L = self.L
         self.edges = list(range(0, nt_max, L)) + [nt_max]
+        nseg = len(self.edges) - 1
+        nslot = L // self.isnap + 1
+        comps = [(c, bit) for c, bit in (("u", _lib.PSV), ("v", _lib.SH), ("w", _lib.PSV))
+                 if self.systems & bit]
+        seg_shot_bytes = nslot * len(comps) * self.nx * self.nz * 4
+        left, first_kept = self.snapshot_bytes, nseg
+        for k in range(nseg - 1, -1, -1):
+            need = seg_shot_bytes * sum(1 for f in self.fw if self.edges[k] < f.nt)
+            if need > left:
+                break
+            left -= need
+            first_kept = k
+        self.first_kept = first_kept
+        self.kept = {}
         self.ckpt = []
         for f in self.fw:
-            ncp = (f.nt + L - 1) // L
+            ncp = min((f.nt + L - 1) // L, first_kept)
             self.ckpt.append(torch.empty((ncp, len(names), self.nx, self.nz), dtype=torch.float32,
                                          device=dev))
-        for k in range(len(self.edges) - 1):
+        spare = None  # snapshot target of shots whose record ended before the segment
+        for k in range(nseg):
             a, b = self.edges[k], self.edges[k + 1]
-            for q, f in enumerate(self.fw):
+            if k < first_kept:
+                for q, f in enumerate(self.fw):
+                    if a < f.nt:
+                        for m, name in enumerate(names):
+                            self.ckpt[q][k, m].copy_(getattr(f, name).tensor)
+                self._batch("trtm_forward_steps_batch", descs, a, b, 0)
+                continue
+            bufs = []
+            for f, d in zip(self.fw, descs):
                 if a < f.nt:
-                    for m, name in enumerate(names):
-                        self.ckpt[q][k, m].copy_(getattr(f, name).tensor)
-            self._batch("trtm_forward_steps_batch", descs, a, b, 0)
+                    buf = {c: torch.empty((nslot, self.nx, self.nz), dtype=torch.float32,
+                                          device=dev) for c, _ in comps}
+                else:
+                    if spare is None:
+                        spare = {c: torch.empty((nslot, self.nx, self.nz), dtype=torch.float32,
+                                                device=dev) for c, _ in comps}
+                    buf = spare
+                bufs.append(buf)
+                d.num_snaps, d.snap_base = nslot, a // self.isnap - 1
+                d.snap_u, d.snap_v, d.snap_w = (_ptr(buf.get(c)) for c in ("u", "v", "w"))
+            self.kept[k] = bufs
+            self._batch("trtm_forward_steps_batch", descs, a, b, 1)
+        for d in descs:
+            d.num_snaps, d.snap_base = 0, 0
+            d.snap_u = d.snap_v = d.snap_w = None
 
     # -- backward -----------------------------------------------------------
 
@@ -292,26 +338,33 @@ class BatchRTM:
         L = self.L
         nslot = L // self.isnap + 1
         bufs = []
-        for f in self.fw:
-            bufs.append({c: (torch.zeros((nslot, self.nx, self.nz), dtype=torch.float32, device=dev)
-                             if f.systems & bit else None)
-                         for c, bit in (("u", _lib.PSV), ("v", _lib.SH), ("w", _lib.PSV))})
+        if self.first_kept > 0:  # recompute buffers
+            for f in self.fw:
+                bufs.append({c: (torch.zeros((nslot, self.nx, self.nz), dtype=torch.float32,
+                                             device=dev) if f.systems & bit else None)
+                             for c, bit in (("u", _lib.PSV), ("v", _lib.SH), ("w", _lib.PSV))})
+        else:
+            bufs = [{} for _ in self.fw]
         fdescs = []
         for f, d0, buf in zip(self.fw, self._fw_descs, bufs):
             d = _lib.StepDesc.from_buffer_copy(d0)
             d.num_snaps = nslot
-            d.snap_u, d.snap_v, d.snap_w = (_ptr(buf[c]) for c in ("u", "v", "w"))
+            d.snap_u, d.snap_v, d.snap_w = (_ptr(buf.get(c)) for c in ("u", "v", "w"))
             d.rec_u = d.rec_v = d.rec_w = None
             fdescs.append(d)
         for k in range(len(self.edges) - 2, -1, -1):
             a, b = self.edges[k], self.edges[k + 1]
             base = a // self.isnap - 1
-            for q, f in enumerate(self.fw):
-                if a < f.nt:
-                    for m, name in enumerate(self._state_names):
-                        getattr(f, name).tensor.copy_(self.ckpt[q][k, m])
-                fdescs[q].snap_base = base
-            self._batch("trtm_forward_steps_batch", fdescs, a, b, 1)
+            if k >= self.first_kept:
+                seg = self.kept.pop(k)  # written by the forward pass
+            else:
+                seg = bufs
+                for q, f in enumerate(self.fw):
+                    if a < f.nt:
+                        for m, name in enumerate(self._state_names):
+                            getattr(f, name).tensor.copy_(self.ckpt[q][k, m])
+                    fdescs[q].snap_base = base
+                self._batch("trtm_forward_steps_batch", fdescs, a, b, 1)
             isns = []
             for q, f in enumerate(self.fw):
                 isn = np.full(nslot, -1, np.int32)
@@ -323,7 +376,7 @@ class BatchRTM:
                 descs[q].snap_base = base
                 descs[q].num_snaps = nslot
                 descs[q].snap_u, descs[q].snap_v, descs[q].snap_w = (
-                    _ptr(bufs[q][c]) for c in ("u", "v", "w"))
+                    _ptr(seg[q].get(c)) for c in ("u", "v", "w"))
                 descs[q].isnaps_host = isn.ctypes.data_as(ctypes.c_void_p)
                 descs[q].step_offset = f.nt
             if k == len(self.edges) - 2:

@@ -76,3 +76,36 @@ def test_checkpoint_interval_model():
     assert checkpoint_interval(100, 3, 1, isnap=4, align=4) % 4 == 0
     with pytest.raises(MemoryError):
         checkpoint_interval(8000, 5, 2, budget_bytes=1e6, cells=8_000_000)
+
+
+@pytest.mark.parametrize("kernel", ["blocked", "wave"])
+@pytest.mark.parametrize("system,isnap", [("psv", 1), ("sh", 3), ("both", 1)])
+def test_segments_kept_by_forward_bitwise(system, isnap, kernel):
+    """snapshot_bytes: the trailing segments' wavefields written by the forward
+    pass (no checkpoint, no recompute) give the recomputing run's images and
+    seismograms bit for bit, with every segment kept and with some."""
+    _case("batch kept segments %s isnap=%d kernel=%s" % (system, isnap, kernel))
+    cfg, mu, lam, shots = survey(system)
+    L = 12 if isnap == 3 else 16
+
+    def run(nbytes):
+        b = BatchRTM(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, mu, lam, cfg.rho, cfg.fw, shots,
+                     isnap=isnap, imaging="checkpoint", kernel=kernel, checkpoint_every=L,
+                     snapshot_bytes=nbytes)
+        assert b.run() == [0] * len(shots)
+        return b, b.images(), b.seismograms()
+
+    ref = run(0)
+    assert ref[0].first_kept == len(ref[0].edges) - 1
+    ncomp = {"psv": 2, "sh": 1, "both": 3}[system]
+    seg = (L // isnap + 1) * ncomp * cfg.nx * cfg.nz * 4
+    for nbytes, full in ((seg * len(shots) * 20, True), (seg * 4, False)):
+        b, imgs, seis = run(nbytes)
+        nseg = len(b.edges) - 1
+        print("[kept] %d bytes: first kept segment %d of %d" % (nbytes, b.first_kept, nseg),
+              flush=True)
+        assert (b.first_kept == 0) if full else (0 < b.first_kept < nseg)
+        for q in range(len(shots)):
+            for c in range(3):
+                assert same(imgs[q][c], ref[1][q][c]), (q, c)
+                assert same(seis[q][c], ref[2][q][c]), (q, c)
