@@ -59,6 +59,8 @@ ALGO_BYTES = {
 MEASURED_TRAFFIC = {
     ("sh_layered_1200x600x4000", "blocked", (32, 128, 1024, 8)): 57.4e6,
     ("sh_layered_1200x600x4000", "blocked", (24, 128, 1024, 8)): 59.4e6,
+    ("sh_layered_1200x600x4000", "blocked", (32, 48, 512, 8)): 59.8e6,
+    ("sh_layered_1200x600x4000", "blocked", (48, 64, 1024, 8)): 60.0e6,
 }
 
 

@@ -537,9 +537,11 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
 // configurations compiled per wave system
 static const BlockCfg kCfgSH[] = {{16, 64, 256, 4},   {16, 64, 256, 8},    {16, 128, 512, 4},
                                   {16, 128, 512, 8},  {32, 128, 1024, 4},  {32, 128, 1024, 8},
-                                  {24, 128, 1024, 8}, {32, 96, 1024, 8}};
+                                  {24, 128, 1024, 8}, {32, 96, 1024, 8},   {48, 64, 1024, 8},
+                                  {32, 48, 512, 8},   {24, 64, 512, 8}};
 static const BlockCfg kCfgPSV[] = {{16, 64, 256, 2},  {16, 64, 256, 4},    {16, 128, 512, 4},
-                                   {16, 128, 512, 8}, {32, 128, 1024, 4}};
+                                   {16, 128, 512, 8}, {32, 128, 1024, 4},  {48, 64, 1024, 4},
+                                   {32, 64, 512, 4}};
 static const BlockCfg kCfgBoth[] = {{16, 64, 256, 2}, {16, 64, 256, 4}, {16, 128, 512, 4}};
 
 static void cfg_list(int systems, const BlockCfg** list, int* n) {
@@ -558,7 +560,7 @@ static void cfg_list(int systems, const BlockCfg** list, int* n) {
 // defaults: the fastest shapes measured on MI355X for the 1200 x 600 SH and
 // 200 x 200 / 2000 x 1000 P-SV workloads
 static BlockCfg default_cfg(int systems) {
-    if (systems == TRTM_SH) return {24, 128, 1024, 8};
+    if (systems == TRTM_SH) return {32, 48, 512, 8};
     if (systems == TRTM_PSV) return {32, 128, 1024, 4};
     return {16, 128, 512, 4};
 }
@@ -804,11 +806,13 @@ static bool launch_sys(BlockCfg c, const KArgs& m, const ShotDev* shots, int n_s
         TRTM_CFG(16, 64, 256, 2) TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 128, 512, 4)
     } else if constexpr (PSV) {
         TRTM_CFG(16, 64, 256, 2) TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 128, 512, 4)
-        TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4)
+        TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4) TRTM_CFG(48, 64, 1024, 4)
+        TRTM_CFG(32, 64, 512, 4)
     } else {
         TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 64, 256, 8) TRTM_CFG(16, 128, 512, 4)
         TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4) TRTM_CFG(32, 128, 1024, 8)
-        TRTM_CFG(24, 128, 1024, 8) TRTM_CFG(32, 96, 1024, 8)
+        TRTM_CFG(24, 128, 1024, 8) TRTM_CFG(32, 96, 1024, 8) TRTM_CFG(48, 64, 1024, 8)
+        TRTM_CFG(32, 48, 512, 8) TRTM_CFG(24, 64, 512, 8)
     }
 #undef TRTM_CFG
     return false;
