@@ -723,7 +723,10 @@ static bool fwi_cfg_fits(int systems, const BlockCfg& c) {
 
 // compiled shapes (tile rows, tile columns, threads, steps per launch)
 static const BlockCfg kFwiCfg[] = {{24, 64, 768, 4},  {32, 64, 1024, 4}, {16, 64, 512, 4},
-                                   {16, 64, 512, 8},  {32, 64, 1024, 2}, {16, 64, 256, 2}};
+                                   {16, 64, 512, 8},  {32, 64, 1024, 2}, {16, 64, 256, 2},
+                                   {16, 48, 512, 4},  {24, 32, 512, 4}};
+// (the 512-thread 16 x 48 and 24 x 32 shapes keep the P-SV adjoint within half
+// the LDS of a CU, so two workgroups share one)
 // defaults, the fastest measured on MI355X for 8 P-SV shots of 600 x 300: the
 // forward passes {24, 64, 768, 4}, the adjoint {32, 64, 1024, 4}; the first
 // listed shape that fits otherwise
@@ -832,6 +835,8 @@ static int launch_cfg(const BlockCfg& c, bool adjoint, bool hist, const S& src, 
     TRTM_FB(16, 64, 512, 8)
     TRTM_FB(32, 64, 1024, 2)
     TRTM_FB(24, 64, 768, 4)
+    TRTM_FB(16, 48, 512, 4)
+    TRTM_FB(24, 32, 512, 4)
 #undef TRTM_FB
     return fail(TRTM_ERR_UNSUPPORTED, "blocked FWI shape %dx%d/%d/%d not compiled", c.tx, c.tz,
                 c.nt, c.k);
