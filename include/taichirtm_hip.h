@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TRTM_ABI_VERSION 4
+#define TRTM_ABI_VERSION 5
 
 #define TRTM_OK 0
 #define TRTM_ERR_ARG (-1)         /* invalid argument or shape */
@@ -222,6 +222,31 @@ int trtm_forward_steps_batch(const trtm_step_desc *descs, int32_t n, void *batch
 int trtm_backward_steps_batch(const trtm_step_desc *descs, int32_t n, void *batch_workspace,
                               int64_t batch_workspace_bytes, int32_t it_begin, int32_t it_end,
                               void *stream);
+
+/*
+ * Captured time stepping: the launches that trtm_forward_steps_batch
+ * (forward != 0, with save) or trtm_backward_steps_batch (forward == 0) would
+ * issue for [it_begin, it_end), recorded once into a HIP graph.
+ * trtm_plan_create validates the call, builds the per-tile lists and shot
+ * tables in the workspaces on `stream` (and waits for it), records the
+ * launches on a stream of its own and instantiates the graph; it runs no
+ * step.  Each trtm_plan_launch replays the steps on `stream`, from whatever
+ * state the buffers then hold, with the same results as the batch call.  The
+ * plan keeps every pointer of the descriptors: the caller keeps those buffers
+ * alive and leaves both workspaces untouched (they hold the lists and tables)
+ * until trtm_plan_destroy; buffer contents (state, medium, wavelets, data,
+ * image, check flags) may change between launches.  Blocked and wave kernels
+ * (kernel TRTM_KERNEL_AUTO, _BLOCKED or _WAVE).  No reference counterpart: it
+ * replaces the per-step kernel calls of the loops in ForwardModeling.run
+ * (forward_modeling.py:539-572) and BackwardModeling.run_calc
+ * (backward_modeling.py:539-576) with one graph launch per range.
+ */
+typedef struct trtm_plan trtm_plan;
+int trtm_plan_create(const trtm_step_desc *descs, int32_t n, void *batch_workspace,
+                     int64_t batch_workspace_bytes, int32_t it_begin, int32_t it_end,
+                     int32_t forward, int32_t save, void *stream, trtm_plan **plan);
+int trtm_plan_launch(trtm_plan *plan, void *stream);
+int trtm_plan_destroy(trtm_plan *plan);
 
 /* ------------------------------------------------------------------------
  * FWI gradient: exact discrete adjoint of the forward step.

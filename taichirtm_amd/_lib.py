@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtaichirtm_hip.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 OK = 0
 ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3
 PSV, SH = 1, 2
@@ -113,6 +113,10 @@ SIGNATURES = {
     "trtm_fwi_adjoint": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _p]),
     "trtm_fwi_chain_gradient": (ctypes.c_int, [_i32, _i32] + [_p] * 15 + [_p]),
     "trtm_transpose": (ctypes.c_int, [_i32, _i32, _p, _p, _p]),
+    "trtm_plan_create": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _p, ctypes.c_int64, _i32,
+                                        _i32, _i32, _i32, _p, ctypes.POINTER(_p)]),
+    "trtm_plan_launch": (ctypes.c_int, [_p, _p]),
+    "trtm_plan_destroy": (ctypes.c_int, [_p]),
 }
 
 _LIB = None

@@ -99,6 +99,61 @@ class BackwardModeling(Propagator):
         ``method`` is accepted and ignored, as in the reference (the body never
         reads it).  Returns 0, or 4 / 5 / 6 when u / v / w became non-finite.
         """
+        d, flags, ws, sci, sysmask, imaging, snaps, isnaps = self._prepare_calc(  # noqa: F841
+            import_fwdata_u, import_fwdata_v, import_fwdata_w, isnaps_field, num_snaps,
+            isnap_interval, stability_check_interval)
+        nx, nz, nt = self.nx, self.nz, self.nt
+        num_snaps, isnap = int(num_snaps), int(isnap_interval)
+        if imaging and sysmask != self.systems:
+            # a propagating system without forward data: its image stays 0,
+            # run it without imaging in a second descriptor
+            return self._run_split(d, sysmask, flags, sci, display_callback, ws)
+
+        image_steps = [nt - 1 - t for t in range(nt - 1, 0, -1)
+                       if imaging and t % isnap == 0 and 0 <= t // isnap - 1 < num_snaps
+                       and isnaps[t // isnap - 1] == t]
+        stops = image_steps if display_callback is not None else ()
+        s = stream_handle()
+        code = 0
+        for a, b in self._chunks(nt, sci, stops):
+            _lib.call("trtm_backward_steps", d, a, b, s)
+            host_flags = flags.cpu().numpy()
+            code, _ = self._first_failure(host_flags, sci, a, b - 1)
+            if code:
+                break
+            if display_callback is not None and (b - 1) in stops:
+                display_callback(self.u.to_numpy(), self.v.to_numpy(), self.w.to_numpy(),
+                                 nt - b, nx, nz, self.dx, self.dz)
+            code, _ = self._first_failure(host_flags, sci, b - 1, b)
+            if code:
+                break
+        if code:
+            return code
+        print("Backward modeling completed")
+        return 0
+
+    def capture_calc(self, import_fwdata_u, import_fwdata_v, import_fwdata_w, isnaps_field,
+                     num_snaps: int, isnap_interval: int, stability_check_interval: int = 100):
+        """run_calc's whole run as a ``CapturedCalc``: recorded once into a HIP
+        graph (blocked or wave kernels) reading the given snapshots, replayed
+        by ``launch()``; no display callback and no early stop (``result()``
+        reads the check flags afterwards, codes 4-6).  Each launch continues
+        from the current state and adds to the current image: call
+        ``reset_state()`` first for a fresh run."""
+        from .plan import StepPlan
+        d, flags, ws, sci, sysmask, imaging, snaps, isnaps = self._prepare_calc(
+            import_fwdata_u, import_fwdata_v, import_fwdata_w, isnaps_field, num_snaps,
+            isnap_interval, stability_check_interval)
+        if imaging and sysmask != self.systems:
+            raise NotImplementedError("capture with partially imaged wave systems")
+        plan = StepPlan([d], 0, self.nt, forward=False,
+                        keep=[ws, flags] + [t for t in snaps.values() if t is not None])
+        return CapturedCalc(self, plan, flags, sci)
+
+    def _prepare_calc(self, import_fwdata_u, import_fwdata_v, import_fwdata_w, isnaps_field,
+                      num_snaps, isnap_interval, stability_check_interval):
+        """Snapshot tensors, descriptor, check flags and workspace of a
+        backward run; the autotuner's choice when kernel="tuned"."""
         sci = int(stability_check_interval)
         if sci < 1:
             raise ValueError("stability_check_interval must be >= 1")
@@ -147,33 +202,8 @@ class BackwardModeling(Propagator):
         self._autotune(d, "backward",
                        lambda dd, n: _lib.call("trtm_backward_steps", dd, 0, n, stream_handle()))
         flags.zero_()
-        if imaging and sysmask != self.systems:
-            # a propagating system without forward data: its image stays 0,
-            # run it without imaging in a second descriptor
-            return self._run_split(d, sysmask, flags, sci, display_callback, ws)
-
-        image_steps = [nt - 1 - t for t in range(nt - 1, 0, -1)
-                       if imaging and t % isnap == 0 and 0 <= t // isnap - 1 < num_snaps
-                       and isnaps[t // isnap - 1] == t]
-        stops = image_steps if display_callback is not None else ()
-        s = stream_handle()
-        code = 0
-        for a, b in self._chunks(nt, sci, stops):
-            _lib.call("trtm_backward_steps", d, a, b, s)
-            host_flags = flags.cpu().numpy()
-            code, _ = self._first_failure(host_flags, sci, a, b - 1)
-            if code:
-                break
-            if display_callback is not None and (b - 1) in stops:
-                display_callback(self.u.to_numpy(), self.v.to_numpy(), self.w.to_numpy(),
-                                 nt - b, nx, nz, self.dx, self.dz)
-            code, _ = self._first_failure(host_flags, sci, b - 1, b)
-            if code:
-                break
-        if code:
-            return code
-        print("Backward modeling completed")
-        return 0
+        sysmask = sysmask if imaging else 0
+        return d, flags, ws, sci, sysmask, imaging, snaps, isnaps
 
     def _run_split(self, d, sysmask, flags, sci, display_callback, ws):
         """Imaging for the systems in sysmask, plain propagation for the rest."""
@@ -215,3 +245,26 @@ class BackwardModeling(Propagator):
         self._syn = {}
         self._coef = {}
         self._inj = []
+
+
+class CapturedCalc:
+    """A backward run captured by ``BackwardModeling.capture_calc``."""
+
+    def __init__(self, prop, plan, flags, sci):
+        self.prop, self.plan, self.flags, self.sci = prop, plan, flags, sci
+
+    def launch(self):
+        """Clear the check flags and replay the steps on the current stream."""
+        self.flags.zero_()
+        self.plan.launch()
+
+    def result(self) -> int:
+        """Wait for the last launch; 0, or the code of the first failing check."""
+        code, _ = self.prop._first_failure(self.flags.cpu().numpy(), self.sci, 0, self.prop.nt)
+        if code:
+            return code
+        print("Backward modeling completed")
+        return 0
+
+    def close(self):
+        self.plan.close()

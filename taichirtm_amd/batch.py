@@ -92,15 +92,16 @@ class BatchRTM:
     stability_check_interval, the temporally blocked kernel ("blocked": LDS
     tiles, "wave": one wavefront per tile with the state in registers, or
     "tuned": family and shape chosen by the autotuner on this batch) and its
-    tile_x / tile_z / block_steps (0 = library default), and snapshot_bytes
+    tile_x / tile_z / block_steps (0 = library default), snapshot_bytes
     (checkpoint mode: device memory for segment wavefields kept from the
-    forward pass).
+    forward pass), and graph (every batched step range recorded into a HIP
+    graph and launched as one; same results).
     """
 
     def __init__(self, nx, nz, dx, dz, fs, mu_field, lam_field, rho_field, absorbing_frame,
                  shots: List[ShotSpec], isnap=1, imaging="snapshot", checkpoint_every=None,
                  stability_check_interval=100, surface_matrix_field=None, tile_x=0, tile_z=0,
-                 block_steps=0, kernel="blocked", snapshot_bytes=0):
+                 block_steps=0, kernel="blocked", snapshot_bytes=0, graph=False):
         if imaging not in ("snapshot", "checkpoint"):
             raise ValueError("imaging must be 'snapshot' or 'checkpoint'")
         if kernel not in ("blocked", "wave", "tuned"):
@@ -121,6 +122,10 @@ class BatchRTM:
         self.snapshot_bytes = int(snapshot_bytes)
         self.kept = {}
         self.first_kept = None
+        # every batched step range recorded into a HIP graph and launched as
+        # one (taichirtm_amd.plan.StepPlan), released at the end of run()
+        self.graph = bool(graph)
+        self._plans = []
         self.tuned = {}
         self.codes = []
         self.fw: List[ForwardModeling] = []
@@ -160,6 +165,13 @@ class BatchRTM:
         reset()
 
     def _batch(self, fn, descs, it_begin, it_end, *extra):
+        if self.graph:
+            from .plan import StepPlan
+            fwd = fn == "trtm_forward_steps_batch"
+            plan = StepPlan(descs, it_begin, it_end, forward=fwd, save=fwd and bool(extra[0]))
+            plan.launch()
+            self._plans.append(plan)
+            return plan._bws
         arr = _desc_array(descs)
         n = len(descs)
         nbytes = int(_lib.load().trtm_batch_workspace_bytes(n))
@@ -499,6 +511,9 @@ class BatchRTM:
         (0, 1-3 forward, 4-6 backward; the first failing check step)."""
         self._forward()
         self._backward()
+        for plan in self._plans:
+            plan.close()
+        self._plans = []
         self.codes = []
         for q, s in enumerate(self.shots):
             code = 0

@@ -29,6 +29,9 @@ int block_config_list(int systems, int kernel, int32_t* out, int max_n);
 ShotDev* block_shot_slot(const trtm_step_desc* d);
 int run_block(const trtm_step_desc* descs, int n, bool fwd, bool save, int32_t it_begin,
               int32_t it_end, ShotDev* shots_dev, const KArgs& m, hipStream_t s);
+int block_capture(const trtm_step_desc* descs, int n, bool fwd, bool save, int32_t it_begin,
+                  int32_t it_end, ShotDev* shots_dev, const KArgs& m, hipStream_t s,
+                  hipGraph_t* graph, hipGraphExec_t* exec);
 
 static thread_local std::string g_error;
 
@@ -396,8 +399,9 @@ int64_t trtm_batch_workspace_bytes(int32_t n) {
     return n < 1 ? -1 : block_shot_table_bytes(n);
 }
 
-static int run_batch(const trtm_step_desc* descs, int32_t n, void* bws, int64_t bws_bytes,
-                     int32_t it_begin, int32_t it_end, bool fwd, bool save, hipStream_t s) {
+// arguments of a batch call (every shot on shot 0's medium and grid)
+static int check_batch_call(const trtm_step_desc* descs, int32_t n, void* bws, int64_t bws_bytes,
+                            int32_t it_begin, int32_t it_end, bool fwd, bool save) {
     if (!descs || n < 1) return fail(TRTM_ERR_ARG, "empty batch");
     if (!bws || bws_bytes < block_shot_table_bytes(n))
         return fail(TRTM_ERR_ARG, "batch workspace needs %lld bytes",
@@ -424,6 +428,14 @@ static int run_batch(const trtm_step_desc* descs, int32_t n, void* bws, int64_t 
             return fail(TRTM_ERR_ARG, "shot %d workspace needs %lld bytes", q,
                         (long long)block_workspace_bytes(d));
     }
+    return TRTM_OK;
+}
+
+static int run_batch(const trtm_step_desc* descs, int32_t n, void* bws, int64_t bws_bytes,
+                     int32_t it_begin, int32_t it_end, bool fwd, bool save, hipStream_t s) {
+    const int rc = check_batch_call(descs, n, bws, bws_bytes, it_begin, it_end, fwd, save);
+    if (rc) return rc;
+    const trtm_step_desc* d0 = &descs[0];
     if (d0->systems == 0 || it_begin >= it_end) return TRTM_OK;
     return run_block(descs, n, fwd, save, it_begin, it_end, static_cast<ShotDev*>(bws),
                      make_kargs(d0), s);
@@ -441,6 +453,51 @@ int trtm_backward_steps_batch(const trtm_step_desc* descs, int32_t n, void* batc
                               void* stream) {
     return run_batch(descs, n, batch_workspace, batch_workspace_bytes, it_begin, it_end, false,
                      false, (hipStream_t)stream);
+}
+
+struct trtm_plan {
+    hipGraph_t graph;
+    hipGraphExec_t exec;  // null: nothing to run
+};
+
+int trtm_plan_create(const trtm_step_desc* descs, int32_t n, void* batch_workspace,
+                     int64_t batch_workspace_bytes, int32_t it_begin, int32_t it_end,
+                     int32_t forward, int32_t save, void* stream, trtm_plan** plan) {
+    if (!plan) return fail(TRTM_ERR_ARG, "plan is NULL");
+    *plan = nullptr;
+    const bool fwd = forward != 0, sv = fwd && save != 0;
+    if (descs && n >= 1 && descs[0].kernel != TRTM_KERNEL_BLOCKED &&
+        descs[0].kernel != TRTM_KERNEL_WAVE && descs[0].kernel != TRTM_KERNEL_AUTO)
+        return fail(TRTM_ERR_UNSUPPORTED, "plans capture the blocked and wave kernels only");
+    int rc = check_batch_call(descs, n, batch_workspace, batch_workspace_bytes, it_begin, it_end,
+                              fwd, sv);
+    if (rc) return rc;
+    trtm_plan* p = new trtm_plan{nullptr, nullptr};
+    if (descs[0].systems != 0 && it_begin < it_end) {
+        rc = block_capture(descs, n, fwd, sv, it_begin, it_end,
+                           static_cast<ShotDev*>(batch_workspace), make_kargs(&descs[0]),
+                           (hipStream_t)stream, &p->graph, &p->exec);
+        if (rc) {
+            delete p;
+            return rc;
+        }
+    }
+    *plan = p;
+    return TRTM_OK;
+}
+
+int trtm_plan_launch(trtm_plan* plan, void* stream) {
+    if (!plan) return fail(TRTM_ERR_ARG, "plan is NULL");
+    if (plan->exec) TRTM_HIP(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+    return TRTM_OK;
+}
+
+int trtm_plan_destroy(trtm_plan* plan) {
+    if (!plan) return TRTM_OK;
+    if (plan->exec) hipGraphExecDestroy(plan->exec);
+    if (plan->graph) hipGraphDestroy(plan->graph);
+    delete plan;
+    return TRTM_OK;
 }
 
 int trtm_forward_steps(const trtm_step_desc* d, int32_t it_begin, int32_t it_end, int32_t save,

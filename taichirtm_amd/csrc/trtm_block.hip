@@ -832,13 +832,29 @@ static bool launch_block(bool fwd, int systems, BlockCfg c, const KArgs& m, cons
     return launch_sys<false, false, true>(c, m, shots, n_shots, it0, tiles_x, tiles_z, s);
 }
 
-// Runs global steps [it_begin, it_end) for n shots; shot q runs its own steps
+// The launches of one call, prepared: per-shot tile lists and the two
+// shot-table halves already in device memory, so issuing them touches no host
+// memory (and can be captured into a graph).
+struct BlockRun {
+    BlockCfg cfg;
+    int n = 0, systems = 0, it_begin = 0, tiles_x = 0, tiles_z = 0;
+    bool fwd = true;
+    KArgs m;
+    ShotDev* shots_dev = nullptr;
+    std::vector<char> active;                       // per launch: a shot takes part
+    std::vector<std::pair<float*, float*>> copies;  // state copied back (to, from)
+    size_t bytes = 0;
+};
+
+// Global steps [it_begin, it_end) for n shots; shot q runs its own steps
 // [it_begin, it_end) + step_offset, clipped to [0, nt).  One launch covers K
 // global steps.  State ping-pongs between the caller's buffers and the shot's
 // workspace, with each shot's phase set by the first launch it takes part in.
-// shots_dev: device room for 2n ShotDev entries.
-int run_block(const trtm_step_desc* descs, int n, bool fwd, bool save, int32_t it_begin,
-              int32_t it_end, ShotDev* shots_dev, const KArgs& m, hipStream_t s) {
+// shots_dev: device room for 2n ShotDev entries.  Builds the tile lists and
+// writes the shot tables on stream s, then waits for s.
+static int block_prepare(const trtm_step_desc* descs, int n, bool fwd, bool save,
+                         int32_t it_begin, int32_t it_end, ShotDev* shots_dev, const KArgs& m,
+                         hipStream_t s, BlockRun* r) {
     const trtm_step_desc* d0 = &descs[0];
     BlockCfg cfg;
     int rc = block_choose(d0, &cfg);
@@ -873,36 +889,96 @@ int run_block(const trtm_step_desc* descs, int n, bool fwd, bool save, int32_t i
                            s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // host tables must outlive the copies
     if (e != hipSuccess) return fail(TRTM_ERR_HIP, "shot table copy: %s", hipGetErrorString(e));
-    for (int j = 0; j < n_launch; j++) {
-        bool any = false;
-        for (int q = 0; q < n && !any; q++)
-            any = first[q] >= 0 && j >= first[q] && j < first[q] + count[q];
-        if (!any) continue;
-        const ShotDev* tab = shots_dev + (j % 2 ? n : 0);
-        if (!launch_block(fwd, d0->systems, cfg, m, tab, n, it_begin + j * K, L.tiles_x,
-                          L.tiles_z, s))
-            return fail(TRTM_ERR_UNSUPPORTED, "no blocked kernel for the chosen configuration");
-    }
-    e = hipGetLastError();
-    if (e != hipSuccess) return fail(TRTM_ERR_HIP, "blocked launch: %s", hipGetErrorString(e));
-    // a shot that took part in an odd number of launches has its latest state
-    // in the workspace: copy it back
-    const size_t bytes = (size_t)d0->nx * d0->nz * sizeof(float);
+
+    r->cfg = cfg;
+    r->n = n;
+    r->systems = d0->systems;
+    r->it_begin = it_begin;
+    r->tiles_x = L.tiles_x;
+    r->tiles_z = L.tiles_z;
+    r->fwd = fwd;
+    r->m = m;
+    r->shots_dev = shots_dev;
+    r->active.assign(n_launch, 0);
+    for (int j = 0; j < n_launch; j++)
+        for (int q = 0; q < n && !r->active[j]; q++)
+            r->active[j] = first[q] >= 0 && j >= first[q] && j < first[q] + count[q];
+    // a shot that takes part in an odd number of launches ends with its latest
+    // state in the workspace: copied back
+    r->bytes = (size_t)d0->nx * d0->nz * sizeof(float);
     const bool psv = d0->systems & TRTM_PSV, sh = d0->systems & TRTM_SH;
     const bool act[8] = {psv, sh, psv, psv, psv, psv, sh, sh};
-    float* alt_base = nullptr;
+    r->copies.clear();
     for (int q = 0; q < n; q++) {
         if (count[q] % 2 == 0) continue;
         const trtm_step_desc* d = &descs[q];
         float* state[8] = {d->u, d->v, d->w, d->sxx, d->szz, d->sxz, d->syx, d->syz};
-        alt_base = reinterpret_cast<float*>(static_cast<char*>(d->workspace) + L.alt);
+        float* alt_base = reinterpret_cast<float*>(static_cast<char*>(d->workspace) + L.alt);
         const size_t nn = (size_t)d->nx * d->nz;
         for (int k = 0; k < 8; k++)
-            if (act[k]) {
-                e = hipMemcpyAsync(state[k], alt_base + k * nn, bytes, hipMemcpyDeviceToDevice, s);
-                if (e != hipSuccess) return fail(TRTM_ERR_HIP, "state copy: %s", hipGetErrorString(e));
-            }
+            if (act[k]) r->copies.emplace_back(state[k], alt_base + k * nn);
     }
+    return TRTM_OK;
+}
+
+// the prepared launches and state copies, device work only
+static int block_launch(const BlockRun& r, hipStream_t s) {
+    const int K = r.cfg.k;
+    for (int j = 0; j < (int)r.active.size(); j++) {
+        if (!r.active[j]) continue;
+        const ShotDev* tab = r.shots_dev + (j % 2 ? r.n : 0);
+        if (!launch_block(r.fwd, r.systems, r.cfg, r.m, tab, r.n, r.it_begin + j * K, r.tiles_x,
+                          r.tiles_z, s))
+            return fail(TRTM_ERR_UNSUPPORTED, "no blocked kernel for the chosen configuration");
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(TRTM_ERR_HIP, "blocked launch: %s", hipGetErrorString(e));
+    for (const auto& c : r.copies) {
+        e = hipMemcpyAsync(c.first, c.second, r.bytes, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return fail(TRTM_ERR_HIP, "state copy: %s", hipGetErrorString(e));
+    }
+    return TRTM_OK;
+}
+
+int run_block(const trtm_step_desc* descs, int n, bool fwd, bool save, int32_t it_begin,
+              int32_t it_end, ShotDev* shots_dev, const KArgs& m, hipStream_t s) {
+    BlockRun r;
+    const int rc = block_prepare(descs, n, fwd, save, it_begin, it_end, shots_dev, m, s, &r);
+    return rc ? rc : block_launch(r, s);
+}
+
+// Prepares the launches on stream s, then records them into a graph on a
+// stream of the library's own (default priority; the caller's stream may be
+// the null stream, which cannot capture) and instantiates it.  Nothing runs.
+int block_capture(const trtm_step_desc* descs, int n, bool fwd, bool save, int32_t it_begin,
+                  int32_t it_end, ShotDev* shots_dev, const KArgs& m, hipStream_t s,
+                  hipGraph_t* graph, hipGraphExec_t* exec) {
+    BlockRun r;
+    int rc = block_prepare(descs, n, fwd, save, it_begin, it_end, shots_dev, m, s, &r);
+    if (rc) return rc;
+    hipStream_t cap;
+    hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(TRTM_ERR_HIP, "capture stream: %s", hipGetErrorString(e));
+    e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        hipStreamDestroy(cap);
+        return fail(TRTM_ERR_HIP, "begin capture: %s", hipGetErrorString(e));
+    }
+    rc = block_launch(r, cap);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cap, &g);  // ends the capture on every path
+    hipStreamDestroy(cap);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(TRTM_ERR_HIP, "end capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        return fail(TRTM_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    *graph = g;
     return TRTM_OK;
 }
 

@@ -73,6 +73,29 @@ class ForwardModeling(Propagator):
         Returns 0 on success or 1 / 2 / 3 when u / v / w became non-finite
         (|x| > 1e30 or NaN) at a check step it % stability_check_interval == 0.
         """
+        d, flags, ws, sci = self._prepare_run(save, stability_check_interval)  # noqa: F841
+        nx, nz, nt = self.nx, self.nz, self.nt
+        s = stream_handle()
+        stops = range(0, nt, self.isnap) if display_callback is not None else ()
+        code, it_fail = 0, None
+        for a, b in self._chunks(nt, sci, stops):
+            _lib.call("trtm_forward_steps", d, a, b, int(bool(save)), s)
+            host_flags = flags.cpu().numpy()
+            code, it_fail = self._first_failure(host_flags, sci, a, b - 1)
+            if code:
+                break
+            # order inside a step: record -> callback -> finite check (:551-566)
+            if display_callback is not None and (b - 1) % self.isnap == 0:
+                display_callback(self.u.to_numpy(), self.v.to_numpy(), self.w.to_numpy(),
+                                 b - 1, nx, nz, self.dx, self.dz)
+            code, it_fail = self._first_failure(host_flags, sci, b - 1, b)
+            if code:
+                break
+        return self._finish_run(save, code, it_fail)
+
+    def _prepare_run(self, save, stability_check_interval):
+        """Snapshot buffers, descriptor, check flags and workspace of a run;
+        the autotuner's choice when kernel="tuned"."""
         sci = int(stability_check_interval)
         if sci < 1:
             raise ValueError("stability_check_interval must be >= 1")
@@ -109,26 +132,17 @@ class ForwardModeling(Propagator):
             d.snap_u, d.snap_v, d.snap_w = (_ptr(getattr(self, c + "_save_field"))
                                             for c in ("u", "v", "w"))
         d.rec_u, d.rec_v, d.rec_w = (_ptr(self._seis.get(c)) for c in ("u", "v", "w"))
-        ws = self._workspace(d)  # noqa: F841  (kept alive for the queued kernels)
+        ws = self._workspace(d)  # kept alive for the queued kernels
         s = stream_handle()
         self._autotune(d, "forward",
                        lambda dd, n: _lib.call("trtm_forward_steps", dd, 0, n, int(bool(save)), s))
         flags.zero_()
-        stops = range(0, nt, self.isnap) if display_callback is not None else ()
-        code, it_fail = 0, None
-        for a, b in self._chunks(nt, sci, stops):
-            _lib.call("trtm_forward_steps", d, a, b, int(bool(save)), s)
-            host_flags = flags.cpu().numpy()
-            code, it_fail = self._first_failure(host_flags, sci, a, b - 1)
-            if code:
-                break
-            # order inside a step: record -> callback -> finite check (:551-566)
-            if display_callback is not None and (b - 1) % self.isnap == 0:
-                display_callback(self.u.to_numpy(), self.v.to_numpy(), self.w.to_numpy(),
-                                 b - 1, nx, nz, self.dx, self.dz)
-            code, it_fail = self._first_failure(host_flags, sci, b - 1, b)
-            if code:
-                break
+        return d, flags, ws, sci
+
+    def _finish_run(self, save, code, it_fail):
+        """isnaps of the written snapshots; on a blow-up, outputs cut at the
+        failing step as the reference's early return leaves them."""
+        nt = self.nt
         last = nt if it_fail is None else it_fail  # steps whose snapshot the reference kept
         if save:
             isn = np.zeros(self.num_snaps, np.int32)
@@ -148,6 +162,20 @@ class ForwardModeling(Propagator):
             return code
         print("Forward modeling completed")
         return 0
+
+    def capture(self, save: bool = False, stability_check_interval: int = 100):
+        """The whole run [0, nt) as a ``CapturedRun``: recorded once into a
+        HIP graph (blocked or wave kernels), replayed by ``launch()``; no
+        display callback and no early stop (``result()`` reads the check
+        flags afterwards, with run()'s codes).  Every launch continues from
+        the current state: call ``reset_state()`` first for a fresh run."""
+        from .plan import StepPlan
+        d, flags, ws, sci = self._prepare_run(save, stability_check_interval)
+        snaps = [getattr(self, c + "_save_field", None) for c in ("u", "v", "w")] if save else []
+        plan = StepPlan([d], 0, self.nt, forward=True, save=save,
+                        keep=[ws, flags] + [f for f in snaps if f is not None])
+        return CapturedRun(self, plan, flags, sci, save)
+
 
     def get_seismogram(self) -> tuple:
         """Seismograms (u, v, w), each (num_receivers, nt) numpy float32
@@ -177,3 +205,24 @@ class ForwardModeling(Propagator):
         self._seis = {}
         self._coef = {}
         self._inj = []
+
+
+class CapturedRun:
+    """A propagator's run captured by ``capture``."""
+
+    def __init__(self, prop, plan, flags, sci, save):
+        self.prop, self.plan, self.flags, self.sci, self.save = prop, plan, flags, sci, save
+
+    def launch(self):
+        """Clear the check flags and replay the steps on the current stream."""
+        self.flags.zero_()
+        self.plan.launch()
+
+    def result(self) -> int:
+        """Wait for the last launch; 0, or the code of the first failing check."""
+        code, it_fail = self.prop._first_failure(self.flags.cpu().numpy(), self.sci, 0,
+                                                 self.prop.nt)
+        return self.prop._finish_run(self.save, code, it_fail)
+
+    def close(self):
+        self.plan.close()

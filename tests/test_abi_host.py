@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _lib.load().trtm_abi_version() == _lib.ABI_VERSION == 4
+    assert _lib.load().trtm_abi_version() == _lib.ABI_VERSION == 5
 
 
 @pytest.mark.parametrize("cname,pystruct", [("trtm_step_desc", _lib.StepDesc),
@@ -138,3 +138,17 @@ def test_block_configs_enumerate_both_families(systems):
             assert tz == 64 - 2 * k
     buf = (ctypes.c_int32 * 4)()
     assert _lib.load().trtm_block_configs(systems, 99, buf, 1) < 0  # not a blocked family
+
+
+def test_plan_entries_check_arguments_without_device():
+    """The captured-stepping entries reject missing handles before any GPU work."""
+    lib = _lib.load()
+    assert lib.trtm_plan_destroy(None) == 0
+    assert lib.trtm_plan_launch(None, None) < 0
+    assert b"plan is NULL" in lib.trtm_last_error()
+    assert lib.trtm_plan_create(None, 0, None, 0, 0, 1, 1, 0, None, None) < 0
+    assert b"plan is NULL" in lib.trtm_last_error()
+    handle = ctypes.c_void_p()
+    assert lib.trtm_plan_create(None, 0, None, 0, 0, 1, 1, 0, None, ctypes.byref(handle)) < 0
+    assert b"empty batch" in lib.trtm_last_error()
+    assert handle.value is None
