@@ -19,6 +19,10 @@ cpu_baseline: the oracle (C/OpenMP restatement of the reference's unfused
          reference does) timed on a bounded sample of the same workload.
 kernel:  "tuned" (default) lets the autotuner pick the blocked kernel family
          and shape per direction during the warmup step.
+graph:   after the warmup the forward and the backward run are each recorded
+         into a HIP graph (trtm_plan_*) and every later step replays them (one
+         more untimed step replays them first); --no-graph issues the step
+         launches eagerly.
 
 Other workloads (--workload psv_batch | fwi | salt, --shots N): the batched
 checkpointed P-SV RTM of the 2000 x 1000 faulted model, the P-SV FWI
@@ -230,6 +234,15 @@ class RTMShot:
         self._copy_obs()
         self.bw = trtm.BackwardModeling(obsdata_u_field=obs[0], obsdata_v_field=obs[1],
                                         obsdata_w_field=obs[2], kernel=kernel, **self.kw)
+        self.cf = self.cb = None
+
+    def capture(self):
+        """Record the forward and the backward run each into a HIP graph
+        (after the kernels are tuned); later steps replay them."""
+        self.cf = self.fw.capture(save=True)
+        self.cb = self.bw.capture_calc(self.fw.u_save_field, self.fw.v_save_field,
+                                       self.fw.w_save_field, self.fw.isnaps_field,
+                                       self.fw.num_snaps, self.isnap)
 
     def _copy_obs(self):
         from taichirtm_amd import _lib
@@ -243,18 +256,26 @@ class RTMShot:
         self.bw.reset_state()
         if ev:
             ev[0].record()
-        code = self.fw.run(save=True)
+        if self.cf:
+            self.cf.launch()
+        else:
+            code = self.fw.run(save=True)
         if ev:
             ev[1].record()
         # forward seismogram -> observed data of the backward pass (device transpose)
         self._copy_obs()
         if ev:
             ev[2].record()
-        code_b = self.bw.run_calc(self.fw.u_save_field, self.fw.v_save_field,
-                                  self.fw.w_save_field, self.fw.isnaps_field, self.fw.num_snaps,
-                                  self.isnap)
+        if self.cb:
+            self.cb.launch()
+        else:
+            code_b = self.bw.run_calc(self.fw.u_save_field, self.fw.v_save_field,
+                                      self.fw.w_save_field, self.fw.isnaps_field,
+                                      self.fw.num_snaps, self.isnap)
         if ev:
             ev[3].record()
+        if self.cf:
+            code, code_b = self.cf.result(), self.cb.result()
         if code or code_b:
             raise RuntimeError("non-finite wavefield (codes %d, %d)" % (code, code_b))
 
@@ -317,6 +338,9 @@ def main():
                     help="salt: device memory of the job; fwi: forward history kept; "
                          "psv_batch: segment wavefields kept")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="single-shot RTM: issue the step launches eagerly instead of replaying "
+                         "the captured forward and backward runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU; multi-GPU runs are not part of this build "
@@ -331,6 +355,10 @@ def main():
     cfg, system = build_workload(args.workload)
     shot = RTMShot(cfg, args.isnap, args.kernel, args.tile_x, args.block_steps, args.tile_z)
     for _ in range(args.warmup):
+        shot.run()
+    graph = not args.no_graph and args.kernel in ("blocked", "wave", "auto", "tuned")
+    if graph:  # each pass one HIP graph launch from here on
+        shot.capture()
         shot.run()
     torch.cuda.synchronize()
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
@@ -386,7 +414,7 @@ def main():
                    "grid": [cfg.nx, cfg.nz], "nt": nt, "dx": cfg.dx, "dt": cfg.dt,
                    "absorbing_frame": cfg.fw, "receivers": len(cfg.shots[0].rec),
                    "kernel": args.kernel, "tile_x": args.tile_x, "tile_z": args.tile_z,
-                   "block_steps": args.block_steps, "system": system,
+                   "block_steps": args.block_steps, "system": system, "graph": graph,
                    "chosen": {"forward": list(shot.fw._tuned or ()),
                               "backward": list(shot.bw._tuned or ())}},
         "shots_per_hour": round(3600.0 / (ms_per_step / 1000.0), 1),

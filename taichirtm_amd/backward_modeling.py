@@ -139,7 +139,10 @@ class BackwardModeling(Propagator):
         by ``launch()``; no display callback and no early stop (``result()``
         reads the check flags afterwards, codes 4-6).  Each launch continues
         from the current state and adds to the current image: call
-        ``reset_state()`` first for a fresh run."""
+        ``reset_state()`` first for a fresh run.  Snapshots passed as a
+        forward propagator's own snapshot fields are read where they live
+        (a replayed forward capture refreshes them); other snapshot arrays and
+        the isnaps are copied when the run is captured."""
         from .plan import StepPlan
         d, flags, ws, sci, sysmask, imaging, snaps, isnaps = self._prepare_calc(
             import_fwdata_u, import_fwdata_v, import_fwdata_w, isnaps_field, num_snaps,

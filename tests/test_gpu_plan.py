@@ -109,48 +109,66 @@ def test_batch_graph_equals_eager(kernel):
 @pytest.mark.parametrize("system,kernel", [("psv", "blocked"), ("sh", "blocked"), ("sh", "wave")])
 def test_captured_rtm_replays_equal_eager(system, kernel):
     """Forward and backward both captured (the single-shot RTM of one shot as
-    two graph launches): three replays from reset state give run() /
-    run_calc()'s seismograms and images bit for bit."""
+    two graph launches), replayed three times from reset state with the
+    source wavelet changed in place in between (x1, x2, x1): every replay
+    gives the seismograms and image of run() / run_calc() with that wavelet,
+    bit for bit, so the graphs read wavelets, forward snapshots and observed
+    data where they live."""
     _case("captured RTM %s %s" % (system, kernel))
     nt = 131
     cfg = (configs.psv_anomaly(nx=60, nz=50, nt=nt, fw=10, radius=8, src=(30, 5),
                                rec_range=(2, 58), f0=40.0) if system == "psv"
            else configs.sh_layered(nx=96, nz=70, nt=nt, fw=12, f0=40.0, seed=3))
     kw = cfg.kwargs(cfg.shots[0])
+    wav = cfg.wavelets(cfg.shots[0])
+
+    def forward(scale):
+        w = [np.asarray(x, np.float32) * np.float32(scale) for x in wav]
+        return trtm.ForwardModeling(wavelet_u_field=w[0], wavelet_v_field=w[1],
+                                    wavelet_w_field=w[2], isnap=3, kernel=kernel, **kw)
 
     def backward(fw):
         seis = fw.get_seismogram()
         return trtm.BackwardModeling(obsdata_u_field=seis[0], obsdata_v_field=seis[1],
                                      obsdata_w_field=seis[2], kernel=kernel, **kw)
 
-    ref_fw = _forward(cfg, kernel)
-    assert ref_fw.run(save=True) == 0
-    ref_bw = backward(ref_fw)
-    assert ref_bw.run_calc(ref_fw.u_save_field, ref_fw.v_save_field, ref_fw.w_save_field,
-                           ref_fw.isnaps_field, ref_fw.num_snaps, 3) == 0
-    ref_img = ref_bw.get_results()
+    ref = {}
+    for scale in (1.0, 2.0):
+        f = forward(scale)
+        assert f.run(save=True) == 0
+        b = backward(f)
+        assert b.run_calc(f.u_save_field, f.v_save_field, f.w_save_field, f.isnaps_field,
+                          f.num_snaps, 3) == 0
+        ref[scale] = (f.get_seismogram(), b.get_results())
 
-    fw = _forward(cfg, kernel)
+    fw = forward(1.0)
     cf = fw.capture(save=True)
     cf.launch()
     assert cf.result() == 0
     bw = backward(fw)
     cb = bw.capture_calc(fw.u_save_field, fw.v_save_field, fw.w_save_field, fw.isnaps_field,
                          fw.num_snaps, 3)
+    wav0 = [t.clone() for t in fw._inj]
     try:
-        for rep in range(3):
+        for rep, scale in enumerate((1.0, 2.0, 1.0)):
+            for t, t0 in zip(fw._inj, wav0):  # the wavelets change in place
+                t.copy_(t0 * scale)
             fw.reset_state()
             bw.reset_state()
             float(torch.randn(1 << 18, device="cuda").abs().max())  # eager work between replays
             cf.launch()
+            for c, t in zip(("u", "v", "w"), bw._inj):  # observed data = this forward's records
+                if t.numel() and c in fw._seis:
+                    t.copy_(fw._seis[c].t())
             cb.launch()
             assert cf.result() == 0 and cb.result() == 0
             img = bw.get_results()
-            print("[plan] RTM replay %d: image max %.3e" % (rep, max(np.abs(x).max() for x in img)),
-                  flush=True)
-            for a, b in zip(img, ref_img):
+            print("[plan] RTM replay %d (source x%g): image max %.3e"
+                  % (rep, scale, max(np.abs(x).max() for x in img)), flush=True)
+            seis_ref, img_ref = ref[scale]
+            for a, b in zip(img, img_ref):
                 assert same(a, b), rep
-            for a, b in zip(fw.get_seismogram(), ref_fw.get_seismogram()):
+            for a, b in zip(fw.get_seismogram(), seis_ref):
                 assert same(a, b), rep
     finally:
         cf.close()
