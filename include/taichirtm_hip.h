@@ -234,8 +234,9 @@ int trtm_backward_steps_batch(const trtm_step_desc *descs, int32_t n, void *batc
  * state the buffers then hold, with the same results as the batch call.  The
  * plan keeps every pointer of the descriptors: the caller keeps those buffers
  * alive and leaves both workspaces untouched (they hold the lists and tables)
- * until trtm_plan_destroy; buffer contents (state, medium, wavelets, data,
- * image, check flags) may change between launches.  Blocked and wave kernels
+ * until trtm_plan_destroy, which is called once the plan's launches have
+ * finished (after a synchronisation of their stream); buffer contents (state,
+ * medium, wavelets, data, image, check flags) may change between launches.  Blocked and wave kernels
  * (kernel TRTM_KERNEL_AUTO, _BLOCKED or _WAVE).  No reference counterpart: it
  * replaces the per-step kernel calls of the loops in ForwardModeling.run
  * (forward_modeling.py:539-572) and BackwardModeling.run_calc

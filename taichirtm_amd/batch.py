@@ -497,12 +497,18 @@ class BatchRTM:
 
     def external_codes(self) -> List[int]:
         """Per-shot backward codes (0, or 4-6 with the first failing check step)."""
+        self._close_plans()
         out = []
         for fl in self.flags_b:
             h = fl.cpu().numpy()
             nz_ = np.nonzero(h)[0]
             out.append(int(h[nz_[0]]) if len(nz_) else 0)
         return out
+
+    def _close_plans(self):
+        for plan in self._plans:
+            plan.close()
+        self._plans = []
 
     # -- driver -------------------------------------------------------------
 
@@ -511,9 +517,7 @@ class BatchRTM:
         (0, 1-3 forward, 4-6 backward; the first failing check step)."""
         self._forward()
         self._backward()
-        for plan in self._plans:
-            plan.close()
-        self._plans = []
+        self._close_plans()
         self.codes = []
         for q, s in enumerate(self.shots):
             code = 0
