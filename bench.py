@@ -24,11 +24,12 @@ graph:   after the warmup the forward and the backward run are each recorded
          more untimed step replays them first); --no-graph issues the step
          launches eagerly.
 
-Other workloads (--workload psv_batch | fwi | salt, --shots N): the batched
-checkpointed P-SV RTM of the 2000 x 1000 faulted model, the P-SV FWI
-gradient of the 600 x 300 x 2500 problem, and the FWI gradient plus RTM image
-of the 4000 x 2000 x 8000 salt model under a device-memory budget
-(--budget-gb); one step is the whole job, the roofline is job-level (useful
+Other workloads (--workload psv_batch | fwi | salt): the batched
+checkpointed P-SV RTM of the 2000 x 1000 faulted model (64 shots in batches
+of 16), the P-SV FWI gradient of the 600 x 300 x 2500 problem (16 shots), and
+the FWI gradient plus RTM image of the 4000 x 2000 x 8000 salt model (32
+shots) under a device-memory budget (--budget-gb; --shots N for fewer
+shots); one step is the whole job, the roofline is job-level (useful
 cell-updates x algorithmic bytes / wall).  Prints one JSON line.
 """
 from __future__ import annotations
@@ -101,6 +102,14 @@ def job_main(args):
     whole job over --shots shots."""
     import torch
     from taichirtm_amd import autotune, configs
+    # defaults: the full survey of each workload and a memory budget that
+    # fits one MI355X beside it
+    full = {"psv_batch": (64, 16, 200.0), "fwi": (16, 0, 200.0), "salt": (32, 0, 128.0)}
+    if args.workload in full:
+        n, g, gb = full[args.workload]
+        args.shots = args.shots or n
+        args.group = args.group if args.group is not None else g
+        args.budget_gb = args.budget_gb if args.budget_gb is not None else gb
     if args.workload == "psv_batch":
         from taichirtm_amd.batch import BatchRTM
         cfg = configs.psv_faulted()
@@ -327,14 +336,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sh_layered")
     ap.add_argument("--kernel", default="tuned")
-    ap.add_argument("--shots", type=int, default=8)
-    ap.add_argument("--group", type=int, default=0,
+    ap.add_argument("--shots", type=int, default=0,
+                    help="jobs: number of shots (default: the full survey, 64 / 16 / 32)")
+    ap.add_argument("--group", type=int, default=None,
                     help="psv_batch: shots per batch (0 = all in one batch)")
     ap.add_argument("--tile-x", type=int, default=0)
     ap.add_argument("--block-steps", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--isnap", type=int, default=1)
-    ap.add_argument("--budget-gb", type=float, default=128.0,
+    ap.add_argument("--budget-gb", type=float, default=None,
                     help="salt: device memory of the job; fwi: forward history kept; "
                          "psv_batch: segment wavefields kept")
     ap.add_argument("--no-cpu-baseline", action="store_true")
