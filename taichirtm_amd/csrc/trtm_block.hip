@@ -680,6 +680,86 @@ __global__ void k_block_lists(int mode, int tx, int tz, int K, int tiles_x, int 
     }
 }
 
+// The same lists in one workgroup per shot, with counts, cursors and scans in
+// LDS (dynamic, 4 * tiles + 2 ints): one launch instead of eight.  Only the
+// list outputs go to global memory.
+constexpr int kListsOneMaxTiles = 3800;  // LDS stays under 64 KB
+
+__global__ __launch_bounds__(1024) void k_block_lists_one(int tx, int tz, int K, int tiles_x,
+                                                          int tiles_z, int n_groups,
+                                                          const int* loc, const int* order,
+                                                          const int* start, int* istart_g,
+                                                          int* iitems, int n_rec,
+                                                          const int* rloc, int* rstart_g,
+                                                          int* ritems) {
+    extern __shared__ int lds[];
+    __shared__ int part[1024];
+    const int t = threadIdx.x, tiles = tiles_x * tiles_z;
+    int* icnt = lds;
+    int* rcnt = lds + tiles;
+    int* ist = lds + 2 * tiles;          // tiles + 1 entries
+    int* rst = lds + 3 * tiles + 1;      // tiles + 1 entries
+    auto zero = [&]() {
+        for (int i = t; i < tiles; i += 1024) icnt[i] = rcnt[i] = 0;
+        __syncthreads();
+    };
+    // mode 0 counts, mode 1 places (same tile rule as k_block_lists)
+    auto pass = [&](int mode) {
+        const int m = n_groups > n_rec ? n_groups : n_rec;
+        for (int q = t; q < m; q += 1024) {
+            if (q < n_groups) {
+                const int k0 = order[start[q]];
+                const int ci = loc[2 * k0], cj = loc[2 * k0 + 1];
+                for (int ti = (ci - K - tx + 1) / tx - 1; ti <= (ci + K) / tx + 1; ti++)
+                    for (int tk = (cj - K - tz + 1) / tz - 1; tk <= (cj + K) / tz + 1; tk++) {
+                        if (ti < 0 || ti >= tiles_x || tk < 0 || tk >= tiles_z) continue;
+                        const int ri0 = ti * tx - K, rj0 = tk * tz - K;
+                        if (ci < ri0 || ci >= ri0 + tx + 2 * K || cj < rj0 || cj >= rj0 + tz + 2 * K)
+                            continue;
+                        const int tile = ti * tiles_z + tk;
+                        const int pos = atomicAdd(icnt + tile, 1);
+                        if (mode == 1) iitems[ist[tile] + pos] = q;
+                    }
+            }
+            if (q < n_rec) {
+                const int tile = (rloc[2 * q] / tx) * tiles_z + rloc[2 * q + 1] / tz;
+                const int pos = atomicAdd(rcnt + tile, 1);
+                if (mode == 1) ritems[rst[tile] + pos] = q;
+            }
+        }
+        __syncthreads();
+    };
+    // exclusive scan of cnt into st[0..tiles] (LDS) and out[0..tiles] (global)
+    auto scan = [&](const int* cnt, int* st, int* out) {
+        const int chunk = (tiles + 1023) / 1024;
+        const int b = t * chunk, e = min(tiles, b + chunk);
+        int sum = 0;
+        for (int k = b; k < e; k++) sum += cnt[k];
+        part[t] = sum;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int v = t >= off ? part[t - off] : 0;
+            __syncthreads();
+            part[t] += v;
+            __syncthreads();
+        }
+        int run = t > 0 ? part[t - 1] : 0;
+        for (int k = b; k < e; k++) {
+            st[k] = run;
+            out[k] = run;
+            run += cnt[k];
+        }
+        if (t == 1023) st[tiles] = out[tiles] = part[1023];
+        __syncthreads();
+    };
+    zero();
+    pass(0);
+    scan(icnt, ist, istart_g);
+    scan(rcnt, rst, rstart_g);
+    zero();
+    pass(1);
+}
+
 static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg, int it_lo,
                       int it_hi, hipStream_t s, ShotDev* sd) {
     const int tx = cfg.tx, K = cfg.k;
@@ -699,6 +779,14 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
         const int rc = wave_build_lists(d, cfg, istart, icur, ws + L.inj_items, rstart, rcur,
                                         ws + L.rec_items, ws + L.maps, s);
         if (rc) return rc;
+    } else if (tiles <= kListsOneMaxTiles) {
+        const size_t lds = (4 * (size_t)tiles + 2) * sizeof(int);
+        hipLaunchKernelGGL(k_block_lists_one, dim3(1), dim3(1024), lds, s, tx, cfg.tz, K,
+                           L.tiles_x, L.tiles_z, ng, d->inj_loc, d->inj_order,
+                           d->inj_group_start, istart, iitems, nr, d->rec_loc, rstart, ritems);
+        e = hipGetLastError();
+        if (e != hipSuccess)
+            return fail(TRTM_ERR_HIP, "blocked tile lists: %s", hipGetErrorString(e));
     } else {
         const dim3 tb(256), tg((tiles + 255) / 256), lg((std::max(ng, nr) + 255) / 256);
         hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, icur, tiles, 0);
