@@ -194,6 +194,12 @@ def test_sh_layered_reduced_parity_and_full_size_kernels_agree():
     h = run_hip(kw, wav, isnap=10)
     o = run_oracle(kw, wav, isnap=10)
     assert_bitwise(h, o)
+    o64 = run_oracle(kw, wav, isnap=10, dtype=np.float64)
+    e_seis = oracle.relative_l2(h["seis"][1], o64["seis"][1])
+    e_img = oracle.relative_l2(h["image"][1], o64["image"][1])
+    print("[fp64] SH 1200x600x300 seismogram v rel-L2 %.3e  image v rel-L2 %.3e"
+          % (e_seis, e_img), flush=True)
+    assert e_seis < 1e-4 and e_img < 1e-3
     cfg = configs.sh_layered()
     kw = cfg.kwargs(cfg.shots[0])
     wav = cfg.wavelets(cfg.shots[0])
@@ -256,3 +262,33 @@ def test_wave_configurations_bitwise(system, cfg):
     h = run_hip(hk, wav, isnap=3, kernel="wave")
     o = run_oracle(kw, wav, isnap=3)
     assert_bitwise(h, o)
+
+
+@pytest.mark.parametrize("workload", ["sh_layered", "psv_faulted"])
+def test_full_size_forward_error_against_float64(workload):
+    """The float32 scheme's error against its float64 run at full size
+    (SH 1200x600x4000; the first shot of the 2000x1000 P-SV survey): the
+    forward seismograms of ForwardModeling (blocked kernels) against the FWI
+    forward step run in double, which is the same scheme (its float32 records
+    equal ForwardModeling's bit for bit)."""
+    from taichirtm_amd.batch import ShotSpec
+    from taichirtm_amd.fwi import ElasticFWI
+    import torch
+    _case("float64 error %s" % workload)
+    cfg = configs.sh_layered() if workload == "sh_layered" else configs.psv_faulted(n_shots=1)
+    shot = cfg.shots[0]
+    kw = cfg.kwargs(shot)
+    wav = cfg.wavelets(shot)
+    h = run_hip(kw, wav, isnap=10, kernel="auto", backward=False)
+    spec = ShotSpec(np.array([shot.src], np.int32), tuple(np.asarray(w, np.float64) for w in wav),
+                    np.asarray(shot.rec, np.int32))
+    p = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, [spec], precision="float64",
+                   checkpoint_every=shot.nt)
+    m = [torch.as_tensor(np.asarray(x, np.float64), device="cuda") for x in (cfg.vp, cfg.vs, cfg.rho)]
+    rec = p.forward(*m)[0].cpu().numpy()
+    comps = [1] if workload == "sh_layered" else [0, 2]
+    for c in comps:
+        e = oracle.relative_l2(h["seis"][c], rec[c])
+        print("[fp64] %s %dx%dx%d seismogram %s rel-L2 %.3e"
+              % (workload, cfg.nx, cfg.nz, shot.nt, "uvw"[c], e), flush=True)
+        assert e < 1e-3
