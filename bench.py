@@ -14,6 +14,8 @@ roofline: the dominant kernel is the backward step with imaging; its average
          launch time is measured with HIP events on the launch stream over the
          timed steps; algorithmic bytes per cell per launch are listed in
          ALGO_BYTES.
+f64_rel_l2: the timed run's forward seismograms against the same scheme in
+         double (relative L2), computed after the timed region.
 cpu_baseline: the oracle (C/OpenMP restatement of the reference's unfused
          per-step kernel sequence, float32, both wave systems updated as the
          reference does) timed on a bounded sample of the same workload.
@@ -439,9 +441,35 @@ def main():
                      "forward_launch_us": round(fw_launch_us, 3),
                      "forward_achieved": round(f_bytes / (fw_launch_us * 1e-6) / 1e9, 1)},
     }
+    out["f64_rel_l2"] = float64_error(cfg, shot)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
+
+
+def float64_error(cfg, shot):
+    """Relative L2 error (||a - b|| / ||b||) of the last timed forward
+    seismograms against the same scheme in double (the FWI forward step, whose
+    float32 records equal ForwardModeling's bit for bit), outside the timed
+    region."""
+    import torch
+    from taichirtm_amd.batch import ShotSpec
+    from taichirtm_amd.fwi import ElasticFWI
+    s = cfg.shots[0]
+    spec = ShotSpec(np.array([s.src], np.int32),
+                    tuple(np.asarray(w, np.float64) for w in cfg.wavelets(s)),
+                    np.asarray(s.rec, np.int32))
+    p = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, [spec], precision="float64",
+                   checkpoint_every=s.nt)
+    m = [torch.as_tensor(np.asarray(x, np.float64), device="cuda") for x in (cfg.vp, cfg.vs, cfg.rho)]
+    ref = p.forward(*m)[0].cpu().numpy()
+    got = shot.fw.get_seismogram()
+    out = {}
+    for c, name in enumerate("uvw"):
+        if np.abs(ref[c]).max() > 0:
+            a, b = np.asarray(got[c], np.float64), np.asarray(ref[c], np.float64)
+            out["seismogram_" + name] = float("%.3e" % (np.linalg.norm(a - b) / np.linalg.norm(b)))
+    return out
 
 
 if __name__ == "__main__":
