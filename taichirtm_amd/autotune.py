@@ -52,6 +52,10 @@ def tune(key: tuple, systems: int, trial: Callable[[Shape], None], repeats: int 
     that shape on the current stream."""
     if key in _CACHE:
         return _CACHE[key]
+    if not systems and shapes is None:
+        # no wave system propagates (e.g. a backward pass with no data): no
+        # step kernel runs, the library's default shape stands
+        return ("blocked", 0, 0, 0)
     times = []
     for shape in (shapes if shapes is not None else candidates(systems)):
         trial(shape)  # first call: tile lists, caches

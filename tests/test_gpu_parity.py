@@ -115,6 +115,28 @@ def test_nonfinite_codes(kernel):
     assert h["code_b"] == o["code_b"] == 6
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_zero_receivers_and_border_source(kernel):
+    """No receivers at all (empty seismograms, nothing injected backward: the
+    image stays zero), and a source on the grid border, where 1 / rho_w is
+    +inf (forward_modeling.py:291-307), so the reference's w blows up and the
+    forward returns code 3 at the first check: both as the oracle."""
+    _case("zero receivers, border source kernel=%s" % kernel)
+    cfg = small_psv(nt=60)
+    kw = cfg.kwargs(cfg.shots[0])
+    kw["recv_loc_field"], kw["num_receivers"] = np.zeros((0, 2), np.int32), 0
+    wav = cfg.wavelets(cfg.shots[0])
+    h = run_hip(kw, wav, isnap=2, kernel=kernel)
+    o = run_oracle(kw, wav, isnap=2)
+    assert_bitwise(h, o)
+    assert all(np.asarray(x).shape == (0, cfg.nt) for x in h["seis"])
+    assert not np.any(h["image"][2])
+    kw["src_loc_field"] = np.array([[0, 1]], np.int32)
+    h = run_hip(kw, wav, isnap=2, kernel=kernel, backward=False)
+    o = run_oracle(kw, wav, isnap=2, backward=False)
+    assert h["code_f"] == o["code_f"] == 3
+
+
 def test_mxz_overflow_reproduced():
     """SI rock moduli overflow mu^4 in float32 (forward_modeling.py:271-276):
     the reference's shear stress turns NaN; the HIP path reports the same code."""
