@@ -137,6 +137,20 @@ def test_zero_receivers_and_border_source(kernel):
     assert h["code_f"] == o["code_f"] == 3
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_short_records(kernel):
+    """Records of 1, 2 and 9 steps (fewer steps than one temporally blocked
+    launch, and one more than K = 8), snapshots every step: as the oracle."""
+    _case("short records kernel=%s" % kernel)
+    for nt in (1, 2, 9):
+        cfg = small_sh(nt=nt)
+        kw = cfg.kwargs(cfg.shots[0])
+        wav = cfg.wavelets(cfg.shots[0])
+        h = run_hip(kw, wav, isnap=1, kernel=kernel)
+        o = run_oracle(kw, wav, isnap=1)
+        assert_bitwise(h, o)
+
+
 def test_mxz_overflow_reproduced():
     """SI rock moduli overflow mu^4 in float32 (forward_modeling.py:271-276):
     the reference's shear stress turns NaN; the HIP path reports the same code."""
