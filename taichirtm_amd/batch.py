@@ -312,6 +312,10 @@ class BatchRTM:
                         obs.append(o)
             self.bw.append(BackwardModeling(obsdata_u_field=obs[0], obsdata_v_field=obs[1],
                                             obsdata_w_field=obs[2], **self._common(s)))
+        if len({b.systems for b in self.bw}) != 1:
+            raise ValueError("the observed data of every shot of a batch must excite the same "
+                             "wave systems (a shot without data migrates to a zero image on "
+                             "its own)")
         self.flags_b = []
         descs, keep, self._isn = [], [], []
         nt_max = max(s.nt for s in self.shots)
