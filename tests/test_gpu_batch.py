@@ -109,3 +109,15 @@ def test_segments_kept_by_forward_bitwise(system, isnap, kernel):
             for c in range(3):
                 assert same(imgs[q][c], ref[1][q][c]), (q, c)
                 assert same(seis[q][c], ref[2][q][c]), (q, c)
+
+
+def test_batch_rejects_shots_with_different_data_systems():
+    """A shot without receivers has no backward wave system: a batch mixing
+    it with shots that carry data says so instead of failing in the library."""
+    _case("batch mixed systems")
+    cfg, mu, lam, shots = survey("psv")
+    shots[1] = ShotSpec(shots[1].src_loc, shots[1].wavelets, np.zeros((0, 2), np.int32))
+    b = BatchRTM(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, mu, lam, cfg.rho, cfg.fw, shots,
+                 isnap=1, imaging="snapshot")
+    with pytest.raises(ValueError, match="same wave systems"):
+        b.run()
