@@ -151,6 +151,23 @@ def test_short_records(kernel):
         assert_bitwise(h, o)
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_sponge_edge_cases(kernel):
+    """The sponge the step kernels evaluate from the cell index, where the
+    reference's strips meet unusually (forward_modeling.py:203-241): no
+    sponge (absorbing_frame 0), a frame wider than half the grid (the right
+    strip overwrites the left, nx < 2 FW), and a frame equal to the depth."""
+    _case("sponge edge cases kernel=%s" % kernel)
+    for nx, nz, fw in ((60, 50, 0), (30, 50, 20), (40, 20, 20)):
+        cfg = configs.psv_anomaly(nx=nx, nz=nz, nt=70, fw=fw, radius=5, src=(nx // 2, 5),
+                                  rec_range=(2, nx - 2), f0=40.0)
+        kw = cfg.kwargs(cfg.shots[0])
+        wav = cfg.wavelets(cfg.shots[0])
+        h = run_hip(kw, wav, isnap=2, kernel=kernel)
+        o = run_oracle(kw, wav, isnap=2)
+        assert_bitwise(h, o)
+
+
 def test_mxz_overflow_reproduced():
     """SI rock moduli overflow mu^4 in float32 (forward_modeling.py:271-276):
     the reference's shear stress turns NaN; the HIP path reports the same code."""
