@@ -538,11 +538,15 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
 static const BlockCfg kCfgSH[] = {{16, 64, 256, 4},   {16, 64, 256, 8},    {16, 128, 512, 4},
                                   {16, 128, 512, 8},  {32, 128, 1024, 4},  {32, 128, 1024, 8},
                                   {24, 128, 1024, 8}, {32, 96, 1024, 8},   {48, 64, 1024, 8},
-                                  {32, 48, 512, 8},   {24, 64, 512, 8}};
+                                  {32, 48, 512, 8},   {24, 64, 512, 8},    {16, 32, 1024, 4},
+                                  {16, 32, 1024, 8}};
 static const BlockCfg kCfgPSV[] = {{16, 64, 256, 2},  {16, 64, 256, 4},    {16, 128, 512, 4},
                                    {16, 128, 512, 8}, {32, 128, 1024, 4},  {48, 64, 1024, 4},
-                                   {32, 64, 512, 4}};
-static const BlockCfg kCfgBoth[] = {{16, 64, 256, 2}, {16, 64, 256, 4}, {16, 128, 512, 4}};
+                                   {32, 64, 512, 4},  {16, 32, 1024, 4},   {16, 32, 1024, 8}};
+static const BlockCfg kCfgBoth[] = {{16, 64, 256, 2}, {16, 64, 256, 4}, {16, 128, 512, 4},
+                                    {16, 32, 1024, 4}};
+// (the 16 x 32 tiles with 1024 threads give every thread one or two region
+// cells: the shortest step on grids too small to fill the GPU)
 
 static void cfg_list(int systems, const BlockCfg** list, int* n) {
     if (systems == TRTM_SH) {
@@ -614,9 +618,18 @@ static BlockLayout block_layout(const trtm_step_desc* d, BlockCfg c) {
     const size_t n = (size_t)d->nx * d->nz;
     L.tiles_x = (d->nx + c.tx - 1) / c.tx;
     L.tiles_z = (d->nz + c.tz - 1) / c.tz;
-    // LDS-tiled kernel: lists per tile, tiles no smaller than 16 x 64; a cell
-    // lies in at most 2 x 2 regions when 2K <= min(TX, TZ)
-    const size_t tiles_max = (size_t)((d->nx + 15) / 16) * ((d->nz + 63) / 64);
+    // LDS-tiled kernel: lists per tile, sized for the compiled shape with the
+    // most tiles on this grid; a cell lies in at most 2 x 2 regions when
+    // 2K <= min(TX, TZ)
+    size_t tiles_max = 0;
+    for (const BlockCfg* lst : {kCfgSH, kCfgPSV, kCfgBoth}) {
+        const int cnt = lst == kCfgSH ? (int)(sizeof(kCfgSH) / sizeof(BlockCfg))
+                        : lst == kCfgPSV ? (int)(sizeof(kCfgPSV) / sizeof(BlockCfg))
+                                         : (int)(sizeof(kCfgBoth) / sizeof(BlockCfg));
+        for (int k = 0; k < cnt; k++)
+            tiles_max = std::max(tiles_max, (size_t)((d->nx + lst[k].tx - 1) / lst[k].tx) *
+                                                ((d->nz + lst[k].tz - 1) / lst[k].tz));
+    }
     size_t ib = tiles_max + 1, rb = tiles_max + 1;
     size_t ii = 4 * (size_t)std::max(d->n_inj_groups, 0) * sizeof(int);
     size_t ri = (size_t)std::max(d->n_rec, 0) * sizeof(int);
@@ -892,15 +905,17 @@ static bool launch_sys(BlockCfg c, const KArgs& m, const ShotDev* shots, int n_s
     }
     if constexpr (PSV && SH) {
         TRTM_CFG(16, 64, 256, 2) TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 128, 512, 4)
+        TRTM_CFG(16, 32, 1024, 4)
     } else if constexpr (PSV) {
         TRTM_CFG(16, 64, 256, 2) TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 128, 512, 4)
         TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4) TRTM_CFG(48, 64, 1024, 4)
-        TRTM_CFG(32, 64, 512, 4)
+        TRTM_CFG(32, 64, 512, 4) TRTM_CFG(16, 32, 1024, 4) TRTM_CFG(16, 32, 1024, 8)
     } else {
         TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 64, 256, 8) TRTM_CFG(16, 128, 512, 4)
         TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4) TRTM_CFG(32, 128, 1024, 8)
         TRTM_CFG(24, 128, 1024, 8) TRTM_CFG(32, 96, 1024, 8) TRTM_CFG(48, 64, 1024, 8)
-        TRTM_CFG(32, 48, 512, 8) TRTM_CFG(24, 64, 512, 8)
+        TRTM_CFG(32, 48, 512, 8) TRTM_CFG(24, 64, 512, 8) TRTM_CFG(16, 32, 1024, 4)
+        TRTM_CFG(16, 32, 1024, 8)
     }
 #undef TRTM_CFG
     return false;
