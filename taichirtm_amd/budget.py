@@ -134,7 +134,8 @@ class GradientAndImage:
             fwi = ElasticFWI(self.nx, self.nz, self.dx, self.dz, self.fs, self.fw,
                              [ShotSpec(s.src_loc, s.wavelets, s.rec_loc) for s in group],
                              precision="float32", checkpoint_every=p.L_fwi,
-                             kernel=self.kernel if self.kernel in ("tuned", "blocked") else "auto")
+                             kernel=self.kernel if self.kernel in ("tuned", "blocked") else "auto",
+                             history_bytes=0)  # the plan holds checkpoints + one segment
             recs = fwi.forward(*m)
             res = []
             for r, s in zip(recs, group):
@@ -197,6 +198,6 @@ def observed_data(cfg, shots: List[ShotSpec], group=4):
     for g0 in range(0, len(shots), group):
         sub = shots[g0:g0 + group]
         p = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, sub, precision="float32",
-                       checkpoint_every=max(s.nt for s in sub))
+                       checkpoint_every=max(s.nt for s in sub), history_bytes=0)
         out += [r.cpu().numpy() for r in p.forward(*m)]
     return out

@@ -6,9 +6,11 @@ back-propagator, not the adjoint of its forward step.  Here:
 
 - ``ElasticFWI.forward(vp, vs, rho)`` runs every shot with the forward scheme
   of forward_modeling.py:539-551 (same expressions) and keeps the full state
-  every ``checkpoint_every`` steps; with ``history_bytes`` it also writes the
-  history (the stresses and velocities the updates read) of as many trailing
-  segments as fit that many bytes, which then need no checkpoint;
+  every ``checkpoint_every`` steps; it also writes the history (the stresses
+  and velocities the updates read) of as many trailing segments as fit
+  ``history_bytes`` (default ``"auto"``: half the free device memory when the
+  forward pass starts; 0 recomputes every segment), which then need no
+  checkpoint;
 - ``ElasticFWI.gradient(data_adjoint)`` walks the segments last to first,
   takes each segment's history from the forward pass or recomputes it from the
   segment's checkpoint, runs the adjoint steps over it and accumulates the gradient in
@@ -93,7 +95,7 @@ class ElasticFWI:
 
     def __init__(self, nx, nz, dx, dz, fs, absorbing_frame, shots: Sequence[ShotSpec],
                  precision="float32", checkpoint_every=None, kernel="auto", tile_x=0, tile_z=0,
-                 block_steps=0, history_bytes=0):
+                 block_steps=0, history_bytes="auto"):
         if precision not in _REAL:
             raise ValueError("precision must be float32 or float64")
         if kernel not in _KERNELS:
@@ -154,7 +156,9 @@ class ElasticFWI:
         self.checkpoint_every = checkpoint_every
         # device memory for history written by the forward pass (the trailing
         # segments that fit skip their recompute in ``gradient``)
-        self.history_bytes = int(history_bytes)
+        if history_bytes != "auto" and int(history_bytes) < 0:
+            raise ValueError("history_bytes must be 'auto' or >= 0")
+        self.history_bytes = history_bytes if history_bytes == "auto" else int(history_bytes)
         self._ck = None
         self._kept = []
         self.kept_segments = 0
@@ -265,7 +269,11 @@ class ElasticFWI:
         S = len(self.geo)
         # trailing segments whose history the forward pass keeps
         seg_bytes = self.L * n_hist * self.nx * self.nz * self.rb * S
-        self.kept_segments = min(nseg, self.history_bytes // seg_bytes)
+        budget = self.history_bytes
+        if budget == "auto":
+            free, _ = torch.cuda.mem_get_info()
+            budget = free // 2
+        self.kept_segments = min(nseg, int(budget) // seg_bytes)
         first_kept = nseg - self.kept_segments
         self._kept = []
         self._ck = None
