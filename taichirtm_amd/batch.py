@@ -19,10 +19,11 @@ Two imaging modes give the same image bit for bit:
   the step kernel.  Memory: nt/L states + L/isnap wavefields per shot, with L
   from a memory budget (about sqrt(nt) by default); cost: one extra forward
   sweep.  Recomputation is deterministic, so the wavefields, and the image,
-  are the snapshot mode's exactly.  With ``snapshot_bytes`` the forward pass
-  writes the segment buffers of as many trailing segments as fit that many
-  bytes (only shots still running in a segment count); those segments need
-  neither checkpoint nor recompute.
+  are the snapshot mode's exactly.  The forward pass also writes the segment
+  buffers of as many trailing segments as fit ``snapshot_bytes`` (default
+  ``"auto"``: half the free device memory when the forward pass starts; only
+  shots still running in a segment count); those segments need neither
+  checkpoint nor recompute.
 """
 from __future__ import annotations
 
@@ -101,7 +102,7 @@ class BatchRTM:
     def __init__(self, nx, nz, dx, dz, fs, mu_field, lam_field, rho_field, absorbing_frame,
                  shots: List[ShotSpec], isnap=1, imaging="snapshot", checkpoint_every=None,
                  stability_check_interval=100, surface_matrix_field=None, tile_x=0, tile_z=0,
-                 block_steps=0, kernel="blocked", snapshot_bytes=0, graph=False):
+                 block_steps=0, kernel="blocked", snapshot_bytes="auto", graph=False):
         if imaging not in ("snapshot", "checkpoint"):
             raise ValueError("imaging must be 'snapshot' or 'checkpoint'")
         if kernel not in ("blocked", "wave", "tuned"):
@@ -119,7 +120,9 @@ class BatchRTM:
         self.block = dict(tile_x=tile_x, tile_z=tile_z, block_steps=block_steps, kernel=kernel)
         # checkpoint mode: device memory for segment snapshots written by the
         # forward pass (the trailing segments that fit need no recompute)
-        self.snapshot_bytes = int(snapshot_bytes)
+        if snapshot_bytes != "auto" and int(snapshot_bytes) < 0:
+            raise ValueError("snapshot_bytes must be 'auto' or >= 0")
+        self.snapshot_bytes = snapshot_bytes if snapshot_bytes == "auto" else int(snapshot_bytes)
         self.kept = {}
         self.first_kept = None
         # every batched step range recorded into a HIP graph and launched as
@@ -247,7 +250,10 @@ class BatchRTM:
         comps = [(c, bit) for c, bit in (("u", _lib.PSV), ("v", _lib.SH), ("w", _lib.PSV))
                  if self.systems & bit]
         seg_shot_bytes = nslot * len(comps) * self.nx * self.nz * 4
-        left, first_kept = self.snapshot_bytes, nseg
+        left = self.snapshot_bytes
+        if left == "auto":
+            left = torch.cuda.mem_get_info()[0] // 2
+        first_kept = nseg
         for k in range(nseg - 1, -1, -1):
             need = seg_shot_bytes * sum(1 for f in self.fw if self.edges[k] < f.nt)
             if need > left:

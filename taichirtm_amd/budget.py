@@ -162,7 +162,7 @@ class GradientAndImage:
             if not self.share_forward:
                 rtm = BatchRTM(self.nx, self.nz, self.dx, self.dz, self.fs, mu, lam, rho_n,
                                self.fw, group, isnap=self.isnap, imaging="checkpoint",
-                               checkpoint_every=p.L_rtm, kernel=rkernel)
+                               checkpoint_every=p.L_rtm, kernel=rkernel, snapshot_bytes=0)
                 codes = rtm.run()
             if any(codes):
                 raise RuntimeError("non-finite wavefield, codes %s" % codes)

@@ -50,7 +50,7 @@ def test_batch_equals_single_shots(system, imaging, isnap, kernel):
     _case("batch %s %s isnap=%d kernel=%s" % (system, imaging, isnap, kernel))
     cfg, mu, lam, shots = survey(system)
     b = BatchRTM(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, mu, lam, cfg.rho, cfg.fw, shots,
-                 isnap=isnap, imaging=imaging, kernel=kernel,
+                 isnap=isnap, imaging=imaging, kernel=kernel, snapshot_bytes=0,
                  checkpoint_every=(12 if isnap == 3 else 16) if imaging == "checkpoint" else None)
     assert b.run() == [0] * len(shots)
     imgs = b.images()
