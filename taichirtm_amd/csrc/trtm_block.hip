@@ -21,6 +21,7 @@
 // The grid's second dimension is the shot: shots share the medium and differ
 // in state, record length, injection and recording geometry (ShotDev).
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -773,6 +774,30 @@ __global__ __launch_bounds__(1024) void k_block_lists_one(int tx, int tz, int K,
     pass(1);
 }
 
+// Host tables to device memory without waiting: each launch carries up to
+// 3.5 KB of the table in its kernel arguments (copied at launch), so the host
+// buffer may be released as soon as the launches are issued.
+struct UploadChunk {
+    unsigned int w[896];
+};
+
+__global__ void k_upload(UploadChunk p, int nwords, unsigned int* dst) {
+    for (int i = threadIdx.x; i < nwords; i += blockDim.x) dst[i] = p.w[i];
+}
+
+static hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    const size_t words = bytes / 4;  // tables are whole 32-bit words
+    const unsigned int* from = static_cast<const unsigned int*>(src);
+    for (size_t off = 0; off < words; off += 896) {
+        UploadChunk c;
+        const size_t n = std::min<size_t>(896, words - off);
+        std::memcpy(c.w, from + off, n * 4);
+        hipLaunchKernelGGL(k_upload, dim3(1), dim3(256), 0, s, c, (int)n,
+                           static_cast<unsigned int*>(dst) + off);
+    }
+    return hipGetLastError();
+}
+
 static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg, int it_lo,
                       int it_hi, hipStream_t s, ShotDev* sd) {
     const int tx = cfg.tx, K = cfg.k;
@@ -874,8 +899,7 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
     x.isnaps = nullptr;
     if (!fwd && x.num_snaps > 0) {
         int* dev = reinterpret_cast<int*>(ws + L.isnaps);
-        e = hipMemcpyAsync(dev, d->isnaps_host, sizeof(int) * d->num_snaps,
-                           hipMemcpyHostToDevice, s);
+        e = upload_async(dev, d->isnaps_host, sizeof(int) * d->num_snaps, s);
         if (e != hipSuccess) return fail(TRTM_ERR_HIP, "isnaps copy: %s", hipGetErrorString(e));
         x.isnaps = dev;
     }
@@ -954,7 +978,8 @@ struct BlockRun {
 // global steps.  State ping-pongs between the caller's buffers and the shot's
 // workspace, with each shot's phase set by the first launch it takes part in.
 // shots_dev: device room for 2n ShotDev entries.  Builds the tile lists and
-// writes the shot tables on stream s, then waits for s.
+// writes the shot tables on stream s (the tables travel in kernel arguments,
+// so nothing waits for the device).
 static int block_prepare(const trtm_step_desc* descs, int n, bool fwd, bool save,
                          int32_t it_begin, int32_t it_end, ShotDev* shots_dev, const KArgs& m,
                          hipStream_t s, BlockRun* r) {
@@ -985,12 +1010,9 @@ static int block_prepare(const trtm_step_desc* descs, int n, bool fwd, bool save
         if (first[q] > 0 && first[q] % 2 == 1) std::swap(even[q], odd[q]);
     }
     const BlockLayout L = block_layout(d0, cfg);
-    hipError_t e = hipMemcpyAsync(shots_dev, even.data(), sizeof(ShotDev) * n,
-                                  hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(shots_dev + n, odd.data(), sizeof(ShotDev) * n, hipMemcpyHostToDevice,
-                           s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);  // host tables must outlive the copies
+    static_assert(sizeof(ShotDev) % 4 == 0, "shot table of whole words");
+    hipError_t e = upload_async(shots_dev, even.data(), sizeof(ShotDev) * n, s);
+    if (e == hipSuccess) e = upload_async(shots_dev + n, odd.data(), sizeof(ShotDev) * n, s);
     if (e != hipSuccess) return fail(TRTM_ERR_HIP, "shot table copy: %s", hipGetErrorString(e));
 
     r->cfg = cfg;
@@ -1059,6 +1081,10 @@ int block_capture(const trtm_step_desc* descs, int n, bool fwd, bool save, int32
     BlockRun r;
     int rc = block_prepare(descs, n, fwd, save, it_begin, it_end, shots_dev, m, s, &r);
     if (rc) return rc;
+    // the lists and tables are in place before the plan can be launched on
+    // any stream
+    hipError_t w = hipStreamSynchronize(s);
+    if (w != hipSuccess) return fail(TRTM_ERR_HIP, "plan preparation: %s", hipGetErrorString(w));
     hipStream_t cap;
     hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(TRTM_ERR_HIP, "capture stream: %s", hipGetErrorString(e));
