@@ -960,8 +960,9 @@ static bool launch_block(bool fwd, int systems, BlockCfg c, const KArgs& m, cons
 }
 
 // The launches of one call, prepared: per-shot tile lists and the two
-// shot-table halves already in device memory, so issuing them touches no host
-// memory (and can be captured into a graph).
+// shot-table halves are written to device memory by work queued on the stream
+// before them, so issuing them touches no host memory (and can be captured
+// into a graph).
 struct BlockRun {
     BlockCfg cfg;
     int n = 0, systems = 0, it_begin = 0, tiles_x = 0, tiles_z = 0;
