@@ -13,7 +13,9 @@ value  = useful cell-updates / second (2 * nx * nz * nt per shot: forward and
 roofline: the dominant kernel is the backward step with imaging; its average
          launch time is measured with HIP events on the launch stream over the
          timed steps; algorithmic bytes per cell per launch are listed in
-         ALGO_BYTES.
+         ALGO_BYTES.  frac is against the 8 TB/s HBM peak; frac_of_copy
+         against copy_gbs, the bandwidth of a 1 GiB device-to-device copy
+         measured in the same process (read + write bytes).
 f64_rel_l2: the timed run's forward seismograms against the same scheme in
          double (relative L2), computed after the timed region.
 cpu_baseline: the oracle (C/OpenMP restatement of the reference's unfused
@@ -104,6 +106,7 @@ def job_main(args):
     whole job over --shots shots."""
     import torch
     from taichirtm_amd import autotune, configs
+    copy_gbs = copy_bandwidth()  # before the job holds its memory budget
     # defaults: the full survey of each workload and a memory budget that
     # fits one MI355X beside it
     full = {"psv_batch": (64, 16, 200.0), "fwi": (16, 0, 200.0), "salt": (32, 0, 128.0)}
@@ -216,6 +219,8 @@ def job_main(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_update": JOB_BYTES[args.workload]},
     }
+    out["roofline"]["copy_gbs"] = round(copy_gbs, 1)
+    out["roofline"]["frac_of_copy"] = round(achieved / copy_gbs, 4)
     if not args.no_cpu_baseline:
         # the oracle's RTM of the first shot on a shorter sample (it keeps every
         # snapshot in host memory); per-cell rate of the CPU path on this grid
@@ -445,10 +450,34 @@ def main():
                      "forward_launch_us": round(fw_launch_us, 3),
                      "forward_achieved": round(f_bytes / (fw_launch_us * 1e-6) / 1e9, 1)},
     }
+    copy_gbs = copy_bandwidth()
+    out["roofline"]["copy_gbs"] = round(copy_gbs, 1)
+    out["roofline"]["frac_of_copy"] = round(achieved / copy_gbs, 4)
     out["f64_rel_l2"] = float64_error(cfg, shot)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=20):
+    """Sustained HBM bandwidth of a device-to-device copy of nbytes (read +
+    write counted), median over reps, timed with events on the current
+    stream, outside the timed region: the measured ceiling the roofline
+    fraction is also quoted against."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    del a, b
+    return 2.0 * nbytes / (float(np.median(ms)) * 1e-3) / 1e9
 
 
 def float64_error(cfg, shot):
