@@ -885,10 +885,14 @@ static int fwi_block_run(const trtm_fwi_desc* d, int n, int it_begin, int it_end
         chunks.insert(chunks.begin() + 1, adjoint ? std::make_pair(first.first, cut)
                                                   : std::make_pair(cut, first.second));
     }
-    for (const auto& ch : chunks) {
-        const int t_a = ch.first, t_b = ch.second;
-        for (int g = 0; g < groups; g++) {
-            const int q0 = g * kShotsPerLaunch, m = std::min(kShotsPerLaunch, n - q0);
+    // Each group runs the whole range before the next one starts: a launch
+    // then reads the state that the group's previous launch has just written,
+    // while it is still in the last-level cache (two 8-shot groups of state
+    // and history between them would not fit).
+    for (int g = 0; g < groups; g++) {
+        const int q0 = g * kShotsPerLaunch, m = std::min(kShotsPerLaunch, n - q0);
+        for (const auto& ch : chunks) {
+            const int t_a = ch.first, t_b = ch.second;
             if (n == 1) {
                 rc = launch_sys(d->systems, c, adjoint, hist, ByValue<FB>{fb[0]}, 1, t_a, t_b,
                                 d->nx, d->nz, s);
