@@ -205,6 +205,59 @@ def test_blocked_kernels_bitwise_equal_step_kernels(system, shape):
         assert torch.equal(got[2][key], ref[2][key]), key
 
 
+INNER_SHAPES = [(16, 48, 4), (24, 32, 4), (32, 64, 4), (16, 64, 8)]
+
+
+def _inner_tiles(nx, nz, fw, shape, points):
+    """Tiles whose K-cell region lies inside the grid interior, outside the
+    sponge strips and holds none of the given source / receiver cells (no
+    per-cell mask, sponge factor or injection applies anywhere in them)."""
+    tx, tz, k = shape
+    n = 0
+    for i0 in range(0, nx, tx):
+        for j0 in range(0, nz, tz):
+            lo_i, hi_i, lo_j, hi_j = i0 - k, i0 + tx + k - 1, j0 - k, j0 + tz + k - 1
+            if lo_i < max(1, fw) or hi_i > min(nx - 2, nx - 1 - fw):
+                continue
+            if lo_j < 1 or hi_j > min(nz - 2, nz - 1 - fw):
+                continue
+            if any(lo_i <= i <= hi_i and lo_j <= j <= hi_j for i, j in points):
+                continue
+            n += 1
+    return n
+
+
+@pytest.mark.parametrize("shape", INNER_SHAPES, ids=lambda s: "%dx%dk%d" % s)
+@pytest.mark.parametrize("system", ["psv", "sh"])
+def test_blocked_inner_tiles_bitwise_equal_step_kernels(system, shape):
+    """A grid large enough for tiles away from the edges, the sponge, the
+    sources and the receivers, with a source and a receiver line inside the
+    grid and the model difference between them, so that the forward and
+    adjoint wavefields cross those tiles: records and gradients of K steps
+    per launch equal one step per launch bit for bit."""
+    _case("FWI blocked inner tiles %s %s" % (system, shape))
+    nx, nz, nt = 200, 150, 500
+    grid, shots, m0, mt = problem(system, nx=nx, nz=nz, nt=nt, n_shots=2)
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    blob = np.exp(-(((ii - 85) / 5.0) ** 2 + ((jj - 75) / 5.0) ** 2))
+    mt = (m0[0] * (1 + 0.08 * blob), m0[1] * (1 - 0.06 * blob), m0[2])
+    rec = np.stack([np.arange(20, 70), np.full(50, 60)], 1).astype(np.int32)
+    shots = [ShotSpec(np.array([[50, 70]], np.int32), shots[0].wavelets, rec),
+             ShotSpec(np.array([[150, 110]], np.int32), shots[1].wavelets, rec[10:40])]
+    assert _inner_tiles(nx, nz, grid[5], shape, [(50, 70)] + [tuple(r) for r in rec]) > 0
+    tx, tz, k = shape
+    ref = _records_and_gradient(grid, shots, m0, mt, kernel="step", checkpoint_every=60)
+    got = _records_and_gradient(grid, shots, m0, mt, kernel="blocked", tile_x=tx, tile_z=tz,
+                                block_steps=k, checkpoint_every=60)
+    assert max(float(x.abs().max()) for x in ref[1]) > 0
+    for a, b in zip(got[0], ref[0]):
+        assert torch.equal(a, b)
+    for a, b in zip(got[1], ref[1]):
+        assert torch.equal(a, b)
+    for key in ref[2]:
+        assert torch.equal(got[2][key], ref[2][key]), key
+
+
 def test_blocked_kernels_many_shots_bitwise():
     """Ten shots: two launches of up to eight shots per step block."""
     _case("FWI blocked 10 shots")
