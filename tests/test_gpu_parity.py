@@ -317,10 +317,11 @@ def test_wave_configurations_bitwise(system, cfg):
     assert_bitwise(h, o)
 
 
-@pytest.mark.parametrize("workload", ["sh_layered", "psv_faulted"])
+@pytest.mark.parametrize("workload", ["sh_layered", "psv_faulted", "psv_salt"])
 def test_full_size_forward_error_against_float64(workload):
     """The float32 scheme's error against its float64 run at full size
-    (SH 1200x600x4000; the first shot of the 2000x1000 P-SV survey): the
+    (SH 1200x600x4000; the first shot of the 2000x1000 P-SV survey; the first
+    shot of the 4000x2000x8000 salt model, snapshots every 100 steps): the
     forward seismograms of ForwardModeling (blocked kernels) against the FWI
     forward step run in double, which is the same scheme (its float32 records
     equal ForwardModeling's bit for bit)."""
@@ -328,11 +329,17 @@ def test_full_size_forward_error_against_float64(workload):
     from taichirtm_amd.fwi import ElasticFWI
     import torch
     _case("float64 error %s" % workload)
-    cfg = configs.sh_layered() if workload == "sh_layered" else configs.psv_faulted(n_shots=1)
+    if workload == "sh_layered":
+        cfg = configs.sh_layered()
+    elif workload == "psv_faulted":
+        cfg = configs.psv_faulted(n_shots=1)
+    else:
+        cfg = configs.psv_salt(n_shots=1)[0]
     shot = cfg.shots[0]
     kw = cfg.kwargs(shot)
     wav = cfg.wavelets(shot)
-    h = run_hip(kw, wav, isnap=10, kernel="auto", backward=False)
+    h = run_hip(kw, wav, isnap=100 if workload == "psv_salt" else 10, kernel="auto",
+                backward=False)
     spec = ShotSpec(np.array([shot.src], np.int32), tuple(np.asarray(w, np.float64) for w in wav),
                     np.asarray(shot.rec, np.int32))
     p = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, [spec], precision="float64",
