@@ -110,6 +110,56 @@ def test_float32_gradient_close_to_float64():
         assert rel < 1e-3
 
 
+@pytest.mark.parametrize("workload", ["fwi", "salt"])
+def test_full_size_gradient_error_against_float64(workload):
+    """The float32 path against the float64 run of the same scheme at full
+    size, one shot: the first of the 600x300x2500 FWI problem, and the middle
+    one of the 4000x2000x8000 salt survey (the first shot's records never
+    reach the salt body, so its residual is zero), observed data from the true model, gradient in the starting
+    model.  Pinned: the records (1e-4) and the float32 adjoint given the
+    float64 residual (1e-4).  Stated: the residual's own float32 error and the
+    end-to-end gradient error it causes (the residual is a small difference
+    of two large record sets, so float32 rounding of the forward runs, ~1e-6
+    of the records, is a large part of it)."""
+    _case("FWI full-size float32 vs float64 %s" % workload)
+    true, start = configs.psv_fwi(n_shots=1) if workload == "fwi" else configs.psv_salt(n_shots=32)
+    s = true.shots[len(true.shots) // 2]  # salt: a shot above the salt body
+    size = "%dx%dx%d" % (true.nx, true.nz, s.nt)
+
+    def rel(a, b):
+        return float(torch.linalg.norm((a.double() - b.double()).ravel())
+                     / torch.linalg.norm(b.double().ravel()))
+
+    run = {}
+    for prec, dtype, npt in (("float64", torch.float64, np.float64),
+                             ("float32", torch.float32, np.float32)):
+        shot = ShotSpec(np.array([s.src], np.int32),
+                        tuple(np.asarray(w, np.float32).astype(npt) for w in true.wavelets(s)),
+                        np.asarray(s.rec, np.int32))
+        p = ElasticFWI(true.nx, true.nz, true.dx, true.dz, true.fs, true.fw, [shot],
+                       precision=prec, history_bytes=0)
+        obs = p.forward(*[_t(x, dtype) for x in (true.vp, true.vs, true.rho)])[0].clone()
+        syn = p.forward(*[_t(x, dtype) for x in (start.vp, start.vs, start.rho)])[0].clone()
+        run[prec] = (p, obs, syn, syn - obs)
+    p64, obs64, syn64, res64 = run["float64"]
+    p32, obs32, syn32, res32 = run["float32"]
+    e_rec = max(rel(obs32, obs64), rel(syn32, syn64))
+    e_res = rel(res32, res64)
+    print("[fp32 vs fp64] %s %s records %.2e, residual %.2e (|residual| / |records| %.2e)"
+          % (workload, size, e_rec, e_res, float(res64.norm() / obs64.norm())),
+          flush=True)
+    g64 = [x.double().clone() for x in p64.gradient([res64])]
+    g32 = [x.double().clone() for x in p32.gradient([res32])]
+    p32.forward(*[_t(x, torch.float32) for x in (start.vp, start.vs, start.rho)])
+    g32s = [x.double().clone() for x in p32.gradient([res64.float()])]
+    assert e_rec < 1e-4
+    for name, a, b, c in zip(("vp", "vs", "rho"), g32, g64, g32s):
+        same_res, end = rel(c, b), rel(a, b)
+        print("[fp32 vs fp64] %s %s gradient %s: float64 residual %.2e, end to end %.2e"
+              % (workload, size, name, same_res, end), flush=True)
+        assert same_res < 1e-4
+
+
 def test_gradient_independent_of_checkpoint_interval():
     _case("FWI checkpoint interval invariance")
     grid, shots, m0, mt = problem("psv", nt=150)
