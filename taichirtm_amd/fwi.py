@@ -38,7 +38,7 @@ from __future__ import annotations
 
 import ctypes
 import math
-from typing import List, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 import torch
@@ -437,10 +437,27 @@ class Seismograms(torch.autograd.Function):
         return (g_vp.to(ctx.dtypes[0]), g_vs.to(ctx.dtypes[1]), g_rho.to(ctx.dtypes[2]), None)
 
 
-def l2_misfit_and_gradient(problem: ElasticFWI, vp, vs, rho, observed):
+def l2_misfit_and_gradient(problem: ElasticFWI, vp, vs, rho, observed,
+                           records_from: Optional[ElasticFWI] = None):
     """J = 1/2 sum (d - d_obs)^2 over every shot, component, receiver and
-    sample, and its gradient (vp, vs, rho) in float64."""
-    recs = problem.forward(vp, vs, rho)
-    res = [r - torch.as_tensor(o, device=r.device).to(r.dtype) for r, o in zip(recs, observed)]
-    J = 0.5 * sum(float((x.double() ** 2).sum()) for x in res)
-    return J, problem.gradient(res)
+    sample, and its gradient (vp, vs, rho) in float64.
+
+    records_from: a float64 ElasticFWI of the same shots.  Its forward pass
+    gives the records, so the residual is formed in double and rounded to
+    ``problem``'s precision once; ``problem`` still runs its own forward pass
+    for the adjoint's checkpoints and history, and the adjoint.  A float32
+    problem then keeps its adjoint's accuracy where the misfit is so small
+    that float32 rounding of the records would dominate the residual."""
+    if records_from is None:
+        recs = problem.forward(vp, vs, rho)
+        res = [r - torch.as_tensor(o, device=r.device).to(r.dtype) for r, o in zip(recs, observed)]
+        J = 0.5 * sum(float((x.double() ** 2).sum()) for x in res)
+        return J, problem.gradient(res)
+    if records_from.dtype != torch.float64:
+        raise ValueError("records_from must be a float64 problem")
+    recs = records_from.forward(*(torch.as_tensor(x).to(torch.float64) for x in (vp, vs, rho)))
+    res64 = [r - torch.as_tensor(o, device=r.device).to(torch.float64)
+             for r, o in zip(recs, observed)]
+    problem.forward(vp, vs, rho)
+    J = 0.5 * sum(float((x ** 2).sum()) for x in res64)
+    return J, problem.gradient([x.to(problem.dtype) for x in res64])

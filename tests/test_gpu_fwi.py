@@ -115,12 +115,15 @@ def test_full_size_gradient_error_against_float64(workload):
     """The float32 path against the float64 run of the same scheme at full
     size, one shot: the first of the 600x300x2500 FWI problem, and the middle
     one of the 4000x2000x8000 salt survey (the first shot's records never
-    reach the salt body, so its residual is zero), observed data from the true model, gradient in the starting
-    model.  Pinned: the records (1e-4) and the float32 adjoint given the
-    float64 residual (1e-4).  Stated: the residual's own float32 error and the
-    end-to-end gradient error it causes (the residual is a small difference
-    of two large record sets, so float32 rounding of the forward runs, ~1e-6
-    of the records, is a large part of it)."""
+    reach the salt body, so its residual is zero).  Observed data come from
+    the true model, misfit and gradient from the starting model.
+
+    Pinned: the records (1e-4) and the float32 adjoint given the float64
+    residual (1e-4; l2_misfit_and_gradient with records_from).  Stated: the
+    residual's own float32 error and the end-to-end gradient error it causes
+    (the residual is a small difference of two large record sets, so float32
+    rounding of the forward runs, ~1e-6 of the records, is a large part of
+    it)."""
     _case("FWI full-size float32 vs float64 %s" % workload)
     true, start = configs.psv_fwi(n_shots=1) if workload == "fwi" else configs.psv_salt(n_shots=32)
     s = true.shots[len(true.shots) // 2]  # salt: a shot above the salt body
@@ -150,8 +153,10 @@ def test_full_size_gradient_error_against_float64(workload):
           flush=True)
     g64 = [x.double().clone() for x in p64.gradient([res64])]
     g32 = [x.double().clone() for x in p32.gradient([res32])]
-    p32.forward(*[_t(x, torch.float32) for x in (start.vp, start.vs, start.rho)])
-    g32s = [x.double().clone() for x in p32.gradient([res64.float()])]
+    # float32 adjoint, residual formed from float64 records (rounded once)
+    g32s = [x.double().clone() for x in l2_misfit_and_gradient(
+        p32, *[_t(x, torch.float32) for x in (start.vp, start.vs, start.rho)], [obs64],
+        records_from=p64)[1]]
     assert e_rec < 1e-4
     for name, a, b, c in zip(("vp", "vs", "rho"), g32, g64, g32s):
         same_res, end = rel(c, b), rel(a, b)
