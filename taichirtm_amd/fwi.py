@@ -420,21 +420,35 @@ class ElasticFWI:
 
 class Seismograms(torch.autograd.Function):
     """(vp, vs, rho) -> records (n_shots, 3, nr, nt); all shots need the same
-    nr and nt.  Backward: the exact adjoint gradient."""
+    nr and nt.  Backward: the exact adjoint gradient.
+
+    records_from (optional): a float64 ElasticFWI of the same shots.  The
+    records returned are then its float64 ones, so a loss formed from them
+    (a residual) carries no float32 rounding of the forward runs; ``problem``
+    still runs its forward pass for the adjoint's checkpoints and history,
+    and backward runs ``problem``'s adjoint on the incoming gradient rounded
+    to its precision."""
 
     @staticmethod
-    def forward(ctx, vp, vs, rho, problem: ElasticFWI):
+    def forward(ctx, vp, vs, rho, problem: ElasticFWI, records_from=None):
         recs = problem.forward(vp.detach(), vs.detach(), rho.detach())
+        out_dtype = vp.dtype
+        if records_from is not None:
+            if records_from.dtype != torch.float64:
+                raise ValueError("records_from must be a float64 problem")
+            recs = records_from.forward(*(x.detach().to(torch.float64) for x in (vp, vs, rho)))
+            out_dtype = torch.float64
         if len({tuple(r.shape) for r in recs}) != 1:
             raise ValueError("autograd records need equal receiver counts and record lengths")
         ctx.problem = problem
         ctx.dtypes = (vp.dtype, vs.dtype, rho.dtype)
-        return torch.stack(recs).to(vp.dtype)
+        return torch.stack(recs).to(out_dtype)
 
     @staticmethod
     def backward(ctx, grad):
         g_vp, g_vs, g_rho = ctx.problem.gradient([grad[q] for q in range(grad.shape[0])])
-        return (g_vp.to(ctx.dtypes[0]), g_vs.to(ctx.dtypes[1]), g_rho.to(ctx.dtypes[2]), None)
+        return (g_vp.to(ctx.dtypes[0]), g_vs.to(ctx.dtypes[1]), g_rho.to(ctx.dtypes[2]), None,
+                None)
 
 
 def l2_misfit_and_gradient(problem: ElasticFWI, vp, vs, rho, observed,
