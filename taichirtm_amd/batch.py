@@ -23,7 +23,8 @@ Two imaging modes give the same image bit for bit:
   buffers of as many trailing segments as fit ``snapshot_bytes`` (default
   ``"auto"``: half the free device memory when the forward pass starts; only
   shots still running in a segment count); those segments need neither
-  checkpoint nor recompute.
+  checkpoint nor recompute.  Either budget is capped at the free memory less
+  what the checkpoints and the backward pass's recompute buffers still need.
 """
 from __future__ import annotations
 
@@ -38,7 +39,7 @@ import torch
 from . import _lib
 from ._propagator import Medium, PSV_STATE, SH_STATE, _ptr
 from .backward_modeling import BackwardModeling
-from .field import SnapshotField, ZeroField, as_host, device, stream_handle
+from .field import SnapshotField, ZeroField, as_host, device, free_device_bytes, stream_handle
 from .forward_modeling import ForwardModeling
 
 
@@ -250,9 +251,14 @@ class BatchRTM:
         comps = [(c, bit) for c, bit in (("u", _lib.PSV), ("v", _lib.SH), ("w", _lib.PSV))
                  if self.systems & bit]
         seg_shot_bytes = nslot * len(comps) * self.nx * self.nz * 4
-        left = self.snapshot_bytes
-        if left == "auto":
-            left = torch.cuda.mem_get_info()[0] // 2
+        # the budget never takes memory that the rest of the job still needs:
+        # a recompute segment per shot, a checkpoint per segment, and slack
+        self.kept, self.ckpt = {}, []  # an earlier run's buffers are released first
+        free = free_device_bytes()
+        reserve = len(self.fw) * (seg_shot_bytes + nseg * len(names) * self.nx * self.nz * 4)
+        reserve += 4 << 30
+        left = free // 2 if self.snapshot_bytes == "auto" else self.snapshot_bytes
+        left = max(0, min(left, free - reserve))
         first_kept = nseg
         for k in range(nseg - 1, -1, -1):
             need = seg_shot_bytes * sum(1 for f in self.fw if self.edges[k] < f.nt)

@@ -22,6 +22,14 @@ def device() -> torch.device:
     return torch.device("cuda", torch.cuda.current_device())
 
 
+def free_device_bytes() -> int:
+    """Device memory that new tensors can use: what the device reports free
+    plus what torch's caching allocator holds without using (blocks freed by
+    earlier batches stay cached and are reused)."""
+    free, _ = torch.cuda.mem_get_info()
+    return int(free + torch.cuda.memory_reserved() - torch.cuda.memory_allocated())
+
+
 def stream_handle():
     return ctypes_void(torch.cuda.current_stream().cuda_stream)
 

@@ -46,7 +46,7 @@ import torch
 from . import _lib
 from ._propagator import SPONGE_A, _ptr
 from .batch import ShotSpec, checkpoint_interval
-from .field import as_tensor, device, stream_handle
+from .field import as_tensor, device, free_device_bytes, stream_handle
 
 _REAL = {"float32": (torch.float32, 4), "float64": (torch.float64, 8)}
 _KERNELS = {"auto": _lib.KERNEL_AUTO, "step": _lib.KERNEL_SIMPLE, "blocked": _lib.KERNEL_BLOCKED,
@@ -269,14 +269,13 @@ class ElasticFWI:
         S = len(self.geo)
         # trailing segments whose history the forward pass keeps
         seg_bytes = self.L * n_hist * self.nx * self.nz * self.rb * S
+        self._kept = []  # an earlier forward's history is released first
+        self._ck = None
         budget = self.history_bytes
         if budget == "auto":
-            free, _ = torch.cuda.mem_get_info()
-            budget = free // 2
+            budget = free_device_bytes() // 2
         self.kept_segments = min(nseg, int(budget) // seg_bytes)
         first_kept = nseg - self.kept_segments
-        self._kept = []
-        self._ck = None
         # every shot's state in one tensor, so a checkpoint is one copy
         self.state_all = torch.zeros((S, len(names)) + shape, dtype=self.dtype, device=dev)
         self.state = [{k: self.state_all[q, m] for m, k in enumerate(names)} for q in range(S)]
