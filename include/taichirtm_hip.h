@@ -20,8 +20,10 @@
  *    trtm_last_error() (thread-local).  Blow-up is not an error return: it is
  *    reported through check_flags with the reference's codes 1-3 (forward) and
  *    4-6 (backward) (forward_modeling.py:522-529, backward_modeling.py:533-537).
- *  - Thread safety: no global mutable state; distinct descriptors may be used
- *    concurrently on distinct streams.
+ *  - Thread safety: distinct descriptors may be used concurrently on distinct
+ *    streams.  The one piece of global mutable state is the static device
+ *    scratch of trtm_forward_steps_noalloc, shared by every call of that entry
+ *    point on one device.
  */
 #ifndef TAICHIRTM_HIP_H
 #define TAICHIRTM_HIP_H
@@ -204,6 +206,24 @@ int trtm_forward_steps(const trtm_step_desc *desc, int32_t it_begin, int32_t it_
  */
 int trtm_backward_steps(const trtm_step_desc *desc, int32_t it_begin, int32_t it_end,
                         void *stream);
+
+/*
+ * trtm_forward_steps without a caller workspace and without any device
+ * allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace (second state buffer, per-tile lists, shot
+ * table) is a static device array of TRTM_NOALLOC_SCRATCH_BYTES that each
+ * device holds until the process ends; desc->workspace must be NULL and
+ * desc->workspace_bytes 0.  A kernel whose workspace exceeds the scratch is
+ * rejected with TRTM_ERR_ARG (a 64 MiB scratch covers the blocked kernel up to
+ * about 1.8 M cells, e.g. 1200 x 1500); TRTM_KERNEL_AUTO falls back to the
+ * fused, then the simple kernel.  Same results as trtm_forward_steps.  Every
+ * call on a device shares that device's scratch: calls that overlap on one
+ * device (two streams, two threads) race and must be serialised by the caller.
+ */
+#define TRTM_NOALLOC_SCRATCH_BYTES (64 * 1024 * 1024)
+int trtm_forward_steps_noalloc(const trtm_step_desc *desc, int32_t it_begin, int32_t it_end,
+                               int32_t save, void *stream);
 
 /*
  * Many shots on one medium in one launch per time block (blocked kernel):

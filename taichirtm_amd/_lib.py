@@ -16,6 +16,7 @@ OK = 0
 ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3
 PSV, SH = 1, 2
 KERNEL_AUTO, KERNEL_SIMPLE, KERNEL_FUSED, KERNEL_BLOCKED, KERNEL_WAVE = 0, 1, 2, 3, 4
+NOALLOC_SCRATCH_BYTES = 64 * 1024 * 1024  # TRTM_NOALLOC_SCRATCH_BYTES
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -102,6 +103,8 @@ SIGNATURES = {
                                                  ctypes.c_int64, _i32, _i32, _p]),
     "trtm_forward_steps": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _i32, _i32, _p]),
     "trtm_backward_steps": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _i32, _p]),
+    "trtm_forward_steps_noalloc": (ctypes.c_int, [ctypes.POINTER(StepDesc), _i32, _i32, _i32,
+                                                  _p]),
     "trtm_check_finite": (ctypes.c_int, [_i32, _i32, _p, _p, _p, _i32, _p, _p]),
     "trtm_prepare_medium_f64": (ctypes.c_int, [_i32, _i32] + [_p] * 10 + [_p]),
     "trtm_fwi_prepare": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _p]),

@@ -6,6 +6,7 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -507,6 +508,80 @@ int trtm_forward_steps(const trtm_step_desc* d, int32_t it_begin, int32_t it_end
 
 int trtm_backward_steps(const trtm_step_desc* d, int32_t it_begin, int32_t it_end, void* stream) {
     return run_steps(d, it_begin, it_end, false, false, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// --- allocation-free forward steps ----------------------------------------------
+//
+// The workspace of trtm_forward_steps (second state buffer, per-tile lists,
+// shot table) comes from this file-scope device array instead of the caller.
+// Each device holds its own copy until the process ends.  Host code never uses
+// the array's name as a device address: the address is looked up once per
+// device with hipGetSymbolAddress and kept in g_scratch_dev.
+static __device__ __align__(256) unsigned char scratch[TRTM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (looked up on the first
+// call per device)
+int scratch_address(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(TRTM_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(TRTM_ERR_UNSUPPORTED, "device %d beyond the %d this library tracks", dev,
+                    kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(TRTM_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return TRTM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int trtm_forward_steps_noalloc(const trtm_step_desc* d, int32_t it_begin, int32_t it_end,
+                               int32_t save, void* stream) {
+    if (!d || d->struct_size != (int32_t)sizeof(trtm_step_desc))
+        return fail(TRTM_ERR_ARG, "bad descriptor");
+    if (d->workspace || d->workspace_bytes)
+        return fail(TRTM_ERR_ARG, "the workspace comes from the library's static scratch; "
+                                  "pass workspace = NULL, workspace_bytes = 0");
+    const int64_t scratch_bytes = TRTM_NOALLOC_SCRATCH_BYTES;
+    trtm_step_desc w = *d;
+    w.workspace = nullptr;
+    w.workspace_bytes = 0;
+    if (w.kernel != TRTM_KERNEL_SIMPLE) {
+        // an explicitly chosen kernel whose workspace exceeds the scratch is
+        // rejected before anything is queued; TRTM_KERNEL_AUTO falls back to
+        // the fused kernel, then the simple one, exactly as with a caller's
+        // workspace of TRTM_NOALLOC_SCRATCH_BYTES
+        const int64_t need = (w.kernel == TRTM_KERNEL_BLOCKED || w.kernel == TRTM_KERNEL_WAVE)
+                                 ? block_workspace_bytes(&w)
+                             : w.kernel == TRTM_KERNEL_FUSED ? fused_workspace_bytes(&w)
+                                                             : 0;
+        if (need > scratch_bytes)
+            return fail(TRTM_ERR_ARG,
+                        "this grid needs %lld workspace bytes; the static scratch holds %lld",
+                        (long long)need, (long long)scratch_bytes);
+        void* p = nullptr;
+        const int rc = scratch_address(&p);
+        if (rc) return rc;
+        w.workspace = p;
+        w.workspace_bytes = scratch_bytes;
+    }
+    return run_steps(&w, it_begin, it_end, true, save != 0, (hipStream_t)stream);
 }
 
 int trtm_check_finite(int32_t nx, int32_t nz, const float* u, const float* v, const float* w,
