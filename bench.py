@@ -1,47 +1,50 @@
 """Benchmark: elastic FD / RTM hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sh_layered]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload salt]
 
-One step = one single-shot RTM of the workload: forward modeling with a
-snapshot every isnap steps, then backward propagation of the recorded
-seismogram with the zero-lag image accumulated in the step kernel
-(the reference's rtm.py:935-1000 sequence).  Inputs (medium, wavelets) are
-resident in HBM before the timed region; the state is reset between steps.
+Default workload (salt): the FWI gradient and the RTM image of the 32-shot
+salt survey, P-SV 4000 x 2000 x 8000, computed together under a 128 GB
+device-memory budget.  One step is one shot group of the budget's plan (3
+shots), taken round robin over the survey's groups; per group, one forward
+pass keeps checkpoints, and walking the segments last to first each
+segment's history is recomputed once and feeds the exact adjoint (gradient)
+and the reference's back-propagation with zero-lag imaging
+(backward_modeling.py:539-576).  Observed data (the true model's records) are
+computed before the timed region and stay resident on the device.
 
-value  = useful cell-updates / second (2 * nx * nz * nt per shot: forward and
-         backward steps; nothing is recomputed).
-roofline: the dominant kernel is the backward step with imaging; its average
-         launch time is measured with HIP events on the launch stream over the
-         timed steps; algorithmic bytes per cell per launch are listed in
-         ALGO_BYTES.  frac is against the 8 TB/s HBM peak; frac_of_copy
-         against copy_gbs, the bandwidth of a 1 GiB device-to-device copy
-         measured in the same process (read + write bytes).
-f64_rel_l2: the timed run's forward seismograms against the same scheme in
-         double (relative L2), computed after the timed region.
+value  = useful cell-updates / second over the timed groups: 3 per cell-step
+         and shot (FWI forward, FWI adjoint, RTM backward with imaging; the
+         forward is shared by both backward passes, and the checkpoint pass
+         and the recompute are overhead, not counted).
+roofline: job-level achieved = useful updates x (60 + 120 + 84) / 3 bytes /
+         wall, against the 8 TB/s HBM peak.  roofline.kernel is the dominant
+         kernel, the blocked FWI adjoint: its average launch time from HIP
+         events on the launch stream around every adjoint call of the timed
+         groups, its algorithmic bytes (120 per cell-step and shot), and its
+         real HBM bytes per launch (roofline.traffic) from two rocprofv3
+         --pmc passes (FETCH_SIZE, WRITE_SIZE) over a short run of the same
+         kernel shape on the same grid and shot count, each scaled by what the
+         counter reads for a 256 MiB elementwise copy in the same pass.
+single_shot: the plain single-shot SH RTM of the 1200 x 600 x 4000 layered
+         model (forward + backward + image, isnap 1), graph-replayed.
 cpu_baseline: the oracle (C/OpenMP restatement of the reference's unfused
-         per-step kernel sequence, float32, both wave systems updated as the
-         reference does) timed on a bounded sample of the same workload.
-kernel:  "tuned" (default) lets the autotuner pick the blocked kernel family
-         and shape per direction during the warmup step.
-graph:   after the warmup the forward and the backward run are each recorded
-         into a HIP graph (trtm_plan_*) and every later step replays them (one
-         more untimed step replays them first); --no-graph issues the step
-         launches eagerly.
+         per-step kernel sequence, float32) timed on a bounded sample of the
+         first shot's RTM and extrapolated per cell-update.
 
-Other workloads (--workload psv_batch | fwi | salt): the batched
-checkpointed P-SV RTM of the 2000 x 1000 faulted model (64 shots in batches
-of 16), the P-SV FWI gradient of the 600 x 300 x 2500 problem (16 shots), and
-the FWI gradient plus RTM image of the 4000 x 2000 x 8000 salt model (32
-shots) under a device-memory budget (--budget-gb; --shots N for fewer
-shots); one step is the whole job, the roofline is job-level (useful
-cell-updates x algorithmic bytes / wall).  Prints one JSON line.
+Other workloads (--workload psv_batch | fwi | sh_layered | psv_anomaly): the
+64-shot batched checkpointed P-SV RTM of the 2000 x 1000 faulted model, the
+16-shot P-SV FWI gradient of the 600 x 300 x 2500 problem with sqrt(N)
+checkpoints (one step = the whole job), and the single-shot RTMs.  Prints one
+JSON line.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -52,8 +55,9 @@ sys.path.insert(0, ROOT)
 METRIC = "GCell-updates/s + RTM shots/hour on 1 MI355X; % HBM roofline; vs Taichi CPU"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per cell per launch (float32 words; sponge from the
-# cell index, 0 bytes; source/receiver traffic ignored):
+# Algorithmic HBM bytes per cell per launch of the single-shot RTM kernels
+# (float32 words; sponge from the cell index, 0 bytes; source/receiver
+# traffic ignored):
 #   SH forward step with snapshot:   v, syx, syz read+write 24 + myx, myz, ir 12 + snapshot write 4
 #   SH backward step with imaging:   24 + 12 + forward snapshot read 4 + image read+write 8
 #   P-SV forward with snapshot:      u, w, sxx, szz, sxz r+w 40 + lam, mu, mxz, iru, irw 20 + u, w snapshot 8
@@ -63,14 +67,16 @@ ALGO_BYTES = {
     ("psv", "forward"): 68, ("psv", "backward"): 84,
 }
 
-
-# measured HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, copy-calibrated)
-MEASURED_TRAFFIC = {
-    ("sh_layered_1200x600x4000", "blocked", (32, 128, 1024, 8)): 57.4e6,
-    ("sh_layered_1200x600x4000", "blocked", (24, 128, 1024, 8)): 59.4e6,
-    ("sh_layered_1200x600x4000", "blocked", (32, 48, 512, 8)): 59.8e6,
-    ("sh_layered_1200x600x4000", "blocked", (48, 64, 1024, 8)): 60.0e6,
-}
+# Algorithmic bytes per useful cell-update of a whole job:
+#   P-SV forward step 60 (state r+w 40, medium 20); backward with imaging 84
+#   (+ forward wavefield 8, image r+w 16); exact adjoint with gradient 120
+#   (adjoint state r+w 40, medium 20, history 20, 5 gradients r+w 40).
+#   psv_batch: forward + backward with imaging; fwi: forward + adjoint;
+#   salt: forward + adjoint + backward with imaging.
+JOB_BYTES = {"psv_batch": (60 + 84) / 2.0, "fwi": (60 + 120) / 2.0,
+             "salt": (60 + 120 + 84) / 3.0}
+JOB_SWEEPS = {"psv_batch": 2, "fwi": 2, "salt": 3}
+ADJOINT_BYTES = 120  # per cell-step and shot of k_fwi_adj_blk
 
 
 def build_workload(name):
@@ -80,13 +86,6 @@ def build_workload(name):
     if name == "psv_anomaly":
         return configs.psv_anomaly(), "psv"
     raise SystemExit("unknown workload %r" % name)
-
-
-# algorithmic bytes per useful cell-update of a whole job (forward and
-# backward/adjoint updates averaged): P-SV RTM with imaging every step
-# (68 + 84) / 2; P-SV FWI forward 60 and adjoint-with-gradient 120
-JOB_BYTES = {"psv_batch": (68 + 84) / 2.0, "fwi": (60 + 120) / 2.0,
-             "salt": (60 + 120 + 68 + 84) / 4.0}
 
 
 def _shot_specs(cfg, n):
@@ -100,21 +99,276 @@ def _shot_specs(cfg, n):
     return out
 
 
-def job_main(args):
-    """Batched checkpointed P-SV RTM (psv_batch), the P-SV FWI gradient (fwi)
-    or the budgeted gradient + image of the salt model (salt): one step = the
-    whole job over --shots shots."""
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# -- the salt survey: one shot group per step ----------------------------------
+
+def salt_main(args):
     import torch
-    from taichirtm_amd import autotune, configs
+    from taichirtm_amd import budget, configs
+    from taichirtm_amd.batch import ShotSpec
     copy_gbs = copy_bandwidth()  # before the job holds its memory budget
-    # defaults: the full survey of each workload and a memory budget that
-    # fits one MI355X beside it
-    full = {"psv_batch": (64, 16, 200.0), "fwi": (16, 0, 200.0), "salt": (32, 0, 128.0)}
-    if args.workload in full:
-        n, g, gb = full[args.workload]
-        args.shots = args.shots or n
-        args.group = args.group if args.group is not None else g
-        args.budget_gb = args.budget_gb if args.budget_gb is not None else gb
+    single = None
+    if not args.no_single_shot:
+        single = single_shot_line(args, steps=10, warmup=1)
+        single.pop("_cfg")
+        torch.cuda.empty_cache()
+    budget_gb = args.budget_gb if args.budget_gb is not None else 128.0
+    true, start, shot0 = budget.salt_survey(n_shots=32)
+    n_shots = args.shots or 32
+    pick = np.linspace(0, len(shot0) - 1, n_shots).round().astype(int)
+    shots = [shot0[q] for q in pick]
+    t1 = time.perf_counter()
+    obs = budget.observed_data(true, shots, on_device=True)  # inputs, resident in HBM
+    _log("observed data of %d shots: %.1f s" % (len(shots), time.perf_counter() - t1))
+    with_obs = [ShotSpec(s.src_loc, s.wavelets, s.rec_loc, tuple(o)) for s, o in zip(shots, obs)]
+    del obs
+    job = budget.GradientAndImage(start.nx, start.nz, start.dx, start.dz, start.fs, start.fw,
+                                  with_obs, budget_gb * 1e9, kernel=args.kernel)
+    job.prepare(start.vp, start.vs, start.rho)
+    n_groups = job.plan.groups
+    cells = start.nx * start.nz
+    for k in range(args.warmup):
+        t1 = time.perf_counter()
+        job.run_group(k % n_groups)
+        torch.cuda.synchronize()
+        _log("warmup group %d done, %.1f s" % (k % n_groups, time.perf_counter() - t1))
+    events = []
+    job.adjoint_events = events
+    order = [(args.warmup + k) % n_groups for k in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, g in enumerate(order):
+        job.run_group(g)
+        _log("timed step %d (group %d) queued, %.1f s since start"
+             % (k, g, time.perf_counter() - t0))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    job.adjoint_events = None
+    J, grads, image = job.result()
+    if not (np.isfinite(J) and all(bool(torch.isfinite(g).all()) for g in grads)
+            and all(bool(torch.isfinite(t).all()) for t in image.values())):
+        raise RuntimeError("non-finite misfit, gradient or image")
+    shot_steps = sum(s.nt for g in order for s in job.group_shots(g))
+    useful = JOB_SWEEPS["salt"] * cells * shot_steps
+    ms_per_step = wall * 1e3 / args.steps
+    g = useful / wall / 1e9
+    achieved = useful * JOB_BYTES["salt"] / wall / 1e9
+    shots_timed = sum(len(job.group_shots(g_)) for g_ in order)
+
+    # dominant kernel: the blocked FWI adjoint of the timed groups
+    ms = sum(e0.elapsed_time(e1) for e0, e1, _ in events)
+    launches = sum(n for _, _, n in events)
+    adj_shape = list(job.shapes["adjoint"])
+    group = job.plan.group
+    adj_bytes = ADJOINT_BYTES * cells * shot_steps  # every timed cell-step and shot
+    kernel = {"name": "k_fwi_adj_blk<P-SV, %d x %d tile, %d steps per launch>"
+                      % (adj_shape[0], adj_shape[1], adj_shape[2]),
+              "shape": adj_shape, "launches": launches,
+              "avg_launch_us": round(1e3 * ms / max(launches, 1), 2),
+              "share_of_wall": round(ms / 1e3 / wall, 4),
+              "algorithmic_bytes_per_launch": int(adj_bytes / max(launches, 1)),
+              "achieved": round(adj_bytes / (ms * 1e-3) / 1e9, 1), "unit": "GB/s"}
+    kernel["frac"] = round(kernel["achieved"] / HBM_PEAK_GBS, 4)
+    plan_L = job.plan.L_fwi
+    tuned = {"fwi": {k: list(v) for k, v in job.shapes.items()},
+             "rtm_backward": list(job.rtm_tuned.get("backward", ()))}
+    del job, with_obs, grads, image
+    torch.cuda.empty_cache()
+    traffic = None
+    if not args.no_traffic and adj_shape[2]:
+        traffic = counter_traffic(adj_shape, group, start)
+        kernel["counters"] = traffic
+        if traffic.get("bytes_per_launch"):
+            tb = traffic["bytes_per_launch"]
+            kernel["traffic"] = tb
+            kernel["traffic_gbs"] = round(tb / (kernel["avg_launch_us"] * 1e-6) / 1e9, 1)
+            kernel["traffic_frac"] = round(kernel["traffic_gbs"] / HBM_PEAK_GBS, 4)
+    desc = ("psv_salt_4000x2000x8000: P-SV FWI gradient + RTM image together under a %.0f GB "
+            "budget, %d shots (receiver spreads 1000-3000), groups of %d, one group per step "
+            "(round robin), layered start model" % (budget_gb, len(shots), group))
+    out = {
+        "metric": METRIC, "value": round(g, 3), "unit": "GCell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded salt model; observed data from the true model)",
+        "config": {"workload": desc, "grid": [start.nx, start.nz], "nt": start.nt,
+                   "dx": start.dx, "dt": start.dt, "shots": len(shots), "group": group,
+                   "groups": n_groups, "timed_groups": order, "checkpoint_every": plan_L,
+                   "budget_gb": budget_gb, "kernel": args.kernel, "tuned": tuned},
+        "shots_per_hour": round(3600.0 * shots_timed / wall, 1),
+        "useful_sweeps": "3 per cell-step and shot: FWI forward (60 B), FWI adjoint (120 B), "
+                         "RTM backward with imaging (84 B)",
+        "roofline": {"bound": "hbm",
+                     "basis": "job: useful cell-updates x 88 B (60 + 120 + 84) / 3 / wall; "
+                              "traffic and kernel: the dominant kernel",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": kernel.get("traffic"),
+                     "bytes_per_update": JOB_BYTES["salt"],
+                     "copy_gbs": round(copy_gbs, 1),
+                     "frac_of_copy": round(achieved / copy_gbs, 4),
+                     "kernel": kernel},
+    }
+    out["f64_rel_l2"] = salt_float64_error(start, shots[0])
+    if single is not None:
+        out["single_shot"] = single
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(start, target_s=12.0, sweeps=3)
+    print(json.dumps(out), flush=True)
+
+
+def salt_float64_error(cfg, spec):
+    """Relative L2 of the first shot's float32 records in the start model
+    against the same scheme in double."""
+    import torch
+    from taichirtm_amd.batch import ShotSpec
+    from taichirtm_amd.fwi import ElasticFWI
+    out = {}
+    recs = []
+    for prec in ("float32", "float64"):
+        dt = np.float32 if prec == "float32" else np.float64
+        s = ShotSpec(spec.src_loc, tuple(np.asarray(w, dt) for w in spec.wavelets), spec.rec_loc)
+        p = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, [s], precision=prec,
+                       history_bytes=0, kernel="auto")
+        m = [torch.as_tensor(np.asarray(x, dt), device="cuda") for x in (cfg.vp, cfg.vs, cfg.rho)]
+        recs.append(p.records(*m)[0].cpu().numpy().astype(np.float64))
+        del p, m
+    for c, name in enumerate("uvw"):
+        b = recs[1][c]
+        if np.abs(b).max() > 0:
+            out["seismogram_" + name] = float("%.3e" % (np.linalg.norm(recs[0][c] - b)
+                                                         / np.linalg.norm(b)))
+    return out
+
+
+# -- real HBM bytes of the dominant kernel, from rocprofv3 counters ------------
+
+def counter_traffic(shape, shots, cfg, nt=48, timeout=240):
+    """FETCH_SIZE and WRITE_SIZE per launch of the blocked FWI adjoint with
+    ``shape`` (tile_x, tile_z, block_steps) on ``shots`` shots of cfg's grid,
+    each from a rocprofv3 --pmc pass over a short run of this script
+    (--probe), scaled by the counter's reading for a 256 MiB elementwise copy
+    (negation) in the same pass.  Returns a dict; bytes_per_launch is None
+    with a reason when the profiler is not available or fails."""
+    import shutil
+    out = {"counters": ["FETCH_SIZE", "WRITE_SIZE"], "probe_steps": nt, "shots": shots}
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        out["bytes_per_launch"] = None
+        out["note"] = "rocprofv3 not found"
+        return out
+    env = dict(os.environ, TMPDIR="/tmp")
+    total = 0.0
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        with tempfile.TemporaryDirectory(prefix="trtm_pmc_", dir="/tmp") as td:
+            cmd = ["timeout", "-s", "KILL", str(timeout), prof, "--pmc", counter,
+                   "--output-format", "csv", "-d", td, "-o", "run", "--",
+                   sys.executable, os.path.join(ROOT, "bench.py"), "--probe", "adjoint",
+                   "--probe-shape", ",".join(str(x) for x in shape),
+                   "--shots", str(shots), "--probe-steps", str(nt)]
+            t1 = time.perf_counter()
+            r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
+            _log("counter pass %s: rc %d, %.1f s" % (counter, r.returncode,
+                                                    time.perf_counter() - t1))
+            vals = _read_counters(td, counter)
+        if not vals.get("adjoint") or not vals.get("copy"):
+            out["bytes_per_launch"] = None
+            out["note"] = "%s pass gave no readings (rc %d): %s" % (
+                counter, r.returncode, (r.stderr or r.stdout)[-300:])
+            return out
+        adj = float(np.mean(vals["adjoint"]))
+        cal = float(np.mean(vals["copy"]))
+        copy_bytes = (1 << 28) * (1 if counter == "FETCH_SIZE" else 1)
+        scale = copy_bytes / (cal * 1024.0)
+        key = counter.lower()
+        out[key + "_kib"] = round(adj, 1)
+        out[key + "_copy_kib"] = round(cal, 1)
+        out[key + "_scale"] = round(scale, 4)
+        out[key + "_dispatches"] = len(vals["adjoint"])
+        total += adj * 1024.0 * scale
+    out["bytes_per_launch"] = int(total)
+    out["command"] = ("rocprofv3 --pmc <FETCH_SIZE | WRITE_SIZE> --output-format csv -- python3 "
+                      "bench.py --probe adjoint --probe-shape %s --shots %d --probe-steps %d"
+                      % (",".join(str(x) for x in shape), shots, nt))
+    return out
+
+
+def _read_counters(folder, counter):
+    """Per-dispatch values of ``counter`` (KiB) from rocprofv3's counter
+    CSV: {"adjoint": [...], "copy": [...]}, summed over the rows of a
+    dispatch."""
+    import csv
+    import glob
+    per = {}
+    for f in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row.get("Kernel_Name", "")
+                kind = ("adjoint" if "k_fwi_adj_blk" in name else
+                        "copy" if "neg_kernel" in name else None)
+                if kind is None:
+                    continue
+                key = (kind, row.get("Dispatch_Id") or row.get("Correlation_Id"))
+                per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+    out = {"adjoint": [], "copy": []}
+    for (kind, _), v in per.items():
+        out[kind].append(v)
+    return out
+
+
+def probe_main(args):
+    """The counter passes' workload: a 256 MiB elementwise copy (negation,
+    the calibration), then the blocked FWI adjoint with the given shape on
+    --shots shots of the salt survey's start model over --probe-steps steps
+    (forward history kept, no recompute)."""
+    import torch
+    from taichirtm_amd import budget
+    from taichirtm_amd.batch import ShotSpec
+    from taichirtm_amd.fwi import ElasticFWI
+    torch.cuda.set_device(0)
+    a = torch.ones(1 << 26, dtype=torch.float32, device="cuda")  # 256 MiB
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    del a, b
+    true, start, shots = budget.salt_survey(n_shots=32)
+    nt = args.probe_steps
+    sub = [ShotSpec(s.src_loc, tuple(np.asarray(w)[:, :nt] for w in s.wavelets), s.rec_loc)
+           for s in shots[:args.shots]]
+    tx, tz, k = (int(x) for x in args.probe_shape.split(","))
+    p = ElasticFWI(start.nx, start.nz, start.dx, start.dz, start.fs, start.fw, sub,
+                   precision="float32", kernel="blocked", tile_x=tx, tile_z=tz, block_steps=k,
+                   checkpoint_every=nt, history_bytes=1 << 40)
+    m = [torch.as_tensor(np.asarray(x, np.float32), device="cuda")
+         for x in (start.vp, start.vs, start.rho)]
+    recs = p.forward(*m)
+    res = [torch.randn_like(r) for r in recs]
+    p.gradient(res)
+    torch.cuda.synchronize()
+    print("probe done: %d shots, %d steps, adjoint shape %s" % (len(sub), nt, args.probe_shape),
+          flush=True)
+
+
+# -- batched jobs: one step = the whole job -------------------------------------
+
+def job_main(args):
+    """Batched checkpointed P-SV RTM (psv_batch) or the P-SV FWI gradient
+    with sqrt(N) checkpoints (fwi): one step = the whole job over --shots
+    shots."""
+    import torch
+    from taichirtm_amd import configs
+    copy_gbs = copy_bandwidth()  # before the job holds its memory budget
+    full = {"psv_batch": (64, 16, 200.0), "fwi": (16, 0, 0.0)}
+    n, g, gb = full[args.workload]
+    args.shots = args.shots or n
+    args.group = args.group if args.group is not None else g
+    args.budget_gb = args.budget_gb if args.budget_gb is not None else gb
     if args.workload == "psv_batch":
         from taichirtm_amd.batch import BatchRTM
         cfg = configs.psv_faulted()
@@ -132,7 +386,7 @@ def job_main(args):
                 b = BatchRTM(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, mu, lam, cfg.rho, cfg.fw, sub,
                              isnap=args.isnap, imaging="checkpoint", kernel=args.kernel,
                              tile_x=args.tile_x, tile_z=args.tile_z, block_steps=args.block_steps,
-                             snapshot_bytes=int(args.budget_gb * 1e9))
+                             snapshot_bytes=int(args.budget_gb * 1e9), graph=not args.no_graph)
                 if b.run() != [0] * len(sub):
                     raise RuntimeError("non-finite wavefield")
                 img = b.stacked_image()
@@ -141,68 +395,47 @@ def job_main(args):
                 tuned["kept_segments"] = [len(b.edges) - 1 - b.first_kept, len(b.edges) - 1]
                 del b, img
             return tuned
-        desc = ("batched checkpointed P-SV RTM, %d shots of %s (uneven record lengths and "
-                "receiver spreads) in batches of %d, isnap=%d, stacked image, segment "
+        desc = ("psv_faulted_2000x1000: batched checkpointed P-SV RTM, %d shots (uneven record "
+                "lengths and receiver spreads) in batches of %d, isnap=%d, stacked image, segment "
                 "wavefields of up to %.0f GB kept from the forward pass"
-                % (len(shots), cfg.name, group, args.isnap, args.budget_gb))
-    elif args.workload == "fwi":
+                % (len(shots), group, args.isnap, args.budget_gb))
+    else:
         from taichirtm_amd.fwi import ElasticFWI, l2_misfit_and_gradient
         cfg, start = configs.psv_fwi()
         shots = _shot_specs(cfg, args.shots)
         fwi_kernel = args.kernel if args.kernel in ("tuned", "blocked", "step") else "auto"
-        # the forward pass keeps the history of as many trailing checkpoint
-        # segments as --budget-gb holds; the adjoint recomputes the rest
+        # sqrt(N) checkpointing; --budget-gb > 0 keeps the history of that
+        # many bytes of trailing segments from the forward pass instead
         prob = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, shots,
                           precision="float32", kernel=fwi_kernel,
                           history_bytes=int(args.budget_gb * 1e9))
         t = lambda x: torch.as_tensor(np.asarray(x, np.float32), device="cuda")
-        obs = [o.clone() for o in prob.forward(t(cfg.vp), t(cfg.vs), t(cfg.rho))]
+        obs = [o.clone() for o in prob.records(t(cfg.vp), t(cfg.vs), t(cfg.rho))]
         m0 = [t(x) for x in (start.vp, start.vs, start.rho)]
 
         def step():
             l2_misfit_and_gradient(prob, *m0, obs)
             out = dict(prob.shapes)
             out["kept_segments"] = [prob.kept_segments, len(prob.edges) - 1]
+            out["checkpoint_every"] = [prob.L]
             return out
-        desc = ("P-SV FWI gradient (forward + checkpointed exact adjoint), %d shots of %s, "
-                "forward history of up to %.0f GB kept" % (len(shots), cfg.name, args.budget_gb))
-    elif args.workload == "salt":
-        from taichirtm_amd import budget
-        true, cfg, shot0 = budget.salt_survey(n_shots=32)
-        pick = np.linspace(0, len(shot0) - 1, args.shots).round().astype(int)
-        shots = [shot0[q] for q in pick]
-        obs = budget.observed_data(true, shots)  # input data, made before the timed region
-        with_obs = [budget.ShotSpec(s.src_loc, s.wavelets, s.rec_loc, tuple(o))
-                    for s, o in zip(shots, obs)]
-        del obs
-        job = budget.GradientAndImage(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, with_obs,
-                                      args.budget_gb * 1e9, kernel=args.kernel)
-
-        def step():
-            job.run(cfg.vp, cfg.vs, cfg.rho)
-            p = job.plan
-            return {"group": [p.group], "L_fwi": [p.L_fwi], "L_rtm": [p.L_rtm]}
-        desc = ("P-SV FWI gradient + RTM image together under a %.0f GB budget, %d of the 32 "
-                "shots of %s (receiver spreads 1000-3000), layered start model"
-                % (args.budget_gb, len(shots), true.name))
-    else:
-        raise SystemExit("unknown workload %r" % args.workload)
+        desc = ("psv_fwi_600x300x2500: P-SV FWI gradient (forward + exact adjoint), %d shots, "
+                "checkpoints every sqrt(N) steps, forward history of %.0f GB kept"
+                % (len(shots), args.budget_gb))
+    tuned = None
     for k in range(args.warmup):
         t1 = time.perf_counter()
         tuned = step()
         torch.cuda.synchronize()
-        print("warmup step %d done, %.1f s" % (k, time.perf_counter() - t1), file=sys.stderr,
-              flush=True)
+        _log("warmup step %d done, %.1f s" % (k, time.perf_counter() - t1))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         tuned = step()
-        print("timed step %d launched, %.1f s since start" % (k, time.perf_counter() - t0),
-              file=sys.stderr, flush=True)
+        _log("timed step %d launched, %.1f s since start" % (k, time.perf_counter() - t0))
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
-    passes = 4.0 if args.workload == "salt" else 2.0  # salt: FWI and RTM, forward + backward each
-    useful = passes * cfg.nx * cfg.nz * sum(s.nt for s in shots)
+    useful = JOB_SWEEPS[args.workload] * cfg.nx * cfg.nz * sum(s.nt for s in shots)
     g = useful / wall / 1e9
     achieved = useful * JOB_BYTES[args.workload] / wall / 1e9
     out = {
@@ -222,9 +455,7 @@ def job_main(args):
     out["roofline"]["copy_gbs"] = round(copy_gbs, 1)
     out["roofline"]["frac_of_copy"] = round(achieved / copy_gbs, 4)
     if not args.no_cpu_baseline:
-        # the oracle's RTM of the first shot on a shorter sample (it keeps every
-        # snapshot in host memory); per-cell rate of the CPU path on this grid
-        out["cpu_baseline"] = cpu_baseline(cfg, target_s=4.0 if args.workload == "salt" else 6.0)
+        out["cpu_baseline"] = cpu_baseline(cfg, target_s=12.0, sweeps=2)
     print(json.dumps(out), flush=True)
 
 
@@ -310,9 +541,23 @@ def stream():
     return stream_handle()
 
 
-def cpu_baseline(cfg, target_s=12.0):
-    """Oracle forward + backward (with imaging every step) on a truncated
-    record of the same model; rate in cell-updates per second."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, target_s=12.0, sweeps=2):
+    """Oracle forward + backward (with imaging every step) of the first shot
+    on a truncated record of the same model (the oracle keeps every snapshot
+    in host memory); rate in cell-updates per second, extrapolated to the
+    full record.  shots_per_hour: a shot of ``sweeps`` sweeps of nt steps at
+    that rate."""
     import oracle
     shot = cfg.shots[0]
     mu, lam = cfg.moduli()
@@ -332,60 +577,36 @@ def cpu_baseline(cfg, target_s=12.0):
     nt = int(min(shot.nt, max(probe, probe * target_s / max(t_probe, 1e-3))))
     t = timed(nt)
     cells = 2.0 * cfg.nx * cfg.nz * nt
+    rate = cells / t
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": cells / t / 1e9, "unit": "GCell-updates/s", "cores": cores,
-            "kind": "port",
-            "sample": "oracle float32 forward+backward+image (isnap=1), %dx%d grid, %d of %d "
-                      "steps each way, %.1f s" % (cfg.nx, cfg.nz, nt, shot.nt, t),
-            "shots_per_hour": 3600.0 / (t * shot.nt / nt)}
+    return {"value": rate / 1e9, "unit": "GCell-updates/s", "cores": cores,
+            "kind": "port", "label": "reference-equivalent CPU restatement",
+            "cpu_model": _cpu_model(), "extrapolated": nt < shot.nt,
+            "sample": "oracle float32 forward + backward + image (isnap=1) of shot 0, %dx%d grid, "
+                      "%d of %d steps each way, %.1f s%s"
+                      % (cfg.nx, cfg.nz, nt, shot.nt, t,
+                         "; rate extrapolated to the full record" if nt < shot.nt else ""),
+            "shots_per_hour": 3600.0 * rate / (sweeps * cfg.nx * cfg.nz * shot.nt)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sh_layered")
-    ap.add_argument("--kernel", default="tuned")
-    ap.add_argument("--shots", type=int, default=0,
-                    help="jobs: number of shots (default: the full survey, 64 / 16 / 32)")
-    ap.add_argument("--group", type=int, default=None,
-                    help="psv_batch: shots per batch (0 = all in one batch)")
-    ap.add_argument("--tile-x", type=int, default=0)
-    ap.add_argument("--block-steps", type=int, default=0)
-    ap.add_argument("--tile-z", type=int, default=0)
-    ap.add_argument("--isnap", type=int, default=1)
-    ap.add_argument("--budget-gb", type=float, default=None,
-                    help="salt: device memory of the job; fwi: forward history kept; "
-                         "psv_batch: segment wavefields kept")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="single-shot RTM: issue the step launches eagerly instead of replaying "
-                         "the captured forward and backward runs")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU; multi-GPU runs are not part of this build "
-              "(shots shard across GPUs with one image all-reduce)",
-              file=sys.stderr)
-        sys.exit(2)
-
+def single_shot_line(args, steps, warmup, workload="sh_layered"):
+    """One JSON object for the plain single-shot RTM of ``workload``: one
+    step = forward with a snapshot every isnap steps + backward with the
+    image, each pass replayed as one HIP graph after the warmup."""
     import torch
-    torch.cuda.set_device(0)
-    if args.workload in JOB_BYTES:
-        return job_main(args)
-    cfg, system = build_workload(args.workload)
+    cfg, system = build_workload(workload)
     shot = RTMShot(cfg, args.isnap, args.kernel, args.tile_x, args.block_steps, args.tile_z)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         shot.run()
     graph = not args.no_graph and args.kernel in ("blocked", "wave", "auto", "tuned")
     if graph:  # each pass one HIP graph launch from here on
         shot.capture()
         shot.run()
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         shot.run(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -394,7 +615,7 @@ def main():
 
     nt = cfg.nt
     cells = cfg.nx * cfg.nz
-    ms_per_step = wall * 1000.0 / args.steps
+    ms_per_step = wall * 1000.0 / steps
     gcells = 2.0 * cells * nt / (ms_per_step / 1000.0) / 1e9
     # time steps per launch of the step kernel (1 for the fused and simple kernels)
     blocked = args.kernel in ("blocked", "wave", "auto", "tuned")
@@ -413,17 +634,13 @@ def main():
         kname = "k_block backward, %d steps per launch, image accumulated in registers" % k
     else:
         kname = "k_fused backward step with imaging"
-    # HBM bytes per launch of the default blocked SH configuration, from the
-    # FETCH_SIZE / WRITE_SIZE passes recorded under profiles/
-    traffic = MEASURED_TRAFFIC.get((cfg.name, family,
-                                    shot.bw.block_config() if blocked else None))
     out = {
         "metric": METRIC,
         "value": round(gcells, 3),
         "unit": "GCell-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True,
         "scaling": "weak",
@@ -443,20 +660,76 @@ def main():
                       "backward": round(float(np.mean(bw_ms)), 3)},
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": b_bytes,
                      "steps_per_launch": k,
                      "avg_launch_us": round(bw_launch_us, 3),
                      "forward_launch_us": round(fw_launch_us, 3),
                      "forward_achieved": round(f_bytes / (fw_launch_us * 1e-6) / 1e9, 1)},
     }
+    out["f64_rel_l2"] = float64_error(cfg, shot)
+    out["_cfg"] = cfg
+    del shot
+    return out
+
+
+def single_main(args):
+    out = single_shot_line(args, args.steps, args.warmup, args.workload)
+    cfg = out.pop("_cfg")
     copy_gbs = copy_bandwidth()
     out["roofline"]["copy_gbs"] = round(copy_gbs, 1)
-    out["roofline"]["frac_of_copy"] = round(achieved / copy_gbs, 4)
-    out["f64_rel_l2"] = float64_error(cfg, shot)
+    out["roofline"]["frac_of_copy"] = round(out["roofline"]["achieved"] / copy_gbs, 4)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg)
+        out["cpu_baseline"] = cpu_baseline(cfg, target_s=12.0)
     print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="salt",
+                    choices=["salt", "psv_batch", "fwi", "sh_layered", "psv_anomaly"])
+    ap.add_argument("--kernel", default="tuned")
+    ap.add_argument("--shots", type=int, default=0,
+                    help="jobs: number of shots (default: the full survey, 32 / 64 / 16)")
+    ap.add_argument("--group", type=int, default=None,
+                    help="psv_batch: shots per batch (0 = all in one batch)")
+    ap.add_argument("--tile-x", type=int, default=0)
+    ap.add_argument("--block-steps", type=int, default=0)
+    ap.add_argument("--tile-z", type=int, default=0)
+    ap.add_argument("--isnap", type=int, default=1)
+    ap.add_argument("--budget-gb", type=float, default=None,
+                    help="salt: device memory of the job (128); fwi: forward history kept (0: "
+                         "sqrt(N) checkpoints); psv_batch: segment wavefields kept (200)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-single-shot", action="store_true",
+                    help="salt: skip the single-shot SH line")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="salt: skip the rocprofv3 counter passes of the dominant kernel")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="single-shot RTM and psv_batch: issue the step launches eagerly instead "
+                         "of replaying captured HIP graphs")
+    ap.add_argument("--probe", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--probe-shape", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--probe-steps", type=int, default=48, help=argparse.SUPPRESS)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU; multi-GPU runs are not part of this build "
+              "(shots shard across GPUs with one image all-reduce)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    torch.cuda.set_device(0)
+    if args.probe:
+        return probe_main(args)
+    if args.workload == "salt":
+        return salt_main(args)
+    if args.workload in JOB_BYTES:
+        return job_main(args)
+    return single_main(args)
 
 
 def copy_bandwidth(nbytes=1 << 30, reps=20):

@@ -66,6 +66,12 @@ def _desc_array(descs):
     return arr
 
 
+def observed_arrays(observed):
+    """A shot's observed data as float32 arrays, or as the device tensors
+    given (data already resident on the device is not copied)."""
+    return [o if isinstance(o, torch.Tensor) else np.asarray(o, np.float32) for o in observed]
+
+
 def checkpoint_interval(nt, n_state, n_snap, isnap=1, align=1, budget_bytes=None, cell_bytes=4,
                         cells=1):
     """Segment length L (a multiple of align) minimising checkpoint storage
@@ -309,7 +315,7 @@ class BatchRTM:
         self.bw = []
         for s, f in zip(self.shots, self.fw):
             if s.observed is not None:
-                obs = [np.asarray(o, np.float32) for o in s.observed]
+                obs = observed_arrays(s.observed)
             else:
                 seis = f.seismogram_tensors()
                 obs = []
@@ -431,7 +437,7 @@ class BatchRTM:
         dev = device()
         self.bw, self.flags_b, descs, keep = [], [], [], []
         for s in self.shots:
-            obs = [np.asarray(o, np.float32) for o in s.observed]
+            obs = observed_arrays(s.observed)
             common = self._common(s)
             if common["kernel"] == "tuned":
                 common["kernel"] = "blocked"

@@ -117,64 +117,92 @@ class GradientAndImage:
         self.share_forward = share_forward
         self.plan = plan(self.nx, self.nz, self.shots, budget_bytes, self.isnap, share_forward)
 
-    def run(self, vp, vs, rho):
-        """(J, (g_vp, g_vs, g_rho) float64, image {"u", "v", "w"} float64)."""
+    def prepare(self, vp, vs, rho):
+        """Model of the next groups: moduli, device copies and empty
+        accumulators (misfit, per-coefficient gradient sum, float64 image)."""
+        dev = device()
+        vp_n, vs_n, rho_n = (np.asarray(torch.as_tensor(x).cpu(), np.float32) for x in (vp, vs, rho))
+        self._mu, self._lam = configs.moduli(vp_n, vs_n, rho_n)
+        self._rho = rho_n
+        self._m = [torch.as_tensor(x, dtype=torch.float32, device=dev) for x in (vp_n, vs_n, rho_n)]
+        self.J = 0.0
+        self._gsum = None
+        self._image = {c: torch.zeros((self.nx, self.nz), dtype=torch.float64, device=dev)
+                       for c in ("u", "v", "w")}
+        self._last = None
+        # per group: (start event, end event, launches) of every adjoint call
+        # when a list (set by the caller before run_group)
+        self.adjoint_events = getattr(self, "adjoint_events", None)
+
+    def group_shots(self, k):
+        """Shots of group k (in survey order)."""
+        g0 = k * self.plan.group
+        return self.shots[g0:g0 + self.plan.group]
+
+    def run_group(self, k):
+        """Gradient and image of group k, added to the accumulators (shot
+        order within and across groups is the caller's: groups in ascending
+        order give the bits of one group holding every shot)."""
         dev = device()
         p = self.plan
-        vp_n, vs_n, rho_n = (np.asarray(torch.as_tensor(x).cpu(), np.float32) for x in (vp, vs, rho))
-        mu, lam = configs.moduli(vp_n, vs_n, rho_n)
-        m = [torch.as_tensor(x, dtype=torch.float32, device=dev) for x in (vp_n, vs_n, rho_n)]
-        J = 0.0
-        gsum = None
-        image = {c: torch.zeros((self.nx, self.nz), dtype=torch.float64, device=dev)
-                 for c in ("u", "v", "w")}
-        last = None
-        for g0 in range(0, len(self.shots), p.group):
-            group = self.shots[g0:g0 + p.group]
-            fwi = ElasticFWI(self.nx, self.nz, self.dx, self.dz, self.fs, self.fw,
-                             [ShotSpec(s.src_loc, s.wavelets, s.rec_loc) for s in group],
-                             precision="float32", checkpoint_every=p.L_fwi,
-                             kernel=self.kernel if self.kernel in ("tuned", "blocked") else "auto",
-                             history_bytes=0)  # the plan holds checkpoints + one segment
-            recs = fwi.forward(*m)
-            res = []
-            for r, s in zip(recs, group):
-                obs = torch.stack([torch.as_tensor(np.asarray(o, np.float32), device=dev)
-                                   for o in s.observed])
-                d = r - obs
-                J += 0.5 * float((d.double() ** 2).sum())
-                res.append(d)
-            del recs
-            rkernel = self.kernel if self.kernel in ("tuned", "blocked", "wave") else "blocked"
-            if self.share_forward:
-                # the RTM backward runs beside the adjoint on each recomputed
-                # segment of the forward history
-                rtm = BatchRTM(self.nx, self.nz, self.dx, self.dz, self.fs, mu, lam, rho_n,
-                               self.fw, group, isnap=1, imaging="checkpoint", kernel=rkernel)
-                rtm.external_begin(fwi.systems, fwi.L)
-                fwi.gradient(res, on_segment=rtm.external_segment)
-                codes = rtm.external_codes()
-            else:
-                fwi.gradient(res)
-            gsum = accumulate_in_shot_order(gsum, fwi.shot_coefficient_gradients)
-            fwi.release()
-            last = fwi
-            if not self.share_forward:
-                rtm = BatchRTM(self.nx, self.nz, self.dx, self.dz, self.fs, mu, lam, rho_n,
-                               self.fw, group, isnap=self.isnap, imaging="checkpoint",
-                               checkpoint_every=p.L_rtm, kernel=rkernel, snapshot_bytes=0)
-                codes = rtm.run()
-            if any(codes):
-                raise RuntimeError("non-finite wavefield, codes %s" % codes)
-            for imgs in rtm.image_tensors():
-                for c in ("u", "v", "w"):
-                    if imgs.get(c) is not None:
-                        image[c] += imgs[c].to(torch.float64)
-            del rtm, res, fwi
-        self.J = J
-        self.coefficient_gradient = gsum
-        gradient = last.chain(gsum)
-        return J, gradient, image
+        group = self.group_shots(k)
+        fwi = ElasticFWI(self.nx, self.nz, self.dx, self.dz, self.fs, self.fw,
+                         [ShotSpec(s.src_loc, s.wavelets, s.rec_loc) for s in group],
+                         precision="float32", checkpoint_every=p.L_fwi,
+                         kernel=self.kernel if self.kernel in ("tuned", "blocked") else "auto",
+                         history_bytes=0)  # the plan holds checkpoints + one segment
+        fwi.adjoint_events = self.adjoint_events
+        recs = fwi.forward(*self._m)
+        res = []
+        for r, s in zip(recs, group):
+            obs = torch.stack([torch.as_tensor(o, dtype=torch.float32, device=dev)
+                               for o in s.observed])
+            d = r - obs
+            self.J += 0.5 * float((d.double() ** 2).sum())
+            res.append(d)
+        del recs
+        rkernel = self.kernel if self.kernel in ("tuned", "blocked", "wave") else "blocked"
+        if self.share_forward:
+            # the RTM backward runs beside the adjoint on each recomputed
+            # segment of the forward history
+            rtm = BatchRTM(self.nx, self.nz, self.dx, self.dz, self.fs, self._mu, self._lam,
+                           self._rho, self.fw, group, isnap=1, imaging="checkpoint",
+                           kernel=rkernel)
+            rtm.external_begin(fwi.systems, fwi.L)
+            fwi.gradient(res, on_segment=rtm.external_segment)
+            codes = rtm.external_codes()
+        else:
+            fwi.gradient(res)
+        self.shapes = dict(fwi.shapes)
+        self._gsum = accumulate_in_shot_order(self._gsum, fwi.shot_coefficient_gradients)
+        fwi.release()
+        self._last = fwi
+        if not self.share_forward:
+            rtm = BatchRTM(self.nx, self.nz, self.dx, self.dz, self.fs, self._mu, self._lam,
+                           self._rho, self.fw, group, isnap=self.isnap, imaging="checkpoint",
+                           checkpoint_every=p.L_rtm, kernel=rkernel, snapshot_bytes=0)
+            codes = rtm.run()
+        if any(codes):
+            raise RuntimeError("non-finite wavefield, codes %s" % codes)
+        self.rtm_tuned = dict(rtm.tuned)
+        for imgs in rtm.image_tensors():
+            for c in ("u", "v", "w"):
+                if imgs.get(c) is not None:
+                    self._image[c] += imgs[c].to(torch.float64)
+        del rtm, res
+
+    def result(self):
+        """(J, (g_vp, g_vs, g_rho) float64, image {"u", "v", "w"} float64) of
+        the groups run since ``prepare``."""
+        self.coefficient_gradient = self._gsum
+        return self.J, self._last.chain(self._gsum), self._image
+
+    def run(self, vp, vs, rho):
+        """(J, (g_vp, g_vs, g_rho) float64, image {"u", "v", "w"} float64)."""
+        self.prepare(vp, vs, rho)
+        for k in range(self.plan.groups):
+            self.run_group(k)
+        return self.result()
 
 
 def salt_survey(n_shots=32, nt=8000, nx=4000, nz=2000, seed=0):
@@ -190,14 +218,17 @@ def salt_survey(n_shots=32, nt=8000, nx=4000, nz=2000, seed=0):
     return true, start, shots
 
 
-def observed_data(cfg, shots: List[ShotSpec], group=4):
+def observed_data(cfg, shots: List[ShotSpec], group=4, on_device=False):
     """Records of ``cfg``'s model for every shot (forward only, one
-    checkpoint), as numpy (3, nr, nt) per shot."""
+    checkpoint), (3, nr, nt) per shot: numpy, or float32 device tensors with
+    ``on_device``."""
     out = []
     m = [torch.as_tensor(np.asarray(x, np.float32), device=device()) for x in (cfg.vp, cfg.vs, cfg.rho)]
     for g0 in range(0, len(shots), group):
         sub = shots[g0:g0 + group]
         p = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, sub, precision="float32",
-                       checkpoint_every=max(s.nt for s in sub), history_bytes=0)
-        out += [r.cpu().numpy() for r in p.forward(*m)]
+                       history_bytes=0)
+        recs = p.records(*m)
+        out += recs if on_device else [r.cpu().numpy() for r in recs]
+        del p
     return out

@@ -68,6 +68,18 @@ def accumulate_in_shot_order(total, per_shot):
     return total
 
 
+def blocked_launches(a, b, K, shots):
+    """Kernel launches of one blocked FWI call over steps [a, b) for ``shots``
+    shots (the library's schedule): ceil((b - a) / K) launches per group of up
+    to 8 shots, one more when that count is odd and the first launch has more
+    than one step (it is then split in two, so the state ends in the caller's
+    buffers)."""
+    n = -(-(b - a) // K)
+    if n % 2 == 1 and min(K, b - a) > 1:
+        n += 1
+    return n * (-(-shots // 8))
+
+
 def _plan(nx, nz, loc):
     loc = np.ascontiguousarray(loc, np.int32).reshape(-1, 2)
     n = len(loc)
@@ -162,6 +174,11 @@ class ElasticFWI:
         self._ck = None
         self._kept = []
         self.kept_segments = 0
+        # bumped by every forward(): a gradient belongs to the last forward
+        self.generation = 0
+        # adjoint_events: when a list, gradient() appends (start event, end
+        # event, kernel launches) around every adjoint call, on the launch stream
+        self.adjoint_events = None
 
     # -- medium ---------------------------------------------------------------
 
@@ -257,6 +274,10 @@ class ElasticFWI:
         keeps the checkpoints for ``gradient``.  All shots advance together
         (one batched call per checkpoint segment)."""
         dev = device()
+        self.generation += 1
+        for g in self.geo:  # release() dropped the blocked kernels' second state buffer
+            if self.blocked and g["ws"] is None:
+                g["ws"] = torch.empty(8 * self.nx * self.nz, dtype=self.dtype, device=dev)
         self.systems = self._systems()
         self.med = self._medium(vp, vs, rho)
         names = self._fields()
@@ -382,7 +403,16 @@ class ElasticFWI:
                 grad_all.zero_()
             if k == nseg - 1:
                 self._tune("adjoint", "trtm_fwi_adjoint_batch", descs, a, b, reset_adjoint)
+            timed = self.adjoint_events is not None and self.blocked and self.shapes["adjoint"][2]
+            if timed:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
             self._call_batch("trtm_fwi_adjoint_batch", descs, a, b, self.shapes["adjoint"])
+            if timed:
+                e1.record()
+                self.adjoint_events.append(
+                    (e0, e1, blocked_launches(a, b, self.shapes["adjoint"][2], S)))
             if on_segment is not None:  # the segment's history is still in place
                 on_segment(a, b, hists)
             if k >= first_kept:
@@ -392,6 +422,25 @@ class ElasticFWI:
         grads = accumulate_in_shot_order(None, self.shot_coefficient_gradients)
         self.coefficient_gradient = grads
         return self.chain(grads)
+
+    def records(self, vp, vs, rho):
+        """Records of every shot, as ``forward`` gives them, without keeping
+        checkpoints or history for a gradient."""
+        saved = (self.checkpoint_every, self.history_bytes)
+        self.checkpoint_every = max(g["nt"] for g in self.geo)
+        self.history_bytes = 0
+        try:
+            return self.forward(vp, vs, rho)
+        finally:
+            self.checkpoint_every, self.history_bytes = saved
+            self.release_forward()
+
+    def release_forward(self):
+        """Drop what forward() keeps for a gradient (checkpoints, kept
+        history, state); the next gradient() needs a new forward()."""
+        self._ck = None
+        self._kept = []
+        self.state_all = self.state = None
 
     def release(self):
         """Drop the checkpoints, state, workspaces and per-shot gradients;
@@ -435,16 +484,21 @@ class Seismograms(torch.autograd.Function):
         if records_from is not None:
             if records_from.dtype != torch.float64:
                 raise ValueError("records_from must be a float64 problem")
-            recs = records_from.forward(*(x.detach().to(torch.float64) for x in (vp, vs, rho)))
+            recs = records_from.records(*(x.detach().to(torch.float64) for x in (vp, vs, rho)))
             out_dtype = torch.float64
         if len({tuple(r.shape) for r in recs}) != 1:
             raise ValueError("autograd records need equal receiver counts and record lengths")
         ctx.problem = problem
+        ctx.generation = problem.generation
         ctx.dtypes = (vp.dtype, vs.dtype, rho.dtype)
         return torch.stack(recs).to(out_dtype)
 
     @staticmethod
     def backward(ctx, grad):
+        if ctx.problem.generation != ctx.generation:
+            raise RuntimeError("the problem ran forward again after these records were made "
+                               "(its checkpoints now belong to another model); apply "
+                               "Seismograms again before calling backward")
         g_vp, g_vs, g_rho = ctx.problem.gradient([grad[q] for q in range(grad.shape[0])])
         return (g_vp.to(ctx.dtypes[0]), g_vs.to(ctx.dtypes[1]), g_rho.to(ctx.dtypes[2]), None,
                 None)
@@ -468,7 +522,7 @@ def l2_misfit_and_gradient(problem: ElasticFWI, vp, vs, rho, observed,
         return J, problem.gradient(res)
     if records_from.dtype != torch.float64:
         raise ValueError("records_from must be a float64 problem")
-    recs = records_from.forward(*(torch.as_tensor(x).to(torch.float64) for x in (vp, vs, rho)))
+    recs = records_from.records(*(torch.as_tensor(x).to(torch.float64) for x in (vp, vs, rho)))
     res64 = [r - torch.as_tensor(o, device=r.device).to(torch.float64)
              for r, o in zip(recs, observed)]
     problem.forward(vp, vs, rho)
