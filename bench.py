@@ -386,7 +386,7 @@ def job_main(args):
                 b = BatchRTM(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, mu, lam, cfg.rho, cfg.fw, sub,
                              isnap=args.isnap, imaging="checkpoint", kernel=args.kernel,
                              tile_x=args.tile_x, tile_z=args.tile_z, block_steps=args.block_steps,
-                             snapshot_bytes=int(args.budget_gb * 1e9), graph=not args.no_graph)
+                             snapshot_bytes=int(args.budget_gb * 1e9), graph=args.graph)
                 if b.run() != [0] * len(sub):
                     raise RuntimeError("non-finite wavefield")
                 img = b.stacked_image()
@@ -709,8 +709,11 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="salt: skip the rocprofv3 counter passes of the dominant kernel")
     ap.add_argument("--no-graph", action="store_true",
-                    help="single-shot RTM and psv_batch: issue the step launches eagerly instead "
-                         "of replaying captured HIP graphs")
+                    help="single-shot RTM: issue the step launches eagerly instead of replaying "
+                         "the captured forward and backward runs")
+    ap.add_argument("--graph", action="store_true",
+                    help="psv_batch: record every batched step range into a HIP graph (measured "
+                         "1 %% slower than eager launches: each range is replayed once)")
     ap.add_argument("--probe", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--probe-shape", default="", help=argparse.SUPPRESS)
     ap.add_argument("--probe-steps", type=int, default=48, help=argparse.SUPPRESS)

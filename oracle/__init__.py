@@ -24,6 +24,10 @@ Parity status of each oracle function and the check that pins it:
             (tests/test_oracle_physics.py)
   imaging   image equals the explicit sum over stored forward/backward
             wavefields with the reference's gating (tests/test_oracle_physics.py)
+  topography  a surface matrix zero on row 0 and one elsewhere equals the flat
+            free surface value for value; a stepped mask equals the same steps
+            with syz, sxz, szz masked in numpy before each step
+            (tests/test_oracle_physics.py)
 """
 from __future__ import annotations
 

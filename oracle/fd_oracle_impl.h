@@ -95,6 +95,9 @@ static void FN(bc)(int nx, int nz, void **st, const void *sm)
                 AT(st[ST_SYZ], i, j) *= s;
                 AT(st[ST_SXZ], i, j) *= s;
                 AT(st[ST_SZZ], i, j) *= s;
+#if ORACLE_PLANT == 7
+                AT(st[ST_SXX], i, j) *= s;
+#endif
             }
     } else {
         for (int i = 0; i < nx; i++) {
@@ -255,8 +258,13 @@ int FN(oracle_forward)(const oracle_problem *p, void **st, const oracle_medium *
 #endif
     const int num_snaps = p->isnap > 0 ? nt / p->isnap : 0;
     for (int it = it0; it < it1; it++) {
+#if ORACLE_PLANT == 8
+        FN(velocity_fw)(nx, nz, st, m, dt, idx, idz);
+        FN(bc)(nx, nz, st, p->surface_matrix);
+#else
         FN(bc)(nx, nz, st, p->surface_matrix);
         FN(velocity_fw)(nx, nz, st, m, dt, idx, idz);
+#endif
         for (int k = 0; k < p->ns; k++) { /* forward_modeling.py:421-443 */
             int i = p->src_loc[2 * k], j = p->src_loc[2 * k + 1];
             REAL iru = AT(m->inv_rho_u, i, j), ir = AT(m->inv_rho, i, j), irw = AT(m->inv_rho_w, i, j);

@@ -30,6 +30,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import warnings
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
@@ -258,21 +259,34 @@ class BatchRTM:
                  if self.systems & bit]
         seg_shot_bytes = nslot * len(comps) * self.nx * self.nz * 4
         # the budget never takes memory that the rest of the job still needs:
-        # a recompute segment per shot, a checkpoint per segment, and slack
+        # the checkpoints of the segments below the first kept one, a
+        # recompute segment per shot when any segment is recomputed, and slack
         self.kept, self.ckpt = {}, []  # an earlier run's buffers are released first
         free = free_device_bytes()
-        reserve = len(self.fw) * (seg_shot_bytes + nseg * len(names) * self.nx * self.nz * 4)
-        reserve += 4 << 30
-        left = free // 2 if self.snapshot_bytes == "auto" else self.snapshot_bytes
-        left = max(0, min(left, free - reserve))
-        first_kept = nseg
+        state_shot = len(names) * self.nx * self.nz * 4
+        budget = free // 2 if self.snapshot_bytes == "auto" else int(self.snapshot_bytes)
+        first_kept, kept_bytes, capped = nseg, 0, False
         for k in range(nseg - 1, -1, -1):
             need = seg_shot_bytes * sum(1 for f in self.fw if self.edges[k] < f.nt)
-            if need > left:
+            if kept_bytes + need > budget:
                 break
-            left -= need
+            ckpt = state_shot * sum(min((f.nt + L - 1) // L, k) for f in self.fw)
+            recompute = seg_shot_bytes * len(self.fw) if k > 0 else 0
+            if kept_bytes + need + ckpt + recompute + (4 << 30) > free:
+                capped = True
+                break
+            kept_bytes += need
             first_kept = k
         self.first_kept = first_kept
+        # bytes of segment wavefields this run keeps; capped: an explicit
+        # snapshot_bytes was cut to what the device leaves free
+        self.snapshot_budget_used = kept_bytes
+        self.snapshot_budget_capped = capped and self.snapshot_bytes != "auto"
+        if self.snapshot_budget_capped:
+            warnings.warn("snapshot_bytes %d capped at %d bytes (%d of %d segments kept): the "
+                          "device has %d bytes free for the job" % (
+                              int(self.snapshot_bytes), kept_bytes, nseg - first_kept, nseg,
+                              free), RuntimeWarning)
         self.kept = {}
         self.ckpt = []
         for f in self.fw:

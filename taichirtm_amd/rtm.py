@@ -385,8 +385,10 @@ class ReverseTimeMigration:
         return self
 
     def save_result(self, directory: str, savename: str):
-        """npz with the reference's keys (rtm.py:1031-1070).  A missing surface
-        matrix is stored as an empty array rather than a pickled None."""
+        """npz with the reference's keys and values (rtm.py:1031-1070); a
+        missing surface matrix is stored as None, as the reference stores it
+        (numpy keeps it as a pickled object entry, which load_rtm_results
+        returns as None without unpickling)."""
         xmin = float(-self.offset)
         xmax = float(self.dx * self.nx - self.offset)
         sm = getattr(self, "surface_matrix", None)
@@ -397,7 +399,7 @@ class ReverseTimeMigration:
             u=self.image_u, v=self.image_v, w=self.image_w, receiver_loc=self.receiver_loc,
             src_loc_step=np.asarray(self.src_loc_step), offset=self.offset, xmin=xmin, xmax=xmax,
             zmin=0.0, zmax=float(self.dz * self.nz), dx=self.dx, dz=self.dz, nx=self.nx,
-            nz=self.nz, surface_matrix=np.zeros((0, 0), np.float32) if sm is None else sm)
+            nz=self.nz, surface_matrix=sm)
         print(f"Results saved to {os.path.join(directory, savename)}.npz")
 
     def get_results(self) -> tuple:

@@ -6,7 +6,9 @@ import oracle
 
 
 def run_hip(kw, wavelets, isnap=1, kernel="fused", obs=None, sci=100, surface=None,
-            callback=None, backward=True):
+            callback=None, backward=True, fw_opts=None, bw_opts=None):
+    """fw_opts / bw_opts: extra keywords of the forward / backward propagator
+    only (e.g. a pinned tile shape per direction)."""
     import taichirtm_amd as trtm
     kw = dict(kw)
     kw.pop("kernel", None)
@@ -14,7 +16,7 @@ def run_hip(kw, wavelets, isnap=1, kernel="fused", obs=None, sci=100, surface=No
         kw["surface_matrix_field"] = surface
     wu, wv, ww = wavelets
     fw = trtm.ForwardModeling(wavelet_u_field=wu, wavelet_v_field=wv, wavelet_w_field=ww,
-                              isnap=isnap, kernel=kernel, **kw)
+                              isnap=isnap, kernel=kernel, **kw, **(fw_opts or {}))
     code_f = fw.run(save=True, display_callback=callback, stability_check_interval=sci)
     seis = fw.get_seismogram()
     out = {"code_f": code_f, "seis": seis, "fw": fw,
@@ -23,7 +25,7 @@ def run_hip(kw, wavelets, isnap=1, kernel="fused", obs=None, sci=100, surface=No
         return out
     ob = seis if obs is None else obs
     bw = trtm.BackwardModeling(obsdata_u_field=ob[0], obsdata_v_field=ob[1],
-                               obsdata_w_field=ob[2], kernel=kernel, **kw)
+                               obsdata_w_field=ob[2], kernel=kernel, **kw, **(bw_opts or {}))
     out["code_b"] = bw.run_calc(fw.u_save_field, fw.v_save_field, fw.w_save_field,
                                 fw.isnaps_field, fw.num_snaps, isnap,
                                 stability_check_interval=sci)
@@ -47,7 +49,8 @@ def run_oracle(kw, wavelets, isnap=1, obs=None, sci=100, surface=None, dtype=np.
     ob = fw["seis"] if obs is None else obs
     snaps = fw["snaps"]
     bw = m.backward(ob, snaps[:3], snaps[3], fw["num_snaps"], isnap)
-    out.update(code_b=bw["code"], it_b=bw["it_stop"], image=bw["result"], state_b=bw["state"])
+    out.update(code_b=bw["code"], it_b=bw["it_stop"], image=bw["result"], state_b=bw["state"],
+               synsrc=bw["synsrc"])
     return out
 
 

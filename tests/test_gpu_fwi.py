@@ -193,6 +193,28 @@ def test_autograd_function_matches_explicit_gradient():
         assert torch.equal(a.grad, b.to(torch.float32))
 
 
+def test_autograd_refuses_a_problem_run_forward_again():
+    """Two applications of one problem before backward (a line-search trial,
+    say): the first loss's backward would read the checkpoints of the second
+    model, so it raises; a fresh application then gives the right gradient."""
+    _case("FWI autograd, stale forward")
+    grid, shots, m0, mt = problem("psv", nt=120)
+    p = ElasticFWI(*grid, shots, precision="float32", checkpoint_every=30)
+    obs = torch.stack(p.forward(*[_t(x, torch.float32) for x in mt])).clone()
+    params = [_t(x, torch.float32).requires_grad_(True) for x in m0]
+    d1 = Seismograms.apply(*params, p)
+    with torch.no_grad():
+        Seismograms.apply(*[(1.01 * x).detach() for x in params], p)
+    loss = 0.5 * ((d1 - obs) ** 2).sum()
+    with pytest.raises(RuntimeError, match="ran forward again"):
+        loss.backward()
+    d = Seismograms.apply(*params, p)
+    (0.5 * ((d - obs) ** 2).sum()).backward()
+    J, g = l2_misfit_and_gradient(p, *[x.detach() for x in params], list(obs))
+    for a, b in zip(params, g):
+        assert torch.equal(a.grad, b.to(torch.float32))
+
+
 def test_autograd_with_float64_records_matches_explicit_gradient():
     """Seismograms with records_from: float64 records out, the float32
     adjoint on the float64 residual; equals l2_misfit_and_gradient with the

@@ -79,6 +79,30 @@ def test_both_systems_duplicates_and_random_obs(kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
+def test_synthetic_source_record_matches_oracle(kernel):
+    """record_synthetic_source=True: the backward propagator's record at the
+    source cells (backward_modeling.py:462-470, row t of step it = nt-1-t),
+    both systems, two sources on one cell and one elsewhere, against the
+    oracle's synsrc bit for bit."""
+    _case("synthetic source kernel=%s" % kernel)
+    cfg = small_psv(nt=150)
+    kw = cfg.kwargs(cfg.shots[0])
+    kw["src_loc_field"] = np.array([[30, 5], [30, 5], [12, 40]], np.int32)
+    kw["num_sources"] = 3
+    w = configs.ricker(40.0, cfg.nt, cfg.dt)
+    wav = (np.stack([w, -0.5 * w, 0.25 * w]), np.stack([0.3 * w, w, w]),
+           np.stack([w, w, -w]))
+    h = run_hip(kw, wav, isnap=3, kernel=kernel, bw_opts=dict(record_synthetic_source=True))
+    o = run_oracle(kw, wav, isnap=3)
+    assert_bitwise(h, o)
+    for c, comp in enumerate("uvw"):
+        got = getattr(h["bw"], "synsrc_" + comp).to_numpy()
+        assert got.shape == (3, cfg.nt)
+        assert same(got, o["synsrc"][c]), comp
+        assert np.abs(got).max() > 0
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 def test_topography_surface_matrix(kernel):
     """surface_matrix multiplies syz, sxz, szz every step (forward_modeling.py:412-419)."""
     _case("topography kernel=%s" % kernel)
@@ -237,10 +261,10 @@ def test_psv_anomaly_full_size_parity():
     assert e_seis < 1e-4 and e_img < 1e-3
 
 
-def test_sh_layered_reduced_parity_and_full_size_kernels_agree():
-    """SH 1200x600: bitwise against the oracle over 300 steps; at the full
-    4000 steps the fused and simple kernels agree bit for bit."""
-    _case("sh_layered 1200x600")
+def test_sh_layered_reduced_parity_and_float64_error():
+    """SH 1200x600: bitwise against the oracle over 300 steps, and the float32
+    scheme's error against the oracle in float64 there."""
+    _case("sh_layered 1200x600x300")
     cfg = configs.sh_layered(nt=300)
     kw = cfg.kwargs(cfg.shots[0])
     wav = cfg.wavelets(cfg.shots[0])
@@ -253,14 +277,43 @@ def test_sh_layered_reduced_parity_and_full_size_kernels_agree():
     print("[fp64] SH 1200x600x300 seismogram v rel-L2 %.3e  image v rel-L2 %.3e"
           % (e_seis, e_img), flush=True)
     assert e_seis < 1e-4 and e_img < 1e-3
+
+
+def test_sh_layered_full_length_bench_kernels_bitwise():
+    """The single-shot bench workload at full length, 1200 x 600 x 4000: the
+    blocked pair the autotuner picks for it on MI355X (forward 48 x 64 tiles,
+    backward 32 x 48 tiles, 8 steps per launch) against the oracle, bit for
+    bit: seismograms, final states, the image (isnap 10, so the oracle keeps
+    400 snapshots) and the codes."""
+    _case("sh_layered 1200x600x4000 blocked 48x64x8 / 32x48x8 vs oracle")
     cfg = configs.sh_layered()
     kw = cfg.kwargs(cfg.shots[0])
     wav = cfg.wavelets(cfg.shots[0])
-    a = run_hip(kw, wav, isnap=10, kernel="fused")
-    b = run_hip(kw, wav, isnap=10, kernel="simple")
-    for c in range(3):
-        assert same(a["seis"][c], b["seis"][c]) and same(a["image"][c], b["image"][c])
-    assert np.abs(a["image"][1]).max() > 0
+    h = run_hip(kw, wav, isnap=10, kernel="blocked",
+                fw_opts=dict(tile_x=48, tile_z=64, block_steps=8),
+                bw_opts=dict(tile_x=32, tile_z=48, block_steps=8))
+    assert h["fw"].block_config() == (48, 64, 1024, 8)
+    assert h["bw"].block_config() == (32, 48, 512, 8)
+    o = run_oracle(kw, wav, isnap=10)
+    assert_bitwise(h, o)
+    assert np.abs(h["image"][1]).max() > 0 and np.abs(h["seis"][1]).max() > 0
+
+
+def test_sh_layered_full_length_image_error_against_float64():
+    """Error of the float32 scheme at full length (1200 x 600 x 4000,
+    isnap 10): the GPU's seismogram and image against the oracle run in
+    float64."""
+    _case("sh_layered 1200x600x4000 vs float64 oracle")
+    cfg = configs.sh_layered()
+    kw = cfg.kwargs(cfg.shots[0])
+    wav = cfg.wavelets(cfg.shots[0])
+    h = run_hip(kw, wav, isnap=10, kernel="tuned")
+    o64 = run_oracle(kw, wav, isnap=10, dtype=np.float64)
+    e_seis = oracle.relative_l2(h["seis"][1], o64["seis"][1])
+    e_img = oracle.relative_l2(h["image"][1], o64["image"][1])
+    print("[fp64] SH 1200x600x4000 seismogram v rel-L2 %.3e  image v rel-L2 %.3e"
+          % (e_seis, e_img), flush=True)
+    assert e_seis < 1e-4 and e_img < 1e-3
 
 
 BLOCK_CFGS = {"psv": [(16, 64, 2), (16, 64, 4), (16, 128, 4), (16, 128, 8), (32, 128, 4),

@@ -366,3 +366,80 @@ def test_fp32_error_and_fma_spread_are_small():
         i64 = oracle.relative_l2(res["f32"]["bw"]["result"][comp], res["f64"]["bw"]["result"][comp])
         ifma = oracle.relative_l2(res["fma"]["bw"]["result"][comp], res["f32"]["bw"]["result"][comp])
         assert e64 < 1e-5 and efma < 1e-5 and i64 < 1e-5 and ifma < 1e-5, (e64, efma, i64, ifma)
+
+
+# -- topography (surface matrix) ---------------------------------------------
+
+def _topo_problem(nx=40, nz=32, nt=70, variant="strict", surface=None):
+    rng = np.random.default_rng(11)
+    mu = (RHO * (VS * (1 + 0.1 * rng.random((nx, nz)))) ** 2).astype(np.float32)
+    lam = (((VP / VS) ** 2 - 2) * mu).astype(np.float32)
+    rho = np.full((nx, nz), RHO, np.float32)
+    src = [(nx // 2, 9)]
+    rec = np.stack([np.arange(2, nx - 2), np.full(nx - 4, 9)], 1)
+    m = oracle.Model(mu, lam, rho, 5.0, 5.0, 2500.0, nt, src, rec, 6, isnap=5, sci=1000,
+                     surface_matrix=surface, variant=variant)
+    w = ricker(60.0, nt, 4e-4)[None].astype(np.float32)
+    return m, [0.5 * w, w, w]
+
+
+def topo_flat_mismatch(variant="strict"):
+    """A surface matrix that is 1 everywhere except the row j = 0, where it is
+    0, must give the flat free surface (forward_modeling.py:404-410, zeroing
+    syz, sxz, szz on j = 0 before each step) value for value, since x * 1 = x
+    and x * 0 = 0 exactly.  Returns the number of differing values."""
+    m0, wav = _topo_problem(variant=variant)
+    sm = np.ones((m0.nx, m0.nz), np.float32)
+    sm[:, 0] = 0
+    m1, _ = _topo_problem(variant=variant, surface=sm)
+    a = m0.forward(wav, save=True)
+    b = m1.forward(wav, save=True)
+    bad = 0
+    for k in oracle.ST_NAMES:
+        bad += int((a["state"][k] != b["state"][k]).sum())
+    for c in range(3):
+        bad += int((a["seis"][c] != b["seis"][c]).sum())
+        bad += int((a["snaps"][c] != b["snaps"][c]).sum())
+    return bad
+
+
+def topo_stepwise_mismatch(variant="strict"):
+    """forward_modeling.py:412-419: before every step, syz, sxz and szz (and
+    nothing else) are multiplied by the surface matrix.  The oracle run with a
+    topography mask (0 above a stepped surface, 1 below) must equal, value for
+    value, the same steps taken one at a time with those three fields masked
+    in numpy first and a surface matrix of ones (multiplying by 1 is exact).
+    Returns the number of differing values."""
+    m, wav = _topo_problem(variant=variant)
+    nx, nz = m.nx, m.nz
+    depth = (3 + 3 * (np.arange(nx) > nx // 2)).astype(int)
+    sm = (np.arange(nz)[None, :] >= depth[:, None]).astype(np.float32)
+    mt, _ = _topo_problem(variant=variant, surface=sm)
+    m1, _ = _topo_problem(variant=variant, surface=np.ones((nx, nz), np.float32))
+    a = mt.forward(wav)
+    st = m1.new_state()
+    seis = [np.zeros((len(m1.rec_loc), m1.nt), np.float32) for _ in range(3)]
+    for it in range(m1.nt):
+        for k in ("syz", "sxz", "szz"):
+            st[k] *= sm
+        m1.forward(wav, state=st, seis=seis, it0=it, it1=it + 1)
+    bad = sum(int((a["state"][k] != st[k]).sum()) for k in oracle.ST_NAMES)
+    bad += sum(int((a["seis"][c] != seis[c]).sum()) for c in range(3))
+    assert np.abs(a["seis"][2]).max() > 0 and np.abs(a["state"]["sxz"]).max() > 0
+    return bad
+
+
+def test_topography_row_zero_equals_flat_surface():
+    assert topo_flat_mismatch() == 0
+
+
+def test_topography_masks_stresses_before_each_step():
+    assert topo_stepwise_mismatch() == 0
+
+
+@pytest.mark.parametrize("plant", ["plant7", "plant8"])
+def test_topography_pins_reject_planted_mistakes(plant):
+    """plant7: the mask also scales sxx; plant8: the mask is applied after the
+    velocity update instead of before it."""
+    assert topo_flat_mismatch(plant) > 0 or topo_stepwise_mismatch(plant) > 0
+    assert topo_stepwise_mismatch(plant) > 0
