@@ -404,24 +404,35 @@ def job_main(args):
         cfg, start = configs.psv_fwi()
         shots = _shot_specs(cfg, args.shots)
         fwi_kernel = args.kernel if args.kernel in ("tuned", "blocked", "step") else "auto"
-        # sqrt(N) checkpointing; --budget-gb > 0 keeps the history of that
-        # many bytes of trailing segments from the forward pass instead
-        prob = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, shots,
-                          precision="float32", kernel=fwi_kernel,
-                          history_bytes=int(args.budget_gb * 1e9))
         t = lambda x: torch.as_tensor(np.asarray(x, np.float32), device="cuda")
-        obs = [o.clone() for o in prob.records(t(cfg.vp), t(cfg.vs), t(cfg.rho))]
         m0 = [t(x) for x in (start.vp, start.vs, start.rho)]
+        mt = [t(x) for x in (cfg.vp, cfg.vs, cfg.rho)]
+        if args.budget_gb > 0:
+            # plain residual against observed records, the history of up to
+            # --budget-gb of trailing segments kept by the forward pass
+            prob = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, shots,
+                              precision="float32", kernel=fwi_kernel,
+                              history_bytes=int(args.budget_gb * 1e9))
+            obs = [o.clone() for o in prob.records(*mt)]
+            run = lambda: l2_misfit_and_gradient(prob, *m0, obs)
+            how = "forward history of up to %.0f GB kept, residual of float32 records" % args.budget_gb
+        else:
+            # sqrt(N) checkpoints, every segment recomputed; the residual
+            # against the true model's data propagated as a scattered field
+            prob = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, shots,
+                              precision="float32", kernel=fwi_kernel, history_bytes=0)
+            run = lambda: l2_misfit_and_gradient(prob, *m0, reference=mt)
+            how = ("checkpoints every sqrt(N) steps and every segment recomputed; residual "
+                   "against the true model's data as a float32 scattered field")
 
         def step():
-            l2_misfit_and_gradient(prob, *m0, obs)
+            run()
             out = dict(prob.shapes)
             out["kept_segments"] = [prob.kept_segments, len(prob.edges) - 1]
             out["checkpoint_every"] = [prob.L]
             return out
-        desc = ("psv_fwi_600x300x2500: P-SV FWI gradient (forward + exact adjoint), %d shots, "
-                "checkpoints every sqrt(N) steps, forward history of %.0f GB kept"
-                % (len(shots), args.budget_gb))
+        desc = ("psv_fwi_600x300x2500: P-SV FWI gradient (forward + exact adjoint), %d shots, %s"
+                % (len(shots), how))
     tuned = None
     for k in range(args.warmup):
         t1 = time.perf_counter()
@@ -454,9 +465,57 @@ def job_main(args):
     }
     out["roofline"]["copy_gbs"] = round(copy_gbs, 1)
     out["roofline"]["frac_of_copy"] = round(achieved / copy_gbs, 4)
+    if args.workload == "fwi":
+        del prob
+        torch.cuda.empty_cache()
+        out["f64_rel_l2"] = fwi_float64_error(cfg, start, scattered=args.budget_gb == 0)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, target_s=12.0, sweeps=2)
     print(json.dumps(out), flush=True)
+
+
+def fwi_float64_error(true, start, scattered=True):
+    """Relative L2 of the first shot's float32 gradient (the timed path: the
+    scattered-field residual, or the plain one) against the float64 run of
+    the same scheme, per parameter, and of its residual."""
+    import torch
+    from taichirtm_amd.batch import ShotSpec
+    from taichirtm_amd.fwi import ElasticFWI, l2_misfit_and_gradient
+    s = true.shots[0]
+    out = {}
+    grads, ress = {}, {}
+    for prec, dtype in (("float64", torch.float64), ("float32", torch.float32)):
+        npt = np.float64 if prec == "float64" else np.float32
+        w = configs_ricker(true, s).astype(npt)
+        z = np.zeros_like(w)
+        shot = ShotSpec(np.array([s.src], np.int32), (z, z, w), np.asarray(s.rec, np.int32))
+        p = ElasticFWI(true.nx, true.nz, true.dx, true.dz, true.fs, true.fw, [shot],
+                       precision=prec, history_bytes=0)
+        m = [torch.as_tensor(np.asarray(x, np.float32), dtype=dtype, device="cuda")
+             for x in (start.vp, start.vs, start.rho)]
+        mt = [torch.as_tensor(np.asarray(x, np.float32), dtype=dtype, device="cuda")
+              for x in (true.vp, true.vs, true.rho)]
+        if prec == "float32" and scattered:
+            ress[prec] = p.residual(*m, mt)[0].double()
+            grads[prec] = l2_misfit_and_gradient(p, *m, reference=mt)[1]
+        else:
+            obs = p.records(*mt)[0].clone()
+            ress[prec] = (p.forward(*m)[0] - obs).double()
+            grads[prec] = p.gradient([ress[prec].to(dtype)])
+        del p
+    rel = lambda a, b: float(torch.linalg.norm((a.double() - b.double()).ravel())
+                             / torch.linalg.norm(b.double().ravel()))
+    out["residual"] = float("%.3e" % rel(ress["float32"], ress["float64"]))
+    for name, a, b in zip(("vp", "vs", "rho"), grads["float32"], grads["float64"]):
+        out["gradient_" + name] = float("%.3e" % rel(a, b))
+    out["sample"] = "shot 0, float32 %s residual, against float64" % (
+        "scattered-field" if scattered else "records")
+    return out
+
+
+def configs_ricker(cfg, shot):
+    from taichirtm_amd import configs
+    return configs.ricker(cfg.f0, shot.nt, cfg.dt)[None]
 
 
 class RTMShot:

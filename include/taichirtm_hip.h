@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define TRTM_ABI_VERSION 5
+#define TRTM_ABI_VERSION 6
 
 #define TRTM_OK 0
 #define TRTM_ERR_ARG (-1)         /* invalid argument or shape */
@@ -319,6 +319,18 @@ typedef struct trtm_fwi_desc {
     int32_t kernel, tile_x, tile_z, block_steps;
     /* blocked kernels: second state buffer of trtm_fwi_workspace_bytes() bytes */
     void *workspace;
+    /* trtm_fwi_forward_scattered_batch only (NULL otherwise): this model's
+     * coefficients minus those of the reference model whose records are the
+     * observed data, per coefficient (lam, mu and the trtm_prepare_medium
+     * outputs), REAL (nx, nz); best formed in double and rounded once, since
+     * they carry the whole difference between the two models ... */
+    const void *dif_lam, *dif_mu, *dif_mxz, *dif_myx, *dif_myz, *dif_inv_rho, *dif_inv_rho_u,
+        *dif_inv_rho_w;
+    /* ... the scattered state, this model's state minus the reference model's
+     * (same field order as the state), updated in place ... */
+    void *du, *dv, *dw, *dsxx, *dszz, *dsxz, *dsyx, *dsyz;
+    /* ... and its second buffer, trtm_fwi_workspace_bytes() bytes */
+    void *scat_workspace;
 } trtm_fwi_desc;
 
 /* Workspace of the blocked FWI kernels for d's grid (8 REAL fields). */
@@ -364,6 +376,24 @@ int trtm_fwi_forward_batch(const trtm_fwi_desc *descs, int32_t n, int32_t it_beg
                            int32_t it_end, void *stream);
 int trtm_fwi_adjoint_batch(const trtm_fwi_desc *descs, int32_t n, int32_t it_begin,
                            int32_t it_end, void *stream);
+
+/*
+ * Scattered-field forward steps [it_begin, it_end), float, blocked kernel (no
+ * reference counterpart).  Advances this model's state exactly as
+ * trtm_fwi_forward_batch does (bit for bit) and, beside it, the scattered
+ * state ds = s(model) - s(reference model) by the linear recurrence
+ *   dv' = A (dv + dt b_ref D ds + dt (b - b_ref) D s) + source x (b - b_ref)
+ *   ds' = A (ds + dt C_ref D dv' + dt (C - C_ref) D v')
+ * with the differences (b - b_ref, C - C_ref) from dif_* and b_ref = b -
+ * dif, C_ref = C - dif
+ * of the forward step (forward_modeling.py:309-402, 421-443), whose float32
+ * rounding is relative to ds itself.  rec_* receive the scattered records,
+ * d(model) - d(reference): the FWI residual against data modelled in the
+ * reference model, without the cancellation of two float32 record sets.
+ * hist must be NULL; up to 4 shots per launch.
+ */
+int trtm_fwi_forward_scattered_batch(const trtm_fwi_desc *descs, int32_t n, int32_t it_begin,
+                                     int32_t it_end, void *stream);
 
 /*
  * Coefficient-space gradient to (lam, mu, rho) through the harmonic and

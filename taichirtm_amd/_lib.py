@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtaichirtm_hip.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 OK = 0
 ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3
 PSV, SH = 1, 2
@@ -82,6 +82,11 @@ class FwiDesc(ctypes.Structure):
         ("cell_map", _p),
         ("kernel", _i32), ("tile_x", _i32), ("tile_z", _i32), ("block_steps", _i32),
         ("workspace", _p),
+        ("dif_lam", _p), ("dif_mu", _p), ("dif_mxz", _p), ("dif_myx", _p), ("dif_myz", _p),
+        ("dif_inv_rho", _p), ("dif_inv_rho_u", _p), ("dif_inv_rho_w", _p),
+        ("du", _p), ("dv", _p), ("dw", _p), ("dsxx", _p), ("dszz", _p), ("dsxz", _p),
+        ("dsyx", _p), ("dsyz", _p),
+        ("scat_workspace", _p),
     ]
 
 
@@ -113,6 +118,8 @@ SIGNATURES = {
     "trtm_fwi_forward": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _p]),
     "trtm_fwi_forward_batch": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _i32, _p]),
     "trtm_fwi_adjoint_batch": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _i32, _p]),
+    "trtm_fwi_forward_scattered_batch": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _i32,
+                                                        _p]),
     "trtm_fwi_adjoint": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _p]),
     "trtm_fwi_chain_gradient": (ctypes.c_int, [_i32, _i32] + [_p] * 15 + [_p]),
     "trtm_transpose": (ctypes.c_int, [_i32, _i32, _p, _p, _p]),

@@ -800,6 +800,28 @@ int trtm_fwi_forward_batch(const trtm_fwi_desc* descs, int32_t n, int32_t it_beg
     return fwi_forward_t<double>(descs, n, it_begin, it_end, (hipStream_t)stream);
 }
 
+int trtm_fwi_forward_scattered_batch(const trtm_fwi_desc* descs, int32_t n, int32_t it_begin,
+                                     int32_t it_end, void* stream) {
+    int rc = check_batch(descs, n, it_begin, it_end, false);
+    if (rc) return rc;
+    for (int q = 0; q < n; q++) {
+        const trtm_fwi_desc* d = &descs[q];
+        if (d->real_bytes != 4 || !d->workspace || !d->scat_workspace)
+            return fail(TRTM_ERR_ARG, "shot %d: the scattered forward is float32 with both "
+                                      "workspaces", q);
+        if (d->hist) return fail(TRTM_ERR_ARG, "shot %d: the scattered forward keeps no history", q);
+        if ((d->systems & TRTM_PSV) &&
+            (!d->du || !d->dw || !d->dsxx || !d->dszz || !d->dsxz || !d->dif_lam || !d->dif_mu ||
+             !d->dif_mxz || !d->dif_inv_rho_u || !d->dif_inv_rho_w))
+            return fail(TRTM_ERR_ARG, "shot %d: P-SV scattered state or medium difference is NULL",
+                        q);
+        if ((d->systems & TRTM_SH) &&
+            (!d->dv || !d->dsyx || !d->dsyz || !d->dif_myx || !d->dif_myz || !d->dif_inv_rho))
+            return fail(TRTM_ERR_ARG, "shot %d: SH scattered state or medium difference is NULL", q);
+    }
+    return fwi_scat_forward(descs, n, it_begin, it_end, (hipStream_t)stream);
+}
+
 int trtm_fwi_adjoint_batch(const trtm_fwi_desc* descs, int32_t n, int32_t it_begin,
                            int32_t it_end, void* stream) {
     int rc = check_batch(descs, n, it_begin, it_end, true);

@@ -706,6 +706,334 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
 }
 
 // ---------------------------------------------------------------------------
+// scattered-field forward, steps t_a .. t_b-1 (t_b - t_a <= K)
+//
+// Advances this model's state exactly as k_fwi_fwd_blk (same expressions, so
+// its checkpoints are bit-identical) and, beside it, the scattered state
+// ds = s(model) - s(reference): with D the difference operators and b, C the
+// buoyancies and moduli of the velocity and stress updates,
+//   dv' = A (dv + dt (b_r D ds) + dt (db D s)) + f factor db   (source cells)
+//   ds' = A (ds + dt (C_r D dv') + dt (dC D v'))
+// where db = b - b_r and dC = C - C_r cell by cell (given, b_r = b - db and
+// C_r = C - dC), and s, v' are this model's pre-step stresses and
+// post-update velocities.  The recurrence is
+// linear in ds, so ds carries float32 rounding relative to itself: records of
+// ds are the residual d(model) - d(reference) without the cancellation of
+// two float32 records that agree to a few 1e-5.
+
+struct FS {
+    int nx, nz, nt, fw;
+    int n_rec, flip, pad0, pad1;
+    float dt, idx, idz, factor;
+    const float* tab;
+    const float *lam, *mu, *mxz, *myx, *myz, *ir, *iru, *irw;         // this model
+    const float *elam, *emu, *emxz, *emyx, *emyz, *eir, *eiru, *eirw;  // minus the reference's
+    float* st[8];    // this model's state u v w sxx szz sxz syx syz
+    float* alt[8];   // its workspace copy
+    float* dst[8];   // scattered state, same order
+    float* dalt[8];  // its workspace copy
+    const int *src_map, *rec_map;
+    const int *src_order, *src_start, *rec_order, *rec_start;
+    const float *wav_u, *wav_v, *wav_w;
+    float *rec_u, *rec_v, *rec_w;  // scattered records, time-major (nt, n_rec)
+};
+constexpr int kScatShotsPerLaunch = 4;  // per-shot entries fit the 4 KB kernel-argument limit
+template <class T>
+struct ByArray4 {
+    T v[kScatShotsPerLaunch];
+    __device__ __forceinline__ const T& get() const { return v[blockIdx.z]; }
+};
+static_assert(sizeof(ByArray4<FS>) + 32 <= 4096, "scattered kernel arguments exceed 4 KB");
+
+template <bool PSV, bool SH, int TX, int TZ, int NT, int K, class S>
+__global__ __launch_bounds__(NT) void k_fwi_scat_blk(S src, int t_a, int t_b, int tiles_x,
+                                                     int tiles_z) {
+    constexpr int RX = TX + 2 * K, RZ = TZ + 2 * K, RN = RX * RZ;
+    constexpr int NC = (RN + NT - 1) / NT;
+    constexpr int NP = PSV ? RN : 1, NS = SH ? RN : 1;
+    __shared__ float sU[NP], sW[NP], sXX[NP], sZZ[NP], sXZ[NP];
+    __shared__ float tU[NP], tW[NP], tXX[NP], tZZ[NP], tXZ[NP];
+    __shared__ float sV[NS], sYX[NS], sYZ[NS];
+    __shared__ float tV[NS], tYX[NS], tYZ[NS];
+
+    const FS& a = src.get();
+    const int tiles = tiles_x * tiles_z;
+    const int tile = xcd_tile_b(blockIdx.x, tiles);
+    const int nt = a.nt;
+    if (tile >= tiles || t_a >= nt) return;
+    const int i0 = (tile / tiles_z) * TX, j0 = (tile % tiles_z) * TZ;
+    const int nx = a.nx, nz = a.nz;
+    const int t = threadIdx.x;
+    const bool has_src = a.src_map != nullptr, has_rec = a.rec_map != nullptr;
+
+    int gc[NC], sg[NC], rg[NC];
+    unsigned fl[NC];  // bit0 in grid, bit1 interior, bit2 in tile, bit3 V ok, bit4 S ok
+    float A[NC];
+    // this model's coefficients (c, d), the reference's (r, q) and their
+    // differences (e, h): P-SV iru irw lam mu mxz, SH ir myx myz
+    float c0[NC], c1[NC], c2[NC], c3[NC], c4[NC], r0[NC], r1[NC], r2[NC], r3[NC], r4[NC];
+    float e0[NC], e1[NC], e2[NC], e3[NC], e4[NC];
+    float d0[NC], d1[NC], d2[NC], q0[NC], q1[NC], q2[NC], h0[NC], h1[NC], h2[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const RegionCell rc = region_cell<TX, TZ, K>(t + NT * k, i0, j0, nx, nz, a.fw);
+        gc[k] = rc.c;
+        fl[k] = rc.f;
+        const int slot = rc.slot;
+        const gptr<const float> tab = G(a.tab);
+        const float sa = tab[slot > 0 ? slot : 0];
+        A[k] = slot >= 0 ? sa : 1.0f;
+    }
+    {
+        float* const* in = a.flip ? a.alt : a.st;
+        float* const* din = a.flip ? a.dalt : a.dst;
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            const int e = t + NT * k;
+            const int c = gc[k];
+            const bool g = fl[k] & 1u;
+            sg[k] = has_src ? ld(a.src_map, c) : -1;
+            rg[k] = has_rec ? ld(a.rec_map, c) : -1;
+            if (!g) sg[k] = -1;
+            if ((fl[k] & 5u) != 5u) rg[k] = -1;
+            if (PSV) {
+                c0[k] = ld(a.iru, c);
+                c1[k] = ld(a.irw, c);
+                c2[k] = ld(a.lam, c);
+                c3[k] = ld(a.mu, c);
+                c4[k] = ld(a.mxz, c);
+                e0[k] = ld(a.eiru, c);
+                e1[k] = ld(a.eirw, c);
+                e2[k] = ld(a.elam, c);
+                e3[k] = ld(a.emu, c);
+                e4[k] = ld(a.emxz, c);
+                r0[k] = c0[k] - e0[k];
+                r1[k] = c1[k] - e1[k];
+                r2[k] = c2[k] - e2[k];
+                r3[k] = c3[k] - e3[k];
+                r4[k] = c4[k] - e4[k];
+                const float u = ld(in[0], c), w = ld(in[2], c), xx = ld(in[3], c), zz = ld(in[4], c),
+                            xz = ld(in[5], c);
+                const float du = ld(din[0], c), dw = ld(din[2], c), dxx = ld(din[3], c),
+                            dzz = ld(din[4], c), dxz = ld(din[5], c);
+                if (e < RN) {
+                    sU[e] = g ? u : 0.0f;
+                    sW[e] = g ? w : 0.0f;
+                    sXX[e] = g ? xx : 0.0f;
+                    sZZ[e] = g ? zz : 0.0f;
+                    sXZ[e] = g ? xz : 0.0f;
+                    tU[e] = g ? du : 0.0f;
+                    tW[e] = g ? dw : 0.0f;
+                    tXX[e] = g ? dxx : 0.0f;
+                    tZZ[e] = g ? dzz : 0.0f;
+                    tXZ[e] = g ? dxz : 0.0f;
+                }
+            }
+            if (SH) {
+                d0[k] = ld(a.ir, c);
+                d1[k] = ld(a.myx, c);
+                d2[k] = ld(a.myz, c);
+                h0[k] = ld(a.eir, c);
+                h1[k] = ld(a.emyx, c);
+                h2[k] = ld(a.emyz, c);
+                q0[k] = d0[k] - h0[k];
+                q1[k] = d1[k] - h1[k];
+                q2[k] = d2[k] - h2[k];
+                const float v = ld(in[1], c), yx = ld(in[6], c), yz = ld(in[7], c);
+                const float dv = ld(din[1], c), dyx = ld(din[6], c), dyz = ld(din[7], c);
+                if (e < RN) {
+                    sV[e] = g ? v : 0.0f;
+                    sYX[e] = g ? yx : 0.0f;
+                    sYZ[e] = g ? yz : 0.0f;
+                    tV[e] = g ? dv : 0.0f;
+                    tYX[e] = g ? dyx : 0.0f;
+                    tYZ[e] = g ? dyz : 0.0f;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    const float idx = a.idx, idz = a.idz, dt = a.dt, factor = a.factor;
+    const int hi = t_b < nt ? t_b : nt;
+    const gptr<const int> sorder = G(a.src_order), sstart = G(a.src_start);
+    const gptr<const int> rorder = G(a.rec_order), rstart = G(a.rec_start);
+    const gptr<const float> wu = G(a.wav_u), wv = G(a.wav_v), ww = G(a.wav_w);
+    const gptr<float> ru = G(a.rec_u), rv = G(a.rec_v), rw = G(a.rec_w);
+
+    bool started = false;  // a step ran: LDS velocities await the sponge
+    for (int s = 0; s < K; s++) {
+        const int it = t_a + s;
+        if (it >= hi) break;
+        // ---- velocity update + source injection (region)
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            if (!(fl[k] & 1u)) continue;
+            const int e = t + NT * k;
+            const bool upd = (fl[k] & 2u) && (fl[k] & 8u);
+            if (PSV) {
+                float u = sU[e], w = sW[e], du = tU[e], dw = tW[e];
+                if (started) {
+                    u = u * A[k];
+                    w = w * A[k];
+                    du = du * A[k];
+                    dw = dw * A[k];
+                }
+                if (upd) {
+                    const float sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
+                    const float szz_z = (sZZ[e] - sZZ[e - 1]) * idz;
+                    const float sxz_x = (sXZ[e] - sXZ[e - RZ]) * idx;
+                    const float sxz_z = (sXZ[e] - sXZ[e - 1]) * idz;
+                    u = u + ((sxx_x + sxz_z) * dt) * c0[k];
+                    w = w + ((sxz_x + szz_z) * dt) * c1[k];
+                    const float dsxx_x = (tXX[e] - tXX[e - RZ]) * idx;
+                    const float dszz_z = (tZZ[e] - tZZ[e - 1]) * idz;
+                    const float dsxz_x = (tXZ[e] - tXZ[e - RZ]) * idx;
+                    const float dsxz_z = (tXZ[e] - tXZ[e - 1]) * idz;
+                    du = du + (((dsxx_x + dsxz_z) * dt) * r0[k] + ((sxx_x + sxz_z) * dt) * e0[k]);
+                    dw = dw + (((dsxz_x + dszz_z) * dt) * r1[k] + ((sxz_x + szz_z) * dt) * e1[k]);
+                }
+                if (sg[k] >= 0)
+                    for (int q = sstart[sg[k]]; q < sstart[sg[k] + 1]; q++) {
+                        const size_t o = (size_t)sorder[q] * nt + it;
+                        const float fu = wu[o] * factor, fw_ = ww[o] * factor;
+                        u = u + fu * c0[k];
+                        w = w + fw_ * c1[k];
+                        du = du + fu * e0[k];
+                        dw = dw + fw_ * e1[k];
+                    }
+                sU[e] = u;
+                sW[e] = w;
+                tU[e] = du;
+                tW[e] = dw;
+            }
+            if (SH) {
+                float v = sV[e], dv = tV[e];
+                if (started) {
+                    v = v * A[k];
+                    dv = dv * A[k];
+                }
+                if (upd) {
+                    const float syx_x = (sYX[e] - sYX[e - RZ]) * idx;
+                    const float syz_z = (sYZ[e] - sYZ[e - 1]) * idz;
+                    v = v + ((syx_x + syz_z) * dt) * d0[k];
+                    const float dyx_x = (tYX[e] - tYX[e - RZ]) * idx;
+                    const float dyz_z = (tYZ[e] - tYZ[e - 1]) * idz;
+                    dv = dv + (((dyx_x + dyz_z) * dt) * q0[k] + ((syx_x + syz_z) * dt) * h0[k]);
+                }
+                if (sg[k] >= 0)
+                    for (int q = sstart[sg[k]]; q < sstart[sg[k] + 1]; q++) {
+                        const float fv = wv[(size_t)sorder[q] * nt + it] * factor;
+                        v = v + fv * d0[k];
+                        dv = dv + fv * h0[k];
+                    }
+                sV[e] = v;
+                tV[e] = dv;
+            }
+        }
+        __syncthreads();
+
+        // ---- stress update + sponge (region), records (tile)
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            if (!(fl[k] & 1u)) continue;
+            const int e = t + NT * k;
+            const bool upd = (fl[k] & 2u) && (fl[k] & 16u);
+            if (PSV) {
+                float sxx = sXX[e], szz = sZZ[e], sxz = sXZ[e];
+                float dxx = tXX[e], dzz = tZZ[e], dxz = tXZ[e];
+                if (upd) {
+                    const float u_x = (sU[e + RZ] - sU[e]) * idx;
+                    const float u_z = (sU[e + 1] - sU[e]) * idz;
+                    const float w_x = (sW[e + RZ] - sW[e]) * idx;
+                    const float w_z = (sW[e + 1] - sW[e]) * idz;
+                    const float div_uw = u_x + w_z;
+                    sxx = sxx + dt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
+                    szz = szz + dt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
+                    sxz = sxz + (dt * c4[k]) * (u_z + w_x);
+                    const float du_x = (tU[e + RZ] - tU[e]) * idx;
+                    const float du_z = (tU[e + 1] - tU[e]) * idz;
+                    const float dw_x = (tW[e + RZ] - tW[e]) * idx;
+                    const float dw_z = (tW[e + 1] - tW[e]) * idz;
+                    const float ddiv = du_x + dw_z;
+                    dxx = dxx + dt * ((r2[k] * ddiv + (2.0f * r3[k]) * du_x) +
+                                      (e2[k] * div_uw + (2.0f * e3[k]) * u_x));
+                    dzz = dzz + dt * ((r2[k] * ddiv + (2.0f * r3[k]) * dw_z) +
+                                      (e2[k] * div_uw + (2.0f * e3[k]) * w_z));
+                    dxz = dxz + dt * (r4[k] * (du_z + dw_x) + e4[k] * (u_z + w_x));
+                }
+                sXX[e] = sxx * A[k];
+                sZZ[e] = szz * A[k];
+                sXZ[e] = sxz * A[k];
+                tXX[e] = dxx * A[k];
+                tZZ[e] = dzz * A[k];
+                tXZ[e] = dxz * A[k];
+            }
+            if (SH) {
+                float syx = sYX[e], syz = sYZ[e], dyx = tYX[e], dyz = tYZ[e];
+                if (upd) {
+                    const float v_x = (sV[e + RZ] - sV[e]) * idx;
+                    const float v_z = (sV[e + 1] - sV[e]) * idz;
+                    syx = syx + (dt * d1[k]) * v_x;
+                    syz = syz + (dt * d2[k]) * v_z;
+                    const float dv_x = (tV[e + RZ] - tV[e]) * idx;
+                    const float dv_z = (tV[e + 1] - tV[e]) * idz;
+                    dyx = dyx + dt * (q1[k] * dv_x + h1[k] * v_x);
+                    dyz = dyz + dt * (q2[k] * dv_z + h2[k] * v_z);
+                }
+                sYX[e] = syx * A[k];
+                sYZ[e] = syz * A[k];
+                tYX[e] = dyx * A[k];
+                tYZ[e] = dyz * A[k];
+            }
+            if (rg[k] >= 0) {
+                const size_t row = (size_t)it * a.n_rec;
+                for (int q = rstart[rg[k]]; q < rstart[rg[k] + 1]; q++) {
+                    const int l = rorder[q];
+                    if (PSV && a.rec_u) {
+                        ru[row + l] = tU[e] * A[k];
+                        rw[row + l] = tW[e] * A[k];
+                    }
+                    if (SH && a.rec_v) rv[row + l] = tV[e] * A[k];
+                }
+            }
+        }
+        started = true;
+        __syncthreads();
+    }
+
+    // ---- the tile's states (post-sponge velocities)
+    float* const* out = a.flip ? a.st : a.alt;
+    float* const* dout = a.flip ? a.dst : a.dalt;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        if ((fl[k] & 5u) != 5u) continue;
+        const int e = t + NT * k;
+        const int c = gc[k];
+        if (PSV) {
+            st(out[0], c, sU[e] * A[k]);
+            st(out[2], c, sW[e] * A[k]);
+            st(out[3], c, sXX[e]);
+            st(out[4], c, sZZ[e]);
+            st(out[5], c, sXZ[e]);
+            st(dout[0], c, tU[e] * A[k]);
+            st(dout[2], c, tW[e] * A[k]);
+            st(dout[3], c, tXX[e]);
+            st(dout[4], c, tZZ[e]);
+            st(dout[5], c, tXZ[e]);
+        }
+        if (SH) {
+            st(out[1], c, sV[e] * A[k]);
+            st(out[6], c, sYX[e]);
+            st(out[7], c, sYZ[e]);
+            st(dout[1], c, tV[e] * A[k]);
+            st(dout[6], c, tYX[e]);
+            st(dout[7], c, tYZ[e]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 template <bool PSV, bool SH, int TX, int TZ, int K>
@@ -921,6 +1249,154 @@ static int fwi_block_run(const trtm_fwi_desc* d, int n, int it_begin, int it_end
                 hipMemcpyAsync(fb[q].st[f], fb[q].alt[f], bytes, hipMemcpyDeviceToDevice, s);
             if (e != hipSuccess)
                 return fail(TRTM_ERR_HIP, "blocked FWI state copy: %s", hipGetErrorString(e));
+        }
+    }
+    return TRTM_OK;
+}
+
+// ---- scattered-field forward
+
+template <bool PSV, bool SH, int TX, int TZ, int K>
+constexpr bool scat_fits() {
+    return 4 * (size_t)(TX + 2 * K) * (TZ + 2 * K) * ((PSV ? 10 : 0) + (SH ? 6 : 0)) <= kLdsLimit;
+}
+
+static bool scat_cfg_fits(int systems, const BlockCfg& c) {
+    const int nf = ((systems & TRTM_PSV) ? 10 : 0) + ((systems & TRTM_SH) ? 6 : 0);
+    return 4 * (size_t)(c.tx + 2 * c.k) * (c.tz + 2 * c.k) * nf <= kLdsLimit;
+}
+
+// the forward default when it fits, else the first listed shape that fits
+// and matches every nonzero request
+static int scat_choose(const trtm_fwi_desc* d, BlockCfg* out) {
+    const BlockCfg& def = kFwiDefaultFwd;
+    if (d->tile_x <= 0 && d->tile_z <= 0 && d->block_steps <= 0 && scat_cfg_fits(d->systems, def)) {
+        *out = def;
+        return TRTM_OK;
+    }
+    for (const BlockCfg& c : kFwiCfg)
+        if (scat_cfg_fits(d->systems, c) && (d->tile_x <= 0 || c.tx == d->tile_x) &&
+            (d->tile_z <= 0 || c.tz == d->tile_z) &&
+            (d->block_steps <= 0 || c.k == d->block_steps)) {
+            *out = c;
+            return TRTM_OK;
+        }
+    return fail(TRTM_ERR_UNSUPPORTED, "scattered FWI kernel has no tile_x=%d tile_z=%d "
+                "block_steps=%d build for these wave systems", d->tile_x, d->tile_z,
+                d->block_steps);
+}
+
+template <bool PSV, bool SH>
+static int launch_scat_cfg(const BlockCfg& c, const ByArray4<FS>& src, int shots, int t_a,
+                           int t_b, int nx, int nz, hipStream_t s) {
+    const int tx_ = (nx + c.tx - 1) / c.tx, tz_ = (nz + c.tz - 1) / c.tz;
+    const int tiles = tx_ * tz_;
+    const dim3 grid(8 * ((tiles + 7) / 8), 1, shots);
+#define TRTM_FS(TX, TZ, NT, K)                                                                  \
+    if constexpr (scat_fits<PSV, SH, TX, TZ, K>()) {                                            \
+        if (c.tx == TX && c.tz == TZ && c.nt == NT && c.k == K) {                               \
+            hipLaunchKernelGGL((k_fwi_scat_blk<PSV, SH, TX, TZ, NT, K, ByArray4<FS>>), grid,    \
+                               dim3(NT), 0, s, src, t_a, t_b, tx_, tz_);                        \
+            return TRTM_OK;                                                                     \
+        }                                                                                       \
+    }
+    TRTM_FS(24, 64, 768, 4)
+    TRTM_FS(32, 64, 1024, 4)
+    TRTM_FS(16, 64, 512, 4)
+    TRTM_FS(16, 64, 512, 8)
+    TRTM_FS(32, 64, 1024, 2)
+    TRTM_FS(16, 64, 256, 2)
+    TRTM_FS(16, 48, 512, 4)
+    TRTM_FS(24, 32, 512, 4)
+#undef TRTM_FS
+    return fail(TRTM_ERR_UNSUPPORTED, "scattered FWI shape %dx%d/%d/%d not compiled", c.tx, c.tz,
+                c.nt, c.k);
+}
+
+static FS make_fs(const trtm_fwi_desc* d) {
+    FS a;
+    a.nx = d->nx; a.nz = d->nz; a.nt = d->nt; a.fw = d->fw;
+    a.n_rec = d->n_rec; a.flip = 0; a.pad0 = a.pad1 = 0;
+    a.dt = (float)d->dt; a.idx = (float)d->inv_dx; a.idz = (float)d->inv_dz;
+    a.factor = (float)d->src_factor;
+    a.tab = (const float*)d->sponge_table;
+    a.lam = (const float*)d->lam; a.mu = (const float*)d->mu; a.mxz = (const float*)d->mxz;
+    a.myx = (const float*)d->myx; a.myz = (const float*)d->myz; a.ir = (const float*)d->inv_rho;
+    a.iru = (const float*)d->inv_rho_u; a.irw = (const float*)d->inv_rho_w;
+    a.elam = (const float*)d->dif_lam; a.emu = (const float*)d->dif_mu;
+    a.emxz = (const float*)d->dif_mxz; a.emyx = (const float*)d->dif_myx;
+    a.emyz = (const float*)d->dif_myz; a.eir = (const float*)d->dif_inv_rho;
+    a.eiru = (const float*)d->dif_inv_rho_u; a.eirw = (const float*)d->dif_inv_rho_w;
+    void* const fwd[8] = {d->u, d->v, d->w, d->sxx, d->szz, d->sxz, d->syx, d->syz};
+    void* const sc[8] = {d->du, d->dv, d->dw, d->dsxx, d->dszz, d->dsxz, d->dsyx, d->dsyz};
+    const size_t n = (size_t)d->nx * d->nz;
+    for (int f = 0; f < 8; f++) {
+        a.st[f] = (float*)fwd[f];
+        a.alt[f] = (float*)d->workspace + f * n;
+        a.dst[f] = (float*)sc[f];
+        a.dalt[f] = (float*)d->scat_workspace + f * n;
+    }
+    a.src_map = d->n_src_groups > 0 ? d->cell_map : nullptr;
+    a.rec_map = d->n_rec_groups > 0 && (d->rec_u || d->rec_v) ? d->cell_map + n : nullptr;
+    a.src_order = d->src_order; a.src_start = d->src_group_start;
+    a.rec_order = d->rec_order; a.rec_start = d->rec_group_start;
+    a.wav_u = (const float*)d->wav_u; a.wav_v = (const float*)d->wav_v;
+    a.wav_w = (const float*)d->wav_w;
+    a.rec_u = (float*)d->rec_u; a.rec_v = (float*)d->rec_v; a.rec_w = (float*)d->rec_w;
+    return a;
+}
+
+// Steps [it_begin, it_end) of n shots in launches of K steps (an even number,
+// the first split when needed), up to kScatShotsPerLaunch shots per launch,
+// each group of shots through the whole range before the next.
+int fwi_scat_forward(const trtm_fwi_desc* d, int n, int it_begin, int it_end, hipStream_t s) {
+    if (it_begin >= it_end) return TRTM_OK;
+    BlockCfg c;
+    int rc = scat_choose(d, &c);
+    if (rc) return rc;
+    std::vector<FS> fs(n);
+    for (int q = 0; q < n; q++) fs[q] = make_fs(&d[q]);
+    const int K = c.k, groups = (n + kScatShotsPerLaunch - 1) / kScatShotsPerLaunch;
+    std::vector<std::pair<int, int>> chunks;
+    for (int l = 0; l * K < it_end - it_begin; l++)
+        chunks.push_back({it_begin + l * K, std::min(it_begin + (l + 1) * K, it_end)});
+    if (chunks.size() % 2 == 1 && chunks[0].second - chunks[0].first > 1) {
+        const auto first = chunks[0];
+        chunks[0] = {first.first, first.first + 1};
+        chunks.insert(chunks.begin() + 1, {first.first + 1, first.second});
+    }
+    for (int g = 0; g < groups; g++) {
+        const int q0 = g * kScatShotsPerLaunch, m = std::min(kScatShotsPerLaunch, n - q0);
+        for (const auto& ch : chunks) {
+            ByArray4<FS> arr;
+            for (int q = 0; q < m; q++) arr.v[q] = fs[q0 + q];
+            for (int q = m; q < kScatShotsPerLaunch; q++) arr.v[q] = fs[q0];
+            if (d->systems == (TRTM_PSV | TRTM_SH))
+                rc = launch_scat_cfg<true, true>(c, arr, m, ch.first, ch.second, d->nx, d->nz, s);
+            else if (d->systems == TRTM_PSV)
+                rc = launch_scat_cfg<true, false>(c, arr, m, ch.first, ch.second, d->nx, d->nz, s);
+            else
+                rc = launch_scat_cfg<false, true>(c, arr, m, ch.first, ch.second, d->nx, d->nz, s);
+            if (rc) return rc;
+            for (int q = q0; q < q0 + m; q++)
+                if (ch.first < fs[q].nt) fs[q].flip ^= 1;
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return fail(TRTM_ERR_HIP, "scattered FWI launch: %s", hipGetErrorString(e));
+    }
+    const size_t bytes = (size_t)d->nx * d->nz * sizeof(float);
+    for (int q = 0; q < n; q++) {
+        if (!fs[q].flip) continue;
+        for (int f = 0; f < 8; f++) {
+            const bool psv_f = f == 0 || f == 2 || (f >= 3 && f <= 5);
+            if (!(psv_f ? (d->systems & TRTM_PSV) : (d->systems & TRTM_SH))) continue;
+            hipError_t e =
+                hipMemcpyAsync(fs[q].st[f], fs[q].alt[f], bytes, hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(fs[q].dst[f], fs[q].dalt[f], bytes, hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess)
+                return fail(TRTM_ERR_HIP, "scattered FWI state copy: %s", hipGetErrorString(e));
         }
     }
     return TRTM_OK;

@@ -119,7 +119,7 @@ class ElasticFWI:
         # checkpointing forward, the forward that records the history, the
         # adjoint (0 = library default)
         shape = (int(tile_x), int(tile_z), int(block_steps))
-        self.shapes = {"forward": shape, "history": shape, "adjoint": shape}
+        self.shapes = {"forward": shape, "history": shape, "adjoint": shape, "scattered": shape}
         self.nx, self.nz = int(nx), int(nz)
         self.dx, self.dz, self.fs = float(dx), float(dz), float(fs)
         self.fw = int(absorbing_frame)
@@ -182,16 +182,19 @@ class ElasticFWI:
 
     # -- medium ---------------------------------------------------------------
 
-    def _medium(self, vp, vs, rho):
+    def _medium(self, vp, vs, rho, dtype=None):
+        """Moduli, averages and reciprocals of (vp, vs, rho) in ``dtype``
+        (default: the problem's precision)."""
         nx, nz = self.nx, self.nz
-        vp = vp.to(device=device(), dtype=self.dtype).contiguous()
-        vs = vs.to(device=device(), dtype=self.dtype).contiguous()
-        rho = rho.to(device=device(), dtype=self.dtype).contiguous()
+        dtype = dtype or self.dtype
+        vp = vp.to(device=device(), dtype=dtype).contiguous()
+        vs = vs.to(device=device(), dtype=dtype).contiguous()
+        rho = rho.to(device=device(), dtype=dtype).contiguous()
         mu, lam = moduli(vp, vs, rho)
         mu, lam = mu.contiguous(), lam.contiguous()
-        c = {k: torch.empty((nx, nz), dtype=self.dtype, device=vp.device)
+        c = {k: torch.empty((nx, nz), dtype=dtype, device=vp.device)
              for k in ("myx", "myz", "mxz", "rho_u", "rho_w", "inv_rho", "inv_rho_u", "inv_rho_w")}
-        fn = "trtm_prepare_medium" if self.rb == 4 else "trtm_prepare_medium_f64"
+        fn = "trtm_prepare_medium" if dtype == torch.float32 else "trtm_prepare_medium_f64"
         _lib.call(fn, nx, nz, _ptr(mu), _ptr(rho), _ptr(c["myx"]), _ptr(c["myz"]), _ptr(c["mxz"]),
                   _ptr(c["rho_u"]), _ptr(c["rho_w"]), _ptr(c["inv_rho"]), _ptr(c["inv_rho_u"]),
                   _ptr(c["inv_rho_w"]), stream_handle())
@@ -257,8 +260,12 @@ class ElasticFWI:
             self._call_batch(fn, descs, a, b, shape[1:])
             reset()
 
-        shape = autotune.tune(key, self.systems, trial,
-                              shapes=autotune.fwi_candidates(self.systems))
+        shapes = autotune.fwi_candidates(self.systems)
+        if name == "scattered":  # twice the forward's LDS: the scattered state beside the state
+            nf = (10 if self.systems & _lib.PSV else 0) + (6 if self.systems & _lib.SH else 0)
+            shapes = [c for c in shapes if 4 * (c[1] + 2 * c[3]) * (c[2] + 2 * c[3]) * nf
+                      <= 160 * 1024]
+        shape = autotune.tune(key, self.systems, trial, shapes=shapes)
         self.tuned[name] = shape[1:]
         self.shapes[name] = shape[1:]
         reset()
@@ -269,10 +276,20 @@ class ElasticFWI:
         arr = (_lib.FwiDesc * len(descs))(*descs)
         _lib.call(fn, arr, len(descs), a, b, stream_handle())
 
-    def forward(self, vp, vs, rho):
+    def forward(self, vp, vs, rho, reference=None):
         """Records of every shot, a list of (3, nr, nt) tensors (u, v, w);
         keeps the checkpoints for ``gradient``.  All shots advance together
-        (one batched call per checkpoint segment)."""
+        (one batched call per checkpoint segment).
+
+        reference: (vp, vs, rho) of the model whose records are the observed
+        data.  The records returned are then the residuals d(model) -
+        d(reference), propagated as a scattered field beside this model's
+        wavefield (trtm_fwi_forward_scattered_batch; float32 blocked kernels),
+        so they carry float32 rounding relative to the residual itself, not to
+        the records; the checkpoints are this model's, bit for bit, and no
+        history is kept (every segment is recomputed by ``gradient``)."""
+        if reference is not None:
+            return self._forward_scattered(vp, vs, rho, reference)
         dev = device()
         self.generation += 1
         for g in self.geo:  # release() dropped the blocked kernels' second state buffer
@@ -333,6 +350,78 @@ class ElasticFWI:
                 del start
             self._call_batch("trtm_fwi_forward_batch", descs, a, b, self.shapes["history"])
         return [torch.stack([rec[c].t() for c in ("u", "v", "w")]).contiguous() for rec in recs]
+
+    def residual(self, vp, vs, rho, reference):
+        """d(model) - d(reference) per shot, (3, nr, nt) each, and the
+        checkpoints for ``gradient`` (``forward`` with ``reference``)."""
+        return self.forward(vp, vs, rho, reference=reference)
+
+    def _forward_scattered(self, vp, vs, rho, reference):
+        if not (self.blocked and self.precision == "float32"):
+            raise ValueError("the scattered-field residual runs the float32 blocked kernels "
+                             "(precision='float32', kernel 'auto', 'blocked' or 'tuned')")
+        dev = device()
+        self.generation += 1
+        for g in self.geo:
+            if g["ws"] is None:
+                g["ws"] = torch.empty(8 * self.nx * self.nz, dtype=self.dtype, device=dev)
+        self.systems = self._systems()
+        self.med = self._medium(vp, vs, rho)
+        # the models' coefficient differences, formed in double and rounded
+        # once: they are the whole source of the scattered field, so rounding
+        # each model's coefficients to float32 first would add noise wherever
+        # the two differ by less than a float32 ulp
+        m64 = self._medium(vp, vs, rho, torch.float64)
+        r64 = self._medium(*(torch.as_tensor(x) for x in reference), dtype=torch.float64)
+        dif = {}
+        for k in ("lam", "mu", "mxz", "myx", "myz", "inv_rho", "inv_rho_u", "inv_rho_w"):
+            a, b = m64[k], r64[k]
+            dif[k] = torch.where(a == b, torch.zeros_like(a), a - b).to(self.dtype).contiguous()
+        del m64, r64
+        names = self._fields()
+        nt_max = max(g["nt"] for g in self.geo)
+        n_hist = (5 if self.systems & _lib.PSV else 0) + (3 if self.systems & _lib.SH else 0)
+        self.L = self.checkpoint_every or checkpoint_interval(nt_max, len(names), n_hist)
+        shape = (self.nx, self.nz)
+        self.edges = list(range(0, nt_max, self.L)) + [nt_max]
+        nseg = len(self.edges) - 1
+        S = len(self.geo)
+        self._kept = []
+        self._ck = None
+        self.kept_segments = 0
+        self.state_all = torch.zeros((S, len(names)) + shape, dtype=self.dtype, device=dev)
+        self.state = [{k: self.state_all[q, m] for m, k in enumerate(names)} for q in range(S)]
+        dstate = torch.zeros_like(self.state_all)
+        dws = [torch.empty(8 * self.nx * self.nz, dtype=self.dtype, device=dev) for _ in range(S)]
+        self._ck = torch.empty((nseg,) + tuple(self.state_all.shape), dtype=self.dtype, device=dev)
+        recs, descs = [], []
+        for q, (g, st) in enumerate(zip(self.geo, self.state)):
+            d = self._desc(self.med, g)
+            for m, k in enumerate(names):
+                setattr(d, k, _ptr(st[k]))
+                setattr(d, "d" + k, _ptr(dstate[q, m]))
+            for k in ("lam", "mu", "mxz", "myx", "myz", "inv_rho", "inv_rho_u", "inv_rho_w"):
+                setattr(d, "dif_" + k, _ptr(dif[k]))
+            d.scat_workspace = _ptr(dws[q])
+            rec = {c: torch.zeros((g["nt"], g["nr"]), dtype=self.dtype, device=dev)
+                   for c in ("u", "v", "w")}
+            d.rec_u, d.rec_v, d.rec_w = (_ptr(rec[c]) for c in ("u", "v", "w"))
+            recs.append(rec)
+            descs.append(d)
+
+        def reset():
+            self.state_all.zero_()
+            dstate.zero_()
+        self._tune("scattered", "trtm_fwi_forward_scattered_batch", descs, self.edges[0],
+                   self.edges[1], reset)
+        for k in range(nseg):
+            a, b = self.edges[k], self.edges[k + 1]
+            self._ck[k].copy_(self.state_all)
+            self._call_batch("trtm_fwi_forward_scattered_batch", descs, a, b,
+                             self.shapes["scattered"])
+        out = [torch.stack([rec[c].t() for c in ("u", "v", "w")]).contiguous() for rec in recs]
+        del dstate, dws, dif
+        return out
 
     # -- gradient ---------------------------------------------------------------
 
@@ -475,11 +564,22 @@ class Seismograms(torch.autograd.Function):
     (a residual) carries no float32 rounding of the forward runs; ``problem``
     still runs its forward pass for the adjoint's checkpoints and history,
     and backward runs ``problem``'s adjoint on the incoming gradient rounded
-    to its precision."""
+    to its precision.
+
+    reference (optional): (vp, vs, rho) of the model the observed data come
+    from.  The output is then the residual d(vp, vs, rho) - d(reference),
+    propagated as a scattered field (``ElasticFWI.residual``), so a loss
+    1/2 |output|^2 is the L2 misfit against those data."""
 
     @staticmethod
-    def forward(ctx, vp, vs, rho, problem: ElasticFWI, records_from=None):
-        recs = problem.forward(vp.detach(), vs.detach(), rho.detach())
+    def forward(ctx, vp, vs, rho, problem: ElasticFWI, records_from=None, reference=None):
+        if reference is not None:
+            if records_from is not None:
+                raise ValueError("give records_from or reference, not both")
+            recs = problem.forward(vp.detach(), vs.detach(), rho.detach(),
+                                   reference=[torch.as_tensor(x).detach() for x in reference])
+        else:
+            recs = problem.forward(vp.detach(), vs.detach(), rho.detach())
         out_dtype = vp.dtype
         if records_from is not None:
             if records_from.dtype != torch.float64:
@@ -501,11 +601,11 @@ class Seismograms(torch.autograd.Function):
                                "Seismograms again before calling backward")
         g_vp, g_vs, g_rho = ctx.problem.gradient([grad[q] for q in range(grad.shape[0])])
         return (g_vp.to(ctx.dtypes[0]), g_vs.to(ctx.dtypes[1]), g_rho.to(ctx.dtypes[2]), None,
-                None)
+                None, None)
 
 
-def l2_misfit_and_gradient(problem: ElasticFWI, vp, vs, rho, observed,
-                           records_from: Optional[ElasticFWI] = None):
+def l2_misfit_and_gradient(problem: ElasticFWI, vp, vs, rho, observed=None,
+                           records_from: Optional[ElasticFWI] = None, reference=None):
     """J = 1/2 sum (d - d_obs)^2 over every shot, component, receiver and
     sample, and its gradient (vp, vs, rho) in float64.
 
@@ -514,7 +614,19 @@ def l2_misfit_and_gradient(problem: ElasticFWI, vp, vs, rho, observed,
     ``problem``'s precision once; ``problem`` still runs its own forward pass
     for the adjoint's checkpoints and history, and the adjoint.  A float32
     problem then keeps its adjoint's accuracy where the misfit is so small
-    that float32 rounding of the records would dominate the residual."""
+    that float32 rounding of the records would dominate the residual.
+
+    reference: (vp, vs, rho) of the model whose records are the observed data
+    (a synthetic inversion).  The residual d(model) - d(reference) is then
+    propagated as a scattered field in float32 (``ElasticFWI.residual``):
+    no float64 run, and no cancellation between two float32 record sets;
+    ``observed`` is not used."""
+    if reference is not None:
+        res = problem.residual(vp, vs, rho, reference)
+        J = 0.5 * sum(float((x.double() ** 2).sum()) for x in res)
+        return J, problem.gradient(res)
+    if observed is None:
+        raise ValueError("observed data or a reference model is needed")
     if records_from is None:
         recs = problem.forward(vp, vs, rho)
         res = [r - torch.as_tensor(o, device=r.device).to(r.dtype) for r, o in zip(recs, observed)]

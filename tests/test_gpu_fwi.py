@@ -163,6 +163,103 @@ def test_full_size_gradient_error_against_float64(workload):
         print("[fp32 vs fp64] %s %s gradient %s: float64 residual %.2e, end to end %.2e"
               % (workload, size, name, same_res, end), flush=True)
         assert same_res < 1e-4
+    # float32 end to end with the residual propagated as a scattered field
+    # (reference = the true model): no float64 run
+    m32 = [_t(x, torch.float32) for x in (start.vp, start.vs, start.rho)]
+    ref32 = [_t(x, torch.float32) for x in (true.vp, true.vs, true.rho)]
+    J_s, g_s = l2_misfit_and_gradient(p32, *m32, reference=ref32)
+    res_s = p32.residual(*m32, ref32)[0]
+    e_res_s = rel(res_s, res64)
+    print("[fp32 scattered vs fp64] %s %s residual %.2e" % (workload, size, e_res_s), flush=True)
+    errs = []
+    for name, a, b in zip(("vp", "vs", "rho"), g_s, g64):
+        errs.append(rel(a, b))
+        print("[fp32 scattered vs fp64] %s %s gradient %s end to end %.2e"
+              % (workload, size, name, errs[-1]), flush=True)
+    assert e_res_s < 1e-3 and max(errs) < 1e-3
+
+
+@pytest.mark.parametrize("system", ["psv", "sh", "both"])
+def test_scattered_residual_against_float64_difference(system):
+    """ElasticFWI.residual (the scattered-field forward): this model's
+    checkpoints equal the plain forward's bit for bit; the residual is zero
+    exactly when the reference is the model itself; against the float64
+    difference of the two models' records it is far closer than the float32
+    difference of two float32 record sets; the gradient from it matches the
+    float64 gradient."""
+    _case("FWI scattered residual %s" % system)
+    grid, shots, m0, mt = problem(system, nt=200)
+
+    def rel(a, b):
+        return float(torch.linalg.norm((a.double() - b.double()).ravel())
+                     / torch.linalg.norm(b.double().ravel()))
+
+    p64 = ElasticFWI(*grid, shots, precision="float64", checkpoint_every=40, history_bytes=0)
+    obs64 = [o.clone() for o in p64.forward(*[_t(x) for x in mt])]
+    J64, g64 = l2_misfit_and_gradient(p64, *[_t(x) for x in m0], obs64)
+    syn64 = [o.clone() for o in p64.forward(*[_t(x) for x in m0])]
+    res64 = [a - b for a, b in zip(syn64, obs64)]
+
+    p = ElasticFWI(*grid, shots, precision="float32", checkpoint_every=40, history_bytes=0,
+                   kernel="blocked")
+    m32 = [_t(x, torch.float32) for x in m0]
+    r32 = [_t(x, torch.float32) for x in mt]
+    obs32 = [o.clone() for o in p.forward(*r32)]
+    syn32 = [o.clone() for o in p.forward(*m32)]
+    ck_plain = p._ck.clone()
+    res_s = p.residual(*m32, r32)
+    assert torch.equal(p._ck, ck_plain)
+    e_plain = max(rel(a - b, c) for a, b, c in zip(syn32, obs32, res64))
+    e_scat = max(rel(a, c) for a, c in zip(res_s, res64))
+    print("[scattered] %s residual vs float64: plain float32 difference %.2e, scattered %.2e"
+          % (system, e_plain, e_scat), flush=True)
+    assert e_scat < 0.5 * e_plain
+    zero = p.residual(*m32, m32)
+    assert all(float(z.abs().max()) == 0.0 for z in zero)
+    J, g = l2_misfit_and_gradient(p, *m32, reference=r32)
+    assert abs(J - J64) <= 1e-4 * J64
+    for name, a, b in zip(("vp", "vs", "rho"), g, g64):
+        if system == "sh" and name == "vp":  # SH does not see vp: both are zero
+            assert float(a.abs().max()) == 0.0 and float(b.abs().max()) == 0.0
+            continue
+        e = rel(a, b)
+        print("[scattered] %s gradient %s vs float64 %.2e" % (system, name, e), flush=True)
+        assert e < 1e-3
+
+
+def test_scattered_residual_same_bits_for_every_shape():
+    """Every compiled scattered-kernel shape that fits gives the same residual
+    bits (the same expressions per cell in the same order)."""
+    _case("FWI scattered shapes")
+    grid, shots, m0, mt = problem("psv", nx=90, nt=97, n_shots=3)
+    m32 = [_t(x, torch.float32) for x in m0]
+    r32 = [_t(x, torch.float32) for x in mt]
+    ref = None
+    for tx, tz, k in ((24, 64, 4), (32, 64, 4), (16, 64, 8), (16, 64, 2), (16, 48, 4),
+                      (24, 32, 4)):
+        p = ElasticFWI(*grid, shots, precision="float32", checkpoint_every=30, history_bytes=0,
+                       kernel="blocked", tile_x=tx, tile_z=tz, block_steps=k)
+        res = torch.stack(p.residual(*m32, r32))
+        if ref is None:
+            ref = res
+            assert float(res.abs().max()) > 0
+        else:
+            assert torch.equal(res, ref), (tx, tz, k)
+
+
+def test_autograd_with_reference_model():
+    """Seismograms.apply(..., reference=...) outputs the residual; the
+    gradient of 1/2 |output|^2 equals l2_misfit_and_gradient's."""
+    _case("FWI autograd, scattered residual")
+    grid, shots, m0, mt = problem("psv", nt=120)
+    p = ElasticFWI(*grid, shots, precision="float32", checkpoint_every=30, kernel="blocked")
+    params = [_t(x, torch.float32).requires_grad_(True) for x in m0]
+    ref = [_t(x, torch.float32) for x in mt]
+    d = Seismograms.apply(*params, p, None, ref)
+    (0.5 * (d ** 2).sum()).backward()
+    J, g = l2_misfit_and_gradient(p, *[x.detach() for x in params], reference=ref)
+    for a, b in zip(params, g):
+        assert torch.equal(a.grad, b.to(torch.float32))
 
 
 def test_gradient_independent_of_checkpoint_interval():
