@@ -163,6 +163,56 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
     const size_t sstride = (size_t)sh->snap_stride;
     const bool ssponge = sh->snap_sponge != 0;
 
+    // The tile's injection groups and receivers, looked up once per launch:
+    // thread t takes group ig0 + t and receiver rb0 + t (lists no longer than
+    // the workgroup; a longer list keeps the per-step loop below).  A group
+    // with one entry also keeps its data row, so a step reads its injected
+    // value with one load, issued before the velocity update.  Only for
+    // shapes with at most two cells per thread: their few workgroups per CU
+    // leave the per-step lookups' latency exposed, and they have registers to
+    // spare; the larger shapes keep the lookups per step.
+    constexpr bool HOIST = NC <= 2;
+    int ig0 = 0, ig1 = 0, rb0 = 0, rb1 = 0;
+    if (n_groups > 0) {
+        const cptr<int> tstart = C(sh->inj_tstart);
+        ig0 = tstart[tile];
+        ig1 = tstart[tile + 1];
+    }
+    if (n_rec > 0) {
+        const cptr<int> rstart = C(sh->rec_tstart);
+        rb0 = rstart[tile];
+        rb1 = rstart[tile + 1];
+    }
+    const bool inj_one_pass = HOIST && ig1 - ig0 <= NT, rec_one_pass = HOIST && rb1 - rb0 <= NT;
+    // this thread's group: LDS index, and its data row (one entry) or -1 - g
+    // (several entries, summed per step in entry order); forward coefficients
+    int g_e = -1, g_row = -1;
+    float g_cu = 0.0f, g_cw = 0.0f, g_cv = 0.0f;
+    if (inj_one_pass && ig0 + t < ig1) {
+        const int g = C(sh->inj_titems)[ig0 + t];
+        const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
+        const int k0 = C(sh->order)[q0];
+        const int ci = C(sh->loc)[2 * k0], cj = C(sh->loc)[2 * k0 + 1];
+        g_e = (ci - ri0) * RZ + (cj - rj0);
+        g_row = q1 - q0 == 1 ? k0 : -1 - g;
+        const int c = ci * nz + cj;
+        if (FWD) {
+            if (PSV) {
+                g_cu = G(m.iru)[c];
+                g_cw = G(m.irw)[c];
+            }
+            if (SH) g_cv = G(m.ir)[c];
+        }
+    }
+    int r_l = -1, r_e = 0;  // this thread's receiver
+    float r_A = 0.0f;
+    if (rec_one_pass && rb0 + t < rb1) {
+        r_l = C(sh->rec_titems)[rb0 + t];
+        const int ci = C(sh->rec_loc)[2 * r_l], cj = C(sh->rec_loc)[2 * r_l + 1];
+        r_e = (ci - ri0) * RZ + (cj - rj0);
+        r_A = sponge_at(ci, cj, nx, nz, m.fw, m.tab);
+    }
+
     bool started = false;  // a step ran in this launch: LDS velocities await the sponge
     for (int s = 0; s < K; s++) {
         const int it = it0 + it_off + s;
@@ -178,9 +228,20 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         }
         if (imaging && tb > 0 && tb % isnap == 0) {
             const int sl = tb / isnap - 1 - snap_base;
-            if (sl >= 0 && sl < num_snaps && G(sh->isnaps)[sl] == tb) img = sl;
+            if (sl >= 0 && sl < num_snaps && C(sh->isnaps)[sl] == tb) img = sl;
         }
         if (cint > 0 && it % cint == 0) chk = it / cint;
+        // this step's injected value of a single-entry group, loaded now and
+        // added after the velocity update
+        float gvu = 0.0f, gvv = 0.0f, gvw = 0.0f;
+        if (g_row >= 0) {
+            const size_t o = (size_t)g_row * (size_t)nt + (FWD ? it : tb);
+            if (PSV) {
+                gvu = C(sh->su)[o];
+                gvw = C(sh->sw)[o];
+            }
+            if (SH) gvv = C(sh->sv)[o];
+        }
         // SH imaging: the forward wavefield of every slot is loaded here, a
         // whole phase before the image update uses it (cells outside the grid
         // read cell 0; only the tile's cells use the value)
@@ -302,16 +363,60 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         __syncthreads();
 
         // ---- injection into the region (one thread per cell, entries in order)
-        if (n_groups > 0) {
-            const gptr<const int> tstart = G(sh->inj_tstart);
-            const int b = tstart[tile], end = tstart[tile + 1];
-            if (b < end) {
+        if (ig1 > ig0) {
+            const gptr<const float> su = G(sh->su), sv = G(sh->sv), sw = G(sh->sw);
+            const int col = FWD ? it : tb;
+            const size_t ntt = (size_t)nt;
+            if (inj_one_pass) {
+                if (g_e >= 0) {
+                    const int e = g_e;
+                    const cptr<int> order = C(sh->order);
+                    if (PSV) {
+                        float x = sU[e], zz = sW[e];
+                        if (g_row >= 0) {
+                            if (FWD) {
+                                x = x + (gvu * m.factor) * g_cu;
+                                zz = zz + (gvw * m.factor) * g_cw;
+                            } else {
+                                x = x + gvu;
+                                zz = zz + gvw;
+                            }
+                        } else {
+                            const int g = -1 - g_row;
+                            const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
+                            for (int q2 = q0; q2 < q1; q2++) {
+                                const size_t o = (size_t)order[q2] * ntt + col;
+                                if (FWD) {
+                                    x = x + (su[o] * m.factor) * g_cu;
+                                    zz = zz + (sw[o] * m.factor) * g_cw;
+                                } else {
+                                    x = x + su[o];
+                                    zz = zz + sw[o];
+                                }
+                            }
+                        }
+                        sU[e] = x;
+                        sW[e] = zz;
+                    }
+                    if (SH) {
+                        float y = sV[e];
+                        if (g_row >= 0) {
+                            y = FWD ? y + (gvv * m.factor) * g_cv : y + gvv;
+                        } else {
+                            const int g = -1 - g_row;
+                            const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
+                            for (int q2 = q0; q2 < q1; q2++) {
+                                const size_t o = (size_t)order[q2] * ntt + col;
+                                y = FWD ? y + (sv[o] * m.factor) * g_cv : y + sv[o];
+                            }
+                        }
+                        sV[e] = y;
+                    }
+                }
+            } else {
                 const gptr<const int> titems = G(sh->inj_titems), gstart = G(sh->start);
                 const gptr<const int> order = G(sh->order), loc = G(sh->loc);
-                const gptr<const float> su = G(sh->su), sv = G(sh->sv), sw = G(sh->sw);
-                const int col = FWD ? it : tb;
-                const size_t ntt = (size_t)nt;
-                for (int q = b + t; q < end; q += NT) {
+                for (int q = ig0 + t; q < ig1; q += NT) {
                     const int g = titems[q];
                     const int e0 = gstart[g], e1 = gstart[g + 1];
                     const int k0 = order[e0];
@@ -342,8 +447,8 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                         sV[e] = y;
                     }
                 }
-                __syncthreads();
             }
+            __syncthreads();
         }
 
         // ---- stress update + sponge (region), outputs (tile)
@@ -479,13 +584,20 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         if (code) atomicMax(sh->check_flags + chk, code + (FWD ? 0 : 3));
 
         // ---- receivers of the tile, after the sponge
-        if (n_rec > 0) {
-            const gptr<const int> rstart = G(sh->rec_tstart);
-            const int b = rstart[tile], end = rstart[tile + 1];
-            if (b < end) {
+        if (rb1 > rb0) {
+            const gptr<float> ru = G(sh->rec_u), rv = G(sh->rec_v), rw = G(sh->rec_w);
+            const size_t row = (size_t)(FWD ? it : tb) * n_rec;
+            if (rec_one_pass) {
+                if (r_l >= 0) {
+                    if (PSV) {
+                        ru[row + r_l] = sU[r_e] * r_A;
+                        rw[row + r_l] = sW[r_e] * r_A;
+                    }
+                    if (SH) rv[row + r_l] = sV[r_e] * r_A;
+                }
+            } else {
+                const int b = rb0, end = rb1;
                 const gptr<const int> ritems = G(sh->rec_titems), rloc = G(sh->rec_loc);
-                const gptr<float> ru = G(sh->rec_u), rv = G(sh->rec_v), rw = G(sh->rec_w);
-                const size_t row = (size_t)(FWD ? it : tb) * n_rec;
                 for (int q = b + t; q < end; q += NT) {
                     const int l = ritems[q];
                     const int ci = rloc[2 * l], cj = rloc[2 * l + 1];
