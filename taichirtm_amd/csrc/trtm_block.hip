@@ -55,6 +55,10 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
     float* const sYX = sYX_ + OS;
     float* const sYZ = sYZ_ + OS;
 
+    // small shapes: injection group of each region cell (see below)
+    constexpr bool HOIST = NC <= 2;
+    __shared__ int gmap[HOIST ? RN : 1];
+
     const ShotDev* __restrict__ sh = shots + blockIdx.y;
     const int it_end = sh->it_end, it_lo = sh->it_lo, it_off = sh->it_off;
     const int tiles = tiles_x * tiles_z;
@@ -163,15 +167,16 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
     const size_t sstride = (size_t)sh->snap_stride;
     const bool ssponge = sh->snap_sponge != 0;
 
-    // The tile's injection groups and receivers, looked up once per launch:
-    // thread t takes group ig0 + t and receiver rb0 + t (lists no longer than
-    // the workgroup; a longer list keeps the per-step loop below).  A group
-    // with one entry also keeps its data row, so a step reads its injected
-    // value with one load, issued before the velocity update.  Only for
-    // shapes with at most two cells per thread: their few workgroups per CU
-    // leave the per-step lookups' latency exposed, and they have registers to
-    // spare; the larger shapes keep the lookups per step.
-    constexpr bool HOIST = NC <= 2;
+    // The tile's injection groups and receivers, looked up once per launch
+    // (lists no longer than the workgroup; a longer list keeps the per-step
+    // loops below): each region cell's thread slot learns its group (the data
+    // row of a single entry, or the group of several) and adds the injection
+    // right after its own velocity update, with no separate injection phase
+    // and barrier, from a value loaded at the start of the step; thread t
+    // records receiver rb0 + t.  Only for shapes with at most two cells per
+    // thread: their few workgroups per CU leave the per-step lookups' latency
+    // exposed, and they have registers to spare; the larger shapes keep the
+    // per-step lookups.
     int ig0 = 0, ig1 = 0, rb0 = 0, rb1 = 0;
     if (n_groups > 0) {
         const cptr<int> tstart = C(sh->inj_tstart);
@@ -184,24 +189,29 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         rb1 = rstart[tile + 1];
     }
     const bool inj_one_pass = HOIST && ig1 - ig0 <= NT, rec_one_pass = HOIST && rb1 - rb0 <= NT;
-    // this thread's group: LDS index, and its data row (one entry) or -1 - g
-    // (several entries, summed per step in entry order); forward coefficients
-    int g_e = -1, g_row = -1;
-    float g_cu = 0.0f, g_cw = 0.0f, g_cv = 0.0f;
-    if (inj_one_pass && ig0 + t < ig1) {
-        const int g = C(sh->inj_titems)[ig0 + t];
-        const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
-        const int k0 = C(sh->order)[q0];
-        const int ci = C(sh->loc)[2 * k0], cj = C(sh->loc)[2 * k0 + 1];
-        g_e = (ci - ri0) * RZ + (cj - rj0);
-        g_row = q1 - q0 == 1 ? k0 : -1 - g;
-        const int c = ci * nz + cj;
-        if (FWD) {
-            if (PSV) {
-                g_cu = G(m.iru)[c];
-                g_cw = G(m.irw)[c];
+    // per slot: the data row of its cell's single-entry group, -1 - g for a
+    // group g of several entries (summed per step in entry order), or kNone
+    constexpr int kNone = -2147483647 - 1;
+    int gk[HOIST ? NC : 1];
+#pragma unroll
+    for (int k = 0; k < (HOIST ? NC : 1); k++) gk[k] = kNone;
+    if constexpr (HOIST) {
+        if (inj_one_pass && ig1 > ig0) {  // uniform
+#pragma unroll
+            for (int k = 0; k < NC; k++)
+                if (t + NT * k < RN) gmap[t + NT * k] = kNone;
+            __syncthreads();
+            if (ig0 + t < ig1) {
+                const int g = C(sh->inj_titems)[ig0 + t];
+                const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
+                const int k0 = C(sh->order)[q0];
+                const int ci = C(sh->loc)[2 * k0], cj = C(sh->loc)[2 * k0 + 1];
+                gmap[(ci - ri0) * RZ + (cj - rj0)] = q1 - q0 == 1 ? k0 : -1 - g;
             }
-            if (SH) g_cv = G(m.ir)[c];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < NC; k++)
+                if (t + NT * k < RN) gk[k] = gmap[t + NT * k];
         }
     }
     int r_l = -1, r_e = 0;  // this thread's receiver
@@ -211,6 +221,39 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         const int ci = C(sh->rec_loc)[2 * r_l], cj = C(sh->rec_loc)[2 * r_l + 1];
         r_e = (ci - ri0) * RZ + (cj - rj0);
         r_A = sponge_at(ci, cj, nx, nz, m.fw, m.tab);
+    }
+
+    // Imaging slot of step it_ (-1: none).  The small shapes with one wave
+    // system load the forward wavefield a whole step before the image update
+    // uses it (the others a phase before): pf0 / pf1 hold the next step's u / w
+    // (or v) of every slot.
+    auto image_slot = [&](int it_) {
+        const int tb_ = nt - 1 - it_;
+        if (!(imaging && tb_ > 0 && tb_ % isnap == 0)) return -1;
+        const int sl = tb_ / isnap - 1 - snap_base;
+        return (sl >= 0 && sl < num_snaps && C(sh->isnaps)[sl] == tb_) ? sl : -1;
+    };
+    constexpr bool PIPE = HOIST && (PSV != SH);
+    constexpr int NF = PIPE ? NC : 1;
+    float pf0[NF], pf1[NF];
+    int img_next = -1;
+    auto load_forward = [&](int sl) {
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+            if (PSV) {
+                pf0[k] = snap_u[(size_t)sl * sstride + gc[k]];
+                pf1[k] = snap_w[(size_t)sl * sstride + gc[k]];
+            } else {
+                pf0[k] = snap_v[(size_t)sl * sstride + gc[k]];
+            }
+        }
+    };
+    if constexpr (PIPE) {
+        const int first = it0 + it_off > it_lo ? it0 + it_off : it_lo;
+        if (first < it_end && first < it0 + it_off + K) {
+            img_next = image_slot(first);
+            if (img_next >= 0) load_forward(img_next);
+        }
     }
 
     bool started = false;  // a step ran in this launch: LDS velocities await the sponge
@@ -226,27 +269,90 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
             const int sl = it / isnap - 1 - snap_base;
             if (sl >= 0 && sl < num_snaps) snap = sl;
         }
-        if (imaging && tb > 0 && tb % isnap == 0) {
+        if constexpr (PIPE) {
+            img = img_next;
+        } else if (imaging && tb > 0 && tb % isnap == 0) {
             const int sl = tb / isnap - 1 - snap_base;
             if (sl >= 0 && sl < num_snaps && C(sh->isnaps)[sl] == tb) img = sl;
         }
         if (cint > 0 && it % cint == 0) chk = it / cint;
-        // this step's injected value of a single-entry group, loaded now and
-        // added after the velocity update
-        float gvu = 0.0f, gvv = 0.0f, gvw = 0.0f;
-        if (g_row >= 0) {
-            const size_t o = (size_t)g_row * (size_t)nt + (FWD ? it : tb);
-            if (PSV) {
-                gvu = C(sh->su)[o];
-                gvw = C(sh->sw)[o];
+        // this step's injected values of the slots' single-entry groups,
+        // loaded now and added after the velocity update
+        constexpr int NG = HOIST ? NC : 1;
+        float gvu[NG], gvv[NG], gvw[NG];
+        const int col = FWD ? it : tb;
+#pragma unroll
+        for (int k = 0; k < NG; k++) {
+            gvu[k] = gvv[k] = gvw[k] = 0.0f;
+            if (HOIST && gk[k] >= 0) {
+                const size_t o = (size_t)gk[k] * (size_t)nt + col;
+                if (PSV) {
+                    gvu[k] = C(sh->su)[o];
+                    gvw[k] = C(sh->sw)[o];
+                }
+                if (SH) gvv[k] = C(sh->sv)[o];
             }
-            if (SH) gvv = C(sh->sv)[o];
         }
+        // the injection of slot k into its velocity (after the update, in
+        // entry order, the expressions of the injection phase below)
+        auto inject_u = [&](int k, float& u, float& w) {
+            if (!HOIST || gk[k] == kNone) return;
+            if (gk[k] >= 0) {
+                if (FWD) {
+                    u = u + (gvu[k] * m.factor) * c0[k];
+                    w = w + (gvw[k] * m.factor) * c1[k];
+                } else {
+                    u = u + gvu[k];
+                    w = w + gvw[k];
+                }
+                return;
+            }
+            const int g = -1 - gk[k];
+            const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
+            for (int q2 = q0; q2 < q1; q2++) {
+                const size_t o = (size_t)C(sh->order)[q2] * (size_t)nt + col;
+                if (FWD) {
+                    u = u + (C(sh->su)[o] * m.factor) * c0[k];
+                    w = w + (C(sh->sw)[o] * m.factor) * c1[k];
+                } else {
+                    u = u + C(sh->su)[o];
+                    w = w + C(sh->sw)[o];
+                }
+            }
+        };
+        auto inject_v = [&](int k, float& v) {
+            if (!HOIST || gk[k] == kNone) return;
+            if (gk[k] >= 0) {
+                v = FWD ? v + (gvv[k] * m.factor) * d0[k] : v + gvv[k];
+                return;
+            }
+            const int g = -1 - gk[k];
+            const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
+            for (int q2 = q0; q2 < q1; q2++) {
+                const size_t o = (size_t)C(sh->order)[q2] * (size_t)nt + col;
+                v = FWD ? v + (C(sh->sv)[o] * m.factor) * d0[k] : v + C(sh->sv)[o];
+            }
+        };
         // SH imaging: the forward wavefield of every slot is loaded here, a
         // whole phase before the image update uses it (cells outside the grid
         // read cell 0; only the tile's cells use the value)
         float fimg[SH && !PSV ? NC : 1], fimu[PSV && !SH ? NC : 1], fimw[PSV && !SH ? NC : 1];
-        if constexpr (SH && !PSV) {
+        if constexpr (PIPE) {
+            if (img >= 0) {
+#pragma unroll
+                for (int k = 0; k < NF; k++) {
+                    if constexpr (PSV) {
+                        fimu[k] = ssponge ? pf0[k] * A[k] : pf0[k];
+                        fimw[k] = ssponge ? pf1[k] * A[k] : pf1[k];
+                    } else {
+                        fimg[k] = ssponge ? pf0[k] * A[k] : pf0[k];
+                    }
+                }
+            }
+            // the next step's forward wavefield, in flight during this step
+            img_next = (s + 1 < K && it + 1 < it_end) ? image_slot(it + 1) : -1;
+            if (img_next >= 0) load_forward(img_next);
+        } else if constexpr (SH && !PSV) {
             if (img >= 0) {
 #pragma unroll
                 for (int k = 0; k < NC; k++) {
@@ -255,7 +361,7 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                 }
             }
         }
-        if constexpr (PSV && !SH) {
+        if constexpr (PSV && !SH && !PIPE) {
             if (img >= 0) {
 #pragma unroll
                 for (int k = 0; k < NC; k++) {
@@ -300,6 +406,7 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                     }
                     v = upd ? vn : v;
                 }
+                inject_v(k, v);
                 sV[e] = v;
             }
         };
@@ -334,6 +441,7 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                             w = w + ((-(sxz_x + szz_z)) * dt) * c1[k];
                         }
                     }
+                    inject_u(k, u, w);
                     sU[e] = u;
                     sW[e] = w;
                 }
@@ -352,6 +460,7 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                             v = v + ((-(syx_x + syz_z)) * dt) * d0[k];
                         }
                     }
+                    inject_v(k, v);
                     sV[e] = v;
                 }
             }
@@ -362,58 +471,12 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
         }
         __syncthreads();
 
-        // ---- injection into the region (one thread per cell, entries in order)
-        if (ig1 > ig0) {
+        // ---- injection into the region (one thread per cell, entries in order;
+        // the small shapes inject in the velocity update above)
+        if (ig1 > ig0 && !inj_one_pass) {
             const gptr<const float> su = G(sh->su), sv = G(sh->sv), sw = G(sh->sw);
-            const int col = FWD ? it : tb;
             const size_t ntt = (size_t)nt;
-            if (inj_one_pass) {
-                if (g_e >= 0) {
-                    const int e = g_e;
-                    const cptr<int> order = C(sh->order);
-                    if (PSV) {
-                        float x = sU[e], zz = sW[e];
-                        if (g_row >= 0) {
-                            if (FWD) {
-                                x = x + (gvu * m.factor) * g_cu;
-                                zz = zz + (gvw * m.factor) * g_cw;
-                            } else {
-                                x = x + gvu;
-                                zz = zz + gvw;
-                            }
-                        } else {
-                            const int g = -1 - g_row;
-                            const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
-                            for (int q2 = q0; q2 < q1; q2++) {
-                                const size_t o = (size_t)order[q2] * ntt + col;
-                                if (FWD) {
-                                    x = x + (su[o] * m.factor) * g_cu;
-                                    zz = zz + (sw[o] * m.factor) * g_cw;
-                                } else {
-                                    x = x + su[o];
-                                    zz = zz + sw[o];
-                                }
-                            }
-                        }
-                        sU[e] = x;
-                        sW[e] = zz;
-                    }
-                    if (SH) {
-                        float y = sV[e];
-                        if (g_row >= 0) {
-                            y = FWD ? y + (gvv * m.factor) * g_cv : y + gvv;
-                        } else {
-                            const int g = -1 - g_row;
-                            const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
-                            for (int q2 = q0; q2 < q1; q2++) {
-                                const size_t o = (size_t)order[q2] * ntt + col;
-                                y = FWD ? y + (sv[o] * m.factor) * g_cv : y + sv[o];
-                            }
-                        }
-                        sV[e] = y;
-                    }
-                }
-            } else {
+            {
                 const gptr<const int> titems = G(sh->inj_titems), gstart = G(sh->start);
                 const gptr<const int> order = G(sh->order), loc = G(sh->loc);
                 for (int q = ig0 + t; q < ig1; q += NT) {
@@ -652,14 +715,17 @@ static const BlockCfg kCfgSH[] = {{16, 64, 256, 4},   {16, 64, 256, 8},    {16, 
                                   {16, 128, 512, 8},  {32, 128, 1024, 4},  {32, 128, 1024, 8},
                                   {24, 128, 1024, 8}, {32, 96, 1024, 8},   {48, 64, 1024, 8},
                                   {32, 48, 512, 8},   {24, 64, 512, 8},    {16, 32, 1024, 4},
-                                  {16, 32, 1024, 8}};
+                                  {16, 32, 1024, 8},  {16, 16, 1024, 8}};
 static const BlockCfg kCfgPSV[] = {{16, 64, 256, 2},  {16, 64, 256, 4},    {16, 128, 512, 4},
                                    {16, 128, 512, 8}, {32, 128, 1024, 4},  {48, 64, 1024, 4},
-                                   {32, 64, 512, 4},  {16, 32, 1024, 4},   {16, 32, 1024, 8}};
+                                   {32, 64, 512, 4},  {16, 32, 1024, 4},   {16, 32, 1024, 8},
+                                   {16, 16, 1024, 8}};
 static const BlockCfg kCfgBoth[] = {{16, 64, 256, 2}, {16, 64, 256, 4}, {16, 128, 512, 4},
                                     {16, 32, 1024, 4}};
-// (the 16 x 32 tiles with 1024 threads give every thread one or two region
-// cells: the shortest step on grids too small to fill the GPU)
+// (the 16 x 32 and 16 x 16 tiles with 1024 threads give every thread one or
+// two region cells: the shortest step on grids too small to fill the GPU; 16
+// x 16 with 576 threads and K = 4, and 8 x 16 with 384 threads and K = 4, were
+// measured slower on the 200 x 200 grid)
 
 static void cfg_list(int systems, const BlockCfg** list, int* n) {
     if (systems == TRTM_SH) {
@@ -1046,12 +1112,13 @@ static bool launch_sys(BlockCfg c, const KArgs& m, const ShotDev* shots, int n_s
         TRTM_CFG(16, 64, 256, 2) TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 128, 512, 4)
         TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4) TRTM_CFG(48, 64, 1024, 4)
         TRTM_CFG(32, 64, 512, 4) TRTM_CFG(16, 32, 1024, 4) TRTM_CFG(16, 32, 1024, 8)
+        TRTM_CFG(16, 16, 1024, 8)
     } else {
         TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 64, 256, 8) TRTM_CFG(16, 128, 512, 4)
         TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4) TRTM_CFG(32, 128, 1024, 8)
         TRTM_CFG(24, 128, 1024, 8) TRTM_CFG(32, 96, 1024, 8) TRTM_CFG(48, 64, 1024, 8)
         TRTM_CFG(32, 48, 512, 8) TRTM_CFG(24, 64, 512, 8) TRTM_CFG(16, 32, 1024, 4)
-        TRTM_CFG(16, 32, 1024, 8)
+        TRTM_CFG(16, 32, 1024, 8) TRTM_CFG(16, 16, 1024, 8)
     }
 #undef TRTM_CFG
     return false;
