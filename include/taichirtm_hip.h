@@ -337,10 +337,11 @@ typedef struct trtm_fwi_desc {
 int64_t trtm_fwi_workspace_bytes(const trtm_fwi_desc *d);
 
 /*
- * Compiled shapes of the blocked FWI kernels: writes up to max_n rows of
- * (tile_x, tile_z, threads, block_steps) to out, returns the number compiled.
+ * Compiled shapes of the blocked FWI kernels of one pass (0 forward, 1
+ * adjoint, 2 scattered forward) that fit these systems: writes up to max_n rows
+ * of (tile_x, tile_z, threads, block_steps) to out, returns the number.
  */
-int trtm_fwi_block_configs(int32_t systems, int32_t *out, int32_t max_n);
+int trtm_fwi_block_configs(int32_t systems, int32_t pass, int32_t *out, int32_t max_n);
 
 /*
  * Fills cell_map from the source and receiver plans (src_*, rec_*): the

@@ -114,7 +114,7 @@ SIGNATURES = {
     "trtm_prepare_medium_f64": (ctypes.c_int, [_i32, _i32] + [_p] * 10 + [_p]),
     "trtm_fwi_prepare": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _p]),
     "trtm_fwi_workspace_bytes": (ctypes.c_int64, [ctypes.POINTER(FwiDesc)]),
-    "trtm_fwi_block_configs": (ctypes.c_int, [_i32, _p, _i32]),
+    "trtm_fwi_block_configs": (ctypes.c_int, [_i32, _i32, _p, _i32]),
     "trtm_fwi_forward": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _p]),
     "trtm_fwi_forward_batch": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _i32, _p]),
     "trtm_fwi_adjoint_batch": (ctypes.c_int, [ctypes.POINTER(FwiDesc), _i32, _i32, _i32, _p]),

@@ -37,10 +37,14 @@ def candidates(systems: int) -> List[Shape]:
     return out
 
 
-def fwi_candidates(systems: int) -> List[Shape]:
-    """Every compiled shape of the blocked FWI kernels that fits these systems."""
+FWI_PASSES = {"forward": 0, "history": 0, "adjoint": 1, "scattered": 2}
+
+
+def fwi_candidates(systems: int, pass_name: str = "adjoint") -> List[Shape]:
+    """Every compiled shape of the blocked FWI kernels of a pass ("forward",
+    "history", "adjoint" or "scattered") that fits these systems."""
     buf = (ctypes.c_int32 * (4 * 64))()
-    n = _lib.call("trtm_fwi_block_configs", systems, buf, 64)
+    n = _lib.call("trtm_fwi_block_configs", systems, FWI_PASSES[pass_name], buf, 64)
     return [("fwi", int(buf[4 * i]), int(buf[4 * i + 1]), int(buf[4 * i + 3]))
             for i in range(min(n, 64))]
 

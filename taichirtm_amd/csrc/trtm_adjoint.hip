@@ -841,9 +841,9 @@ int64_t trtm_fwi_workspace_bytes(const trtm_fwi_desc* d) {
     return (int64_t)8 * d->nx * d->nz * d->real_bytes;
 }
 
-int trtm_fwi_block_configs(int32_t systems, int32_t* out, int32_t max_n) {
-    if (!out || max_n < 0) return fail(TRTM_ERR_ARG, "bad arguments");
-    return fwi_block_configs(systems, out, max_n);
+int trtm_fwi_block_configs(int32_t systems, int32_t pass, int32_t* out, int32_t max_n) {
+    if (!out || max_n < 0 || pass < 0 || pass > 2) return fail(TRTM_ERR_ARG, "bad arguments");
+    return fwi_block_configs(systems, pass, out, max_n);
 }
 
 int trtm_fwi_chain_gradient(int32_t nx, int32_t nz, const double* mu, const double* rho,

@@ -22,7 +22,7 @@ struct ByArray {
 // float only; descriptors already checked by the caller
 int fwi_block_forward(const trtm_fwi_desc* d, int n, int it_begin, int it_end, hipStream_t s);
 int fwi_block_adjoint(const trtm_fwi_desc* d, int n, int it_begin, int it_end, hipStream_t s);
-int fwi_block_configs(int systems, int32_t* out, int max_n);
+int fwi_block_configs(int systems, int pass, int32_t* out, int max_n);
 int fwi_scat_forward(const trtm_fwi_desc* d, int n, int it_begin, int it_end, hipStream_t s);
 
 }  // namespace trtm

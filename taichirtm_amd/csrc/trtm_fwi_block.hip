@@ -1042,6 +1042,16 @@ constexpr bool fwi_fits() {
     return fwd <= kLdsLimit && adj_lds_bytes<PSV, SH, TX, TZ, K>() <= kLdsLimit;
 }
 
+template <bool PSV, bool SH, int TX, int TZ, int K>
+constexpr bool fwd_fits() {
+    return 4 * (size_t)(TX + 2 * K) * (TZ + 2 * K) * ((PSV ? 5 : 0) + (SH ? 3 : 0)) <= kLdsLimit;
+}
+
+static bool fwd_cfg_fits(int systems, const BlockCfg& c) {
+    const int nf = ((systems & TRTM_PSV) ? 5 : 0) + ((systems & TRTM_SH) ? 3 : 0);
+    return 4 * (size_t)(c.tx + 2 * c.k) * (c.tz + 2 * c.k) * nf <= kLdsLimit;
+}
+
 static bool fwi_cfg_fits(int systems, const BlockCfg& c) {
     const int nf = ((systems & TRTM_PSV) ? 5 : 0) + ((systems & TRTM_SH) ? 3 : 0);
     const size_t rn = (size_t)(c.tx + 2 * c.k) * (c.tz + 2 * c.k);
@@ -1059,12 +1069,18 @@ static const BlockCfg kFwiCfg[] = {{24, 64, 768, 4},  {32, 64, 1024, 4}, {16, 64
 // forward passes {24, 64, 768, 4}, the adjoint {32, 64, 1024, 4}; the first
 // listed shape that fits otherwise
 static const BlockCfg kFwiDefaultFwd = {24, 64, 768, 4}, kFwiDefaultAdj = {32, 64, 1024, 4};
+// forward-only shapes (their adjoint would not fit the LDS): 8 steps per
+// launch halve the state and coefficient traffic per step of the
+// checkpointing and history passes
+static const BlockCfg kFwiFwdOnly[] = {{32, 64, 1024, 8}, {24, 64, 768, 8}, {48, 64, 1024, 8}};
 
-// shapes whose forward and adjoint LDS fit the workgroup limit for these systems
-int fwi_block_configs(int systems, int32_t* out, int max_n) {
+static bool scat_cfg_fits(int systems, const BlockCfg& c);
+
+// shapes of a pass (0 forward, 1 adjoint, 2 scattered forward) whose LDS fits
+// the workgroup limit for these systems
+int fwi_block_configs(int systems, int pass, int32_t* out, int max_n) {
     int n = 0;
-    for (const BlockCfg& c : kFwiCfg) {
-        if (!fwi_cfg_fits(systems, c)) continue;
+    auto add = [&](const BlockCfg& c) {
         if (n < max_n) {
             out[4 * n] = c.tx;
             out[4 * n + 1] = c.tz;
@@ -1072,7 +1088,12 @@ int fwi_block_configs(int systems, int32_t* out, int max_n) {
             out[4 * n + 3] = c.k;
         }
         n++;
-    }
+    };
+    for (const BlockCfg& c : kFwiCfg)
+        if (fwi_cfg_fits(systems, c) && (pass != 2 || scat_cfg_fits(systems, c))) add(c);
+    if (pass != 1)
+        for (const BlockCfg& c : kFwiFwdOnly)
+            if (pass == 0 ? fwd_cfg_fits(systems, c) : scat_cfg_fits(systems, c)) add(c);
     return n;
 }
 
@@ -1084,15 +1105,24 @@ static int fwi_choose(const trtm_fwi_desc* d, bool adjoint, BlockCfg* out) {
         *out = def;
         return TRTM_OK;
     }
+    auto match = [&](const BlockCfg& c) {
+        return (d->tile_x <= 0 || c.tx == d->tile_x) && (d->tile_z <= 0 || c.tz == d->tile_z) &&
+               (d->block_steps <= 0 || c.k == d->block_steps);
+    };
     for (const BlockCfg& c : kFwiCfg)
-        if (fwi_cfg_fits(d->systems, c) && (d->tile_x <= 0 || c.tx == d->tile_x) &&
-            (d->tile_z <= 0 || c.tz == d->tile_z) &&
-            (d->block_steps <= 0 || c.k == d->block_steps)) {
+        if (fwi_cfg_fits(d->systems, c) && match(c)) {
             *out = c;
             return TRTM_OK;
         }
-    return fail(TRTM_ERR_UNSUPPORTED, "blocked FWI kernel has no tile_x=%d tile_z=%d block_steps=%d build",
-                d->tile_x, d->tile_z, d->block_steps);
+    if (!adjoint)
+        for (const BlockCfg& c : kFwiFwdOnly)
+            if (fwd_cfg_fits(d->systems, c) && match(c)) {
+                *out = c;
+                return TRTM_OK;
+            }
+    return fail(TRTM_ERR_UNSUPPORTED, "blocked FWI %s kernel has no tile_x=%d tile_z=%d "
+                "block_steps=%d build", adjoint ? "adjoint" : "forward", d->tile_x, d->tile_z,
+                d->block_steps);
 }
 
 static FB make_fb(const trtm_fwi_desc* d, bool adjoint) {
@@ -1166,6 +1196,25 @@ static int launch_cfg(const BlockCfg& c, bool adjoint, bool hist, const S& src, 
     TRTM_FB(16, 48, 512, 4)
     TRTM_FB(24, 32, 512, 4)
 #undef TRTM_FB
+    // forward-only shapes
+#define TRTM_FF(TX, TZ, NT, K)                                                                  \
+    if constexpr (fwd_fits<PSV, SH, TX, TZ, K>()) {                                             \
+        if (!adjoint && c.tx == TX && c.tz == TZ && c.nt == NT && c.k == K) {                   \
+            const int tiles = tx_ * tz_;                                                        \
+            const dim3 grid(8 * ((tiles + 7) / 8), 1, shots);                                   \
+            if (hist)                                                                           \
+                hipLaunchKernelGGL((k_fwi_fwd_blk<PSV, SH, TX, TZ, NT, K, true, S>), grid,      \
+                                   dim3(NT), 0, s, src, t_a, t_b, tx_, tz_);                    \
+            else                                                                                \
+                hipLaunchKernelGGL((k_fwi_fwd_blk<PSV, SH, TX, TZ, NT, K, false, S>), grid,     \
+                                   dim3(NT), 0, s, src, t_a, t_b, tx_, tz_);                    \
+            return TRTM_OK;                                                                     \
+        }                                                                                       \
+    }
+    TRTM_FF(32, 64, 1024, 8)
+    TRTM_FF(24, 64, 768, 8)
+    TRTM_FF(48, 64, 1024, 8)
+#undef TRTM_FF
     return fail(TRTM_ERR_UNSUPPORTED, "blocked FWI shape %dx%d/%d/%d not compiled", c.tx, c.tz,
                 c.nt, c.k);
 }
@@ -1274,13 +1323,19 @@ static int scat_choose(const trtm_fwi_desc* d, BlockCfg* out) {
         *out = def;
         return TRTM_OK;
     }
-    for (const BlockCfg& c : kFwiCfg)
-        if (scat_cfg_fits(d->systems, c) && (d->tile_x <= 0 || c.tx == d->tile_x) &&
-            (d->tile_z <= 0 || c.tz == d->tile_z) &&
-            (d->block_steps <= 0 || c.k == d->block_steps)) {
-            *out = c;
-            return TRTM_OK;
+    for (const BlockCfg* list : {&kFwiCfg[0], &kFwiFwdOnly[0]}) {
+        const int cnt = list == kFwiCfg ? (int)(sizeof(kFwiCfg) / sizeof(BlockCfg))
+                                        : (int)(sizeof(kFwiFwdOnly) / sizeof(BlockCfg));
+        for (int q = 0; q < cnt; q++) {
+            const BlockCfg& c = list[q];
+            if (scat_cfg_fits(d->systems, c) && (d->tile_x <= 0 || c.tx == d->tile_x) &&
+                (d->tile_z <= 0 || c.tz == d->tile_z) &&
+                (d->block_steps <= 0 || c.k == d->block_steps)) {
+                *out = c;
+                return TRTM_OK;
+            }
         }
+    }
     return fail(TRTM_ERR_UNSUPPORTED, "scattered FWI kernel has no tile_x=%d tile_z=%d "
                 "block_steps=%d build for these wave systems", d->tile_x, d->tile_z,
                 d->block_steps);
@@ -1308,6 +1363,9 @@ static int launch_scat_cfg(const BlockCfg& c, const ByArray4<FS>& src, int shots
     TRTM_FS(16, 64, 256, 2)
     TRTM_FS(16, 48, 512, 4)
     TRTM_FS(24, 32, 512, 4)
+    TRTM_FS(32, 64, 1024, 8)
+    TRTM_FS(24, 64, 768, 8)
+    TRTM_FS(48, 64, 1024, 8)
 #undef TRTM_FS
     return fail(TRTM_ERR_UNSUPPORTED, "scattered FWI shape %dx%d/%d/%d not compiled", c.tx, c.tz,
                 c.nt, c.k);

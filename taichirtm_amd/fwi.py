@@ -260,12 +260,8 @@ class ElasticFWI:
             self._call_batch(fn, descs, a, b, shape[1:])
             reset()
 
-        shapes = autotune.fwi_candidates(self.systems)
-        if name == "scattered":  # twice the forward's LDS: the scattered state beside the state
-            nf = (10 if self.systems & _lib.PSV else 0) + (6 if self.systems & _lib.SH else 0)
-            shapes = [c for c in shapes if 4 * (c[1] + 2 * c[3]) * (c[2] + 2 * c[3]) * nf
-                      <= 160 * 1024]
-        shape = autotune.tune(key, self.systems, trial, shapes=shapes)
+        shape = autotune.tune(key, self.systems, trial,
+                              shapes=autotune.fwi_candidates(self.systems, name))
         self.tuned[name] = shape[1:]
         self.shapes[name] = shape[1:]
         reset()

@@ -236,7 +236,7 @@ def test_scattered_residual_same_bits_for_every_shape():
     r32 = [_t(x, torch.float32) for x in mt]
     ref = None
     for tx, tz, k in ((24, 64, 4), (32, 64, 4), (16, 64, 8), (16, 64, 2), (16, 48, 4),
-                      (24, 32, 4)):
+                      (24, 32, 4), (32, 64, 8), (24, 64, 8)):
         p = ElasticFWI(*grid, shots, precision="float32", checkpoint_every=30, history_bytes=0,
                        kernel="blocked", tile_x=tx, tile_z=tz, block_steps=k)
         res = torch.stack(p.residual(*m32, r32))
@@ -383,6 +383,38 @@ def _records_and_gradient(grid, shots, m0, mt, **kw):
     return recs, g, {k: v.clone() for k, v in p.coefficient_gradient.items()}
 
 
+FORWARD_ONLY_SHAPES = [(32, 64, 8), (24, 64, 8), (48, 64, 8)]
+
+
+@pytest.mark.parametrize("shape", FORWARD_ONLY_SHAPES, ids=lambda s: "%dx%dk%d" % s)
+@pytest.mark.parametrize("system", ["psv", "sh", "both"])
+def test_forward_only_shapes_bitwise_equal_step_kernels(system, shape):
+    """Shapes listed for the forward passes only (their adjoint would exceed
+    the LDS): checkpointing and history forward with them, the adjoint with
+    the default shape; records and gradients equal the per-step kernels' bit
+    for bit."""
+    _case("FWI forward-only shape %s %s" % (system, shape))
+    from taichirtm_amd import _lib, autotune
+    systems = {"psv": _lib.PSV, "sh": _lib.SH, "both": _lib.PSV | _lib.SH}[system]
+    if ("fwi",) + tuple(shape) not in autotune.fwi_candidates(systems, "forward"):
+        pytest.skip("shape exceeds the LDS of a workgroup for these wave systems")
+    assert ("fwi",) + tuple(shape) not in autotune.fwi_candidates(systems, "adjoint")
+    grid, shots, m0, mt = problem(system, nt=130, n_shots=3)
+    shots = _uneven(shots, (130, 97, 121), ((0, 52), (5, 30), (20, 45)))
+    shots[2] = ShotSpec(np.array([[1, 1]], np.int32), shots[2].wavelets, shots[2].rec_loc)
+    ref = _records_and_gradient(grid, shots, m0, mt, kernel="step", checkpoint_every=29)
+    p = ElasticFWI(*grid, shots, precision="float32", kernel="blocked", checkpoint_every=29,
+                   history_bytes=0)
+    p.shapes["forward"] = p.shapes["history"] = shape
+    obs = [o.clone() for o in p.forward(*[_t(x, torch.float32) for x in mt])]
+    recs = [r.clone() for r in p.forward(*[_t(x, torch.float32) for x in m0])]
+    g = p.gradient([r - o for r, o in zip(recs, obs)])
+    for a, b in zip(recs, ref[0]):
+        assert torch.equal(a, b)
+    for a, b in zip(g, ref[1]):
+        assert torch.equal(a, b)
+
+
 @pytest.mark.parametrize("shape", BLOCK_SHAPES, ids=lambda s: "%dx%dk%d" % s)
 @pytest.mark.parametrize("system", ["psv", "sh", "both"])
 def test_blocked_kernels_bitwise_equal_step_kernels(system, shape):
@@ -393,7 +425,7 @@ def test_blocked_kernels_bitwise_equal_step_kernels(system, shape):
     _case("FWI blocked %s %s" % (system, shape))
     if system == "both":
         from taichirtm_amd import _lib, autotune
-        if ("fwi",) + tuple(shape) not in autotune.fwi_candidates(_lib.PSV | _lib.SH):
+        if ("fwi",) + tuple(shape) not in autotune.fwi_candidates(_lib.PSV | _lib.SH, "adjoint"):
             pytest.skip("shape exceeds the LDS of a workgroup for both wave systems")
     grid, shots, m0, mt = problem(system, nt=130, n_shots=3)
     shots = _uneven(shots, (130, 97, 121), ((0, 52), (5, 30), (20, 45)))
