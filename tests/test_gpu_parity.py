@@ -261,6 +261,23 @@ def test_psv_anomaly_full_size_parity():
     assert e_seis < 1e-4 and e_img < 1e-3
 
 
+def test_psv_anomaly_full_size_bench_kernels_bitwise():
+    """The 200x200x1500 P-SV RTM with the blocked shape the autotuner picks for
+    it on MI355X in both directions (16 x 16 tiles, 1024 threads, 8 steps per
+    launch; sources and receivers looked up once per launch) against the
+    oracle bit for bit."""
+    _case("psv_anomaly 200x200x1500 blocked 16x16x8 vs oracle")
+    cfg = configs.psv_anomaly()
+    kw = cfg.kwargs(cfg.shots[0])
+    wav = cfg.wavelets(cfg.shots[0])
+    opts = dict(tile_x=16, tile_z=16, block_steps=8)
+    h = run_hip(kw, wav, isnap=1, kernel="blocked", fw_opts=opts, bw_opts=opts)
+    assert h["fw"].block_config() == (16, 16, 1024, 8)
+    assert h["bw"].block_config() == (16, 16, 1024, 8)
+    o = run_oracle(kw, wav, isnap=1)
+    assert_bitwise(h, o)
+
+
 def test_sh_layered_reduced_parity_and_float64_error():
     """SH 1200x600: bitwise against the oracle over 300 steps, and the float32
     scheme's error against the oracle in float64 there."""
