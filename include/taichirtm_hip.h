@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define TRTM_ABI_VERSION 6
+#define TRTM_ABI_VERSION 7
 
 #define TRTM_OK 0
 #define TRTM_ERR_ARG (-1)         /* invalid argument or shape */
@@ -299,11 +299,18 @@ typedef struct trtm_fwi_desc {
     int32_t n_rec_groups, hist_base;
     void *rec_u, *rec_v, *rec_w;       /* forward records, REAL time-major (nt, n_rec) */
     const void *res_u, *res_v, *res_w; /* data adjoint dJ/d(record), REAL (nt, n_rec) */
-    /* history of steps [hist_base, hist_base + hist_len): for each step the
-     * pre-step stresses and the post-update velocities, REAL
-     * (hist_len, n_hist_fields, nx, nz), n_hist_fields = 5 (P-SV) + 3 (SH) for the
-     * active systems in the order sxx, szz, sxz, u, w, syx, syz, v */
-    int32_t hist_len, reserved;
+    /* history of steps [hist_base, hist_base + hist_len), in one of two layouts.
+     * hist_every = 0 (full): for each step the pre-step stresses and the
+     * post-update velocities, REAL (hist_len, n_hist_fields, nx, nz),
+     * n_hist_fields = 5 (P-SV) + 3 (SH) for the active systems in the order
+     * sxx, szz, sxz, u, w, syx, syz, v; read by the one-step-per-launch adjoint.
+     * hist_every = K > 0 (compact, blocked kernels only): the post-update
+     * velocities of every step, REAL (hist_len, n_vel, nx, nz), n_vel = 2 (P-SV:
+     * u, w) + 1 (SH: v), followed by the pre-step stresses of the steps
+     * hist_base + m K, REAL (ceil(hist_len / K), n_str, nx, nz), n_str = 3 (P-SV:
+     * sxx, szz, sxz) + 2 (SH: syx, syz); read by the blocked adjoint, whose
+     * block_steps must equal K (it recomputes the other steps' stresses). */
+    int32_t hist_len, hist_every;
     void *hist;
     /* gradient accumulators in coefficient space, REAL (nx, nz), += */
     void *g_lam, *g_mu, *g_mxz, *g_myx, *g_myz, *g_ir, *g_iru, *g_irw;

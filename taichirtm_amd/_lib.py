@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtaichirtm_hip.so")
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 OK = 0
 ERR_ARG, ERR_HIP, ERR_UNSUPPORTED = -1, -2, -3
 PSV, SH = 1, 2
@@ -75,7 +75,7 @@ class FwiDesc(ctypes.Structure):
         ("n_rec_groups", _i32), ("hist_base", _i32),
         ("rec_u", _p), ("rec_v", _p), ("rec_w", _p),
         ("res_u", _p), ("res_v", _p), ("res_w", _p),
-        ("hist_len", _i32), ("reserved", _i32),
+        ("hist_len", _i32), ("hist_every", _i32),
         ("hist", _p),
         ("g_lam", _p), ("g_mu", _p), ("g_mxz", _p), ("g_myx", _p), ("g_myz", _p), ("g_ir", _p),
         ("g_iru", _p), ("g_irw", _p),

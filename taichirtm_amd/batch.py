@@ -433,20 +433,25 @@ class BatchRTM:
 
     # -- backward fed with forward wavefields from elsewhere ------------------
 
-    def external_begin(self, fwd_systems, L):
+    def external_begin(self, fwd_systems, L, layout=None):
         """Prepare the backward passes for ``external_segment``: the forward
         wavefield comes segment by segment from an FWI history (the FWI
         forward is the reference's forward step, forward_modeling.py:539-551,
         so A times its post-injection velocity is the forward state after each
-        step).  No forward propagators, checkpoints or snapshot buffers are
-        held here; imaging every step (isnap 1), blocked kernel."""
+        step).  ``layout`` = (stride, {component: offset}) of those velocities
+        in a history buffer (``ElasticFWI.history_velocity_layout``; default:
+        the full layout, 5 P-SV + 3 SH fields per step).  No forward
+        propagators, checkpoints or snapshot buffers are held here; imaging
+        every step (isnap 1), blocked kernel."""
         if self.isnap != 1:
             raise ValueError("external forward wavefields need isnap = 1")
         if any(s.observed is None for s in self.shots):
             raise ValueError("external forward wavefields need every shot's observed data")
         self.block["kernel"] = "blocked" if self.block["kernel"] == "wave" else self.block["kernel"]
-        self._ext_fields = {"u": 3, "w": 4, "v": 7 if fwd_systems & _lib.PSV else 2}
-        self._ext_nhist = (5 if fwd_systems & _lib.PSV else 0) + (3 if fwd_systems & _lib.SH else 0)
+        if layout is None:
+            layout = ((5 if fwd_systems & _lib.PSV else 0) + (3 if fwd_systems & _lib.SH else 0),
+                      {"u": 3, "w": 4, "v": 7 if fwd_systems & _lib.PSV else 2})
+        self._ext_nhist, self._ext_fields = int(layout[0]), dict(layout[1])
         self.L = int(L)
         dev = device()
         self.bw, self.flags_b, descs, keep = [], [], [], []
@@ -480,9 +485,9 @@ class BatchRTM:
 
     def external_segment(self, a, b, hists):
         """Backward steps of forward times b-1 down to a for every shot, the
-        image taking the forward wavefield F(t) = sponge x hist[t - a] of the
-        shot's history (fields in the FWI order sxx, szz, sxz, u, w, syx, syz,
-        v of the active systems)."""
+        image taking the forward wavefield F(t) = sponge x the post-injection
+        velocities of step t in the shot's history buffer (at the layout given
+        to ``external_begin``)."""
         n = self.nx * self.nz
         nslot = b - a
         if nslot > self.L:

@@ -31,7 +31,7 @@ import torch
 from . import configs
 from .batch import BatchRTM, ShotSpec, checkpoint_interval
 from .field import device
-from .fwi import ElasticFWI, accumulate_in_shot_order
+from .fwi import HIST_EVERY, ElasticFWI, accumulate_in_shot_order
 
 _F32 = 4
 
@@ -49,8 +49,10 @@ class Plan:
 
 def _fwi_shot_bytes(cells, nt, nr, L):
     # state 5 + adjoint 5 + gradients 8 + workspace 8 + cell map 2 fields,
-    # checkpoints ceil(nt/L) x 5, history L x 5, records / residuals / data
-    fields = 28 + math.ceil(nt / L) * 5 + L * 5
+    # checkpoints ceil(nt/L) x 5, the blocked kernels' compact history of L
+    # steps (velocities 2 per step, stresses 3 per HIST_EVERY steps),
+    # records / residuals / data
+    fields = 28 + math.ceil(nt / L) * 5 + L * 2 + math.ceil(L / HIST_EVERY) * 3
     return fields * cells * _F32 + 12 * nr * nt * _F32
 
 
@@ -78,7 +80,7 @@ def plan(nx, nz, shots: Sequence[ShotSpec], budget_bytes, isnap=1, share_forward
     # both media (27 fields), image accumulators (6), gradient sum (8), the
     # float64 chain rule (30), margin
     shared = 80 * cells * _F32
-    L_fwi = checkpoint_interval(nt, 5, 5)
+    L_fwi = checkpoint_interval(nt, 5, 2 + 3 / HIST_EVERY)
     per_fwi = _fwi_shot_bytes(cells, nt, nr, L_fwi)
     if share_forward:
         L_rtm = L_fwi
@@ -168,7 +170,7 @@ class GradientAndImage:
             rtm = BatchRTM(self.nx, self.nz, self.dx, self.dz, self.fs, self._mu, self._lam,
                            self._rho, self.fw, group, isnap=1, imaging="checkpoint",
                            kernel=rkernel)
-            rtm.external_begin(fwi.systems, fwi.L)
+            rtm.external_begin(fwi.systems, fwi.L, fwi.history_velocity_layout())
             fwi.gradient(res, on_segment=rtm.external_segment)
             codes = rtm.external_codes()
         else:

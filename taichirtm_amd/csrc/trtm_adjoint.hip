@@ -510,10 +510,22 @@ static int check_fwi(const trtm_fwi_desc* d, int it_begin, int it_end) {
     return TRTM_OK;
 }
 
-// blocked kernels: float with a workspace, unless one step per launch is asked for
-static bool use_block(const trtm_fwi_desc* d) {
+// blocked kernels: float with a workspace, unless one step per launch is asked
+// for, or (automatic choice) the adjoint is given the full history layout,
+// which the one-step-per-launch adjoint reads
+static bool use_block(const trtm_fwi_desc* d, bool adjoint) {
     if (d->kernel == TRTM_KERNEL_SIMPLE) return false;
+    if (adjoint && d->kernel == TRTM_KERNEL_AUTO && d->hist_every == 0) return false;
     return d->real_bytes == 4 && d->workspace != nullptr && d->nx >= 3 && d->nz >= 3;
+}
+
+// the compact history layout is written and read by the blocked kernels only
+static int check_layout(const trtm_fwi_desc* d, bool adjoint) {
+    if (d->hist_every < 0) return fail(TRTM_ERR_ARG, "hist_every must be >= 0");
+    if (d->hist && d->hist_every > 0 && !use_block(d, adjoint))
+        return fail(TRTM_ERR_ARG, "the compact history (hist_every > 0) needs the blocked "
+                                  "kernels (float, the workspace, a grid of at least 3 x 3)");
+    return TRTM_OK;
 }
 
 static int hist_fields(int systems) {
@@ -751,8 +763,9 @@ static int check_batch(const trtm_fwi_desc* d, int n, int it_begin, int it_end, 
             e->hist_len != d->hist_len || (e->hist == nullptr) != (d->hist == nullptr))
             return fail(TRTM_ERR_ARG, "shot %d does not share the grid, precision, systems and "
                                       "history window of shot 0", q);
-        if (use_block(e) != use_block(d) || e->tile_x != d->tile_x || e->tile_z != d->tile_z ||
-            e->block_steps != d->block_steps)
+        if (use_block(e, adjoint) != use_block(d, adjoint) || e->tile_x != d->tile_x ||
+            e->tile_z != d->tile_z || e->block_steps != d->block_steps ||
+            e->hist_every != d->hist_every)
             return fail(TRTM_ERR_ARG, "shot %d does not share the step kernel of shot 0", q);
         nt_max = std::max(nt_max, e->nt);
     }
@@ -772,17 +785,17 @@ static int check_window(const trtm_fwi_desc* d, int it_begin, int it_end) {
 }
 
 int trtm_fwi_forward(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, void* stream) {
-    const int rc = check_fwd(d, it_begin, it_end);
-    if (rc) return rc;
-    if (use_block(d)) return fwi_block_forward(d, 1, it_begin, it_end, (hipStream_t)stream);
+    int rc = check_fwd(d, it_begin, it_end);
+    if (rc || (rc = check_layout(d, false))) return rc;
+    if (use_block(d, false)) return fwi_block_forward(d, 1, it_begin, it_end, (hipStream_t)stream);
     if (d->real_bytes == 4) return fwi_forward_t<float>(d, 1, it_begin, it_end, (hipStream_t)stream);
     return fwi_forward_t<double>(d, 1, it_begin, it_end, (hipStream_t)stream);
 }
 
 int trtm_fwi_adjoint(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, void* stream) {
     int rc = check_adj(d, it_begin, it_end);
-    if (rc) return rc;
-    if (use_block(d)) {
+    if (rc || (rc = check_layout(d, true))) return rc;
+    if (use_block(d, true)) {
         if ((rc = check_window(d, it_begin, it_end))) return rc;
         return fwi_block_adjoint(d, 1, it_begin, it_end, (hipStream_t)stream);
     }
@@ -792,9 +805,9 @@ int trtm_fwi_adjoint(const trtm_fwi_desc* d, int32_t it_begin, int32_t it_end, v
 
 int trtm_fwi_forward_batch(const trtm_fwi_desc* descs, int32_t n, int32_t it_begin,
                            int32_t it_end, void* stream) {
-    const int rc = check_batch(descs, n, it_begin, it_end, false);
-    if (rc) return rc;
-    if (use_block(descs)) return fwi_block_forward(descs, n, it_begin, it_end, (hipStream_t)stream);
+    int rc = check_batch(descs, n, it_begin, it_end, false);
+    if (rc || (rc = check_layout(descs, false))) return rc;
+    if (use_block(descs, false)) return fwi_block_forward(descs, n, it_begin, it_end, (hipStream_t)stream);
     if (descs->real_bytes == 4)
         return fwi_forward_t<float>(descs, n, it_begin, it_end, (hipStream_t)stream);
     return fwi_forward_t<double>(descs, n, it_begin, it_end, (hipStream_t)stream);
@@ -825,8 +838,8 @@ int trtm_fwi_forward_scattered_batch(const trtm_fwi_desc* descs, int32_t n, int3
 int trtm_fwi_adjoint_batch(const trtm_fwi_desc* descs, int32_t n, int32_t it_begin,
                            int32_t it_end, void* stream) {
     int rc = check_batch(descs, n, it_begin, it_end, true);
-    if (rc) return rc;
-    if (use_block(descs)) {
+    if (rc || (rc = check_layout(descs, true))) return rc;
+    if (use_block(descs, true)) {
         if ((rc = check_window(descs, it_begin, it_end))) return rc;
         return fwi_block_adjoint(descs, n, it_begin, it_end, (hipStream_t)stream);
     }

@@ -9,8 +9,11 @@
 // is bit-identical to the one-step-per-launch FWI kernels.  Sources and
 // receivers come from the per-cell group maps (source groups are injected at
 // every region cell that holds one, records are written by the tile that owns
-// the cell).  With HIST the tile also writes the step's history (pre-step
-// stresses, post-injection velocities) that the adjoint reads.
+// the cell).  With HIST the tile also writes the step's history that the
+// adjoint reads: the post-injection velocities of every step and the pre-step
+// stresses of every hist_every-th step (the compact layout, hist_every > 0), or
+// both of every step (hist_every = 0, the layout of the one-step-per-launch
+// adjoint).
 //
 // Adjoint (k_fwi_adj_blk): the exact adjoint step (residual injection, sponge
 // transpose, stress-update transpose with the lam / mu / mxz (myx / myz)
@@ -20,7 +23,12 @@
 // cells in registers; only the values neighbours gather (the stress
 // transposes q and the scaled velocity adjoints b) pass through LDS.  The
 // gradients of the tile's cells are accumulated in registers in the same
-// order as the per-step kernels and written once per launch.
+// order as the per-step kernels and written once per launch.  It reads the
+// compact history with hist_every = K: a launch lies inside one interval of
+// K steps whose first step's stresses are stored, and the pre-step stresses of
+// the interval's later steps are recomputed at the start of the launch, over
+// the tile and its low ring, by the forward stress update and sponge from the
+// stored velocities (the forward's expressions in its order, so bit for bit).
 //
 // State ping-pongs between the caller's buffers and a workspace copy (tiles
 // read the halo that neighbouring tiles write).  A call issues an even number
@@ -37,6 +45,7 @@ namespace trtm {
 struct FB {
     int nx, nz, nt, fw;
     int n_rec, hist_base, hist_len, flip;  // flip: the current state is in alt
+    int hist_every;                        // compact history: stresses of every hist_every-th step
     float dt, idx, idz, factor;
     const float* tab;
     const float *lam, *mu, *mxz, *myx, *myz, *ir, *iru, *irw;
@@ -59,6 +68,17 @@ struct HF {
     static constexpr int sxx = 0, szz = 1, sxz = 2, u = 3, w = 4;
     static constexpr int syx = PSV ? 5 : 0, syz = PSV ? 6 : 1, v = PSV ? 7 : 2;
     static constexpr int n = (PSV ? 5 : 0) + (SH ? 3 : 0);
+};
+
+// compact history of hist_len steps: the velocity slots (one per step: u, w, v
+// of the active systems), then the stress slots (one per hist_every steps,
+// from hist_base: sxx, szz, sxz, syx, syz of the active systems)
+template <bool PSV, bool SH>
+struct HC {
+    static constexpr int u = 0, w = 1, v = PSV ? 2 : 0;
+    static constexpr int nv = (PSV ? 2 : 0) + (SH ? 1 : 0);
+    static constexpr int sxx = 0, szz = 1, sxz = 2, syx = PSV ? 3 : 0, syz = PSV ? 4 : 1;
+    static constexpr int ns = (PSV ? 3 : 0) + (SH ? 2 : 0);
 };
 
 // Entry of the sponge table that sponge_at reads for cell (i, j), or -1 for
@@ -97,19 +117,27 @@ __device__ __forceinline__ void lds_barrier() {
 __device__ __forceinline__ void vm_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // Adjoint history window of a tile: rows i0-1 .. i0+TX, columns j0-1 .. j0+TZ,
-// in 64-cell chunks per field, two buffers (the step being taken and the next).
+// in 64-cell chunks per field.  The velocity windows come in two buffers (the
+// step being taken and the next); the stress windows in `slots` buffers: the
+// recomputed pre-step stresses of the interval's later steps, and the stored
+// stresses of its first step.
 template <bool PSV, bool SH, int TX, int TZ>
 struct HistWindow {
     static constexpr int HX = TX + 2, HZ = TZ + 2;
     static constexpr int chunks = (HX * HZ + 63) / 64;
-    static constexpr int fields = (PSV ? 5 : 0) + (SH ? 3 : 0);
-    static constexpr int floats = 2 * fields * chunks * 64;
+    static constexpr int field = chunks * 64;  // floats per field
+    static constexpr int vfloats = HC<PSV, SH>::nv * field;  // one velocity window
+    static constexpr int sfloats = HC<PSV, SH>::ns * field;  // one stress window
 };
+template <int K>
+constexpr int adj_stress_slots() { return K - 1 > 2 ? K - 1 : 2; }
 
 template <bool PSV, bool SH, int TX, int TZ, int K>
 constexpr size_t adj_lds_bytes() {
+    using W = HistWindow<PSV, SH, TX, TZ>;
     constexpr int RN = (TX + 2 * K) * (TZ + 2 * K);
-    return 4 * ((size_t)HistWindow<PSV, SH, TX, TZ>::floats + (PSV ? 5 : 0) * RN + (SH ? 3 : 0) * RN);
+    return 4 * (2 * (size_t)W::vfloats + (size_t)adj_stress_slots<K>() * W::sfloats +
+                (size_t)((PSV ? 5 : 0) + (SH ? 3 : 0)) * RN);
 }
 constexpr size_t kLdsLimit = 160 * 1024;
 
@@ -232,26 +260,40 @@ __global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int
     for (int s = 0; s < K; s++) {
         const int it = t_a + s;
         if (it >= hi) break;
-        float* hh = nullptr;
+        // this step's history: stress fields (hs) and velocity fields (hv)
+        float *hs = nullptr, *hv = nullptr;
         if (HIST) {
             const int slot = it - a.hist_base;
-            if (slot >= 0 && slot < a.hist_len) hh = a.hist + (size_t)slot * (H::n * n);
+            if (slot >= 0 && slot < a.hist_len) {
+                if (a.hist_every > 0) {
+                    hv = a.hist + (size_t)slot * (HC<PSV, SH>::nv * n);
+                    if (slot % a.hist_every == 0)
+                        hs = a.hist + ((size_t)a.hist_len * HC<PSV, SH>::nv +
+                                       (size_t)(slot / a.hist_every) * HC<PSV, SH>::ns) * n;
+                } else {
+                    hs = hv = a.hist + (size_t)slot * (H::n * n);
+                }
+            }
         }
+        const bool cmp = a.hist_every > 0;
+        const int o_syx = cmp ? HC<PSV, SH>::syx : H::syx, o_syz = cmp ? HC<PSV, SH>::syz : H::syz;
+        const int o_u = cmp ? HC<PSV, SH>::u : H::u, o_w = cmp ? HC<PSV, SH>::w : H::w;
+        const int o_v = cmp ? HC<PSV, SH>::v : H::v;
         // ---- history: the stresses this step's velocity update reads
-        if (HIST && hh) {
+        if (HIST && hs) {
 #pragma unroll
             for (int k = 0; k < NC; k++) {
                 if ((fl[k] & 5u) != 5u) continue;
                 const int e = t + NT * k;
                 const int c = gc[k];
                 if (PSV) {
-                    st(hh + H::sxx * n, c, sXX[e]);
-                    st(hh + H::szz * n, c, sZZ[e]);
-                    st(hh + H::sxz * n, c, sXZ[e]);
+                    st(hs + H::sxx * n, c, sXX[e]);
+                    st(hs + H::szz * n, c, sZZ[e]);
+                    st(hs + H::sxz * n, c, sXZ[e]);
                 }
                 if (SH) {
-                    st(hh + H::syx * n, c, sYX[e]);
-                    st(hh + H::syz * n, c, sYZ[e]);
+                    st(hs + o_syx * n, c, sYX[e]);
+                    st(hs + o_syz * n, c, sYZ[e]);
                 }
             }
         }
@@ -283,9 +325,9 @@ __global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int
                     }
                 sU[e] = u;
                 sW[e] = w;
-                if (HIST && hh && (fl[k] & 4u)) {
-                    st(hh + H::u * n, gc[k], u);
-                    st(hh + H::w * n, gc[k], w);
+                if (HIST && hv && (fl[k] & 4u)) {
+                    st(hv + o_u * n, gc[k], u);
+                    st(hv + o_w * n, gc[k], w);
                 }
             }
             if (SH) {
@@ -300,7 +342,7 @@ __global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int
                     for (int q = sstart[sg[k]]; q < sstart[sg[k] + 1]; q++)
                         v = v + (wv[(size_t)sorder[q] * nt + it] * factor) * d0[k];
                 sV[e] = v;
-                if (HIST && hh && (fl[k] & 4u)) st(hh + H::v * n, gc[k], v);
+                if (HIST && hv && (fl[k] & 4u)) st(hv + o_v * n, gc[k], v);
             }
         }
         __syncthreads();
@@ -377,8 +419,9 @@ __global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int
 }
 
 // ---------------------------------------------------------------------------
-// adjoint, steps t_b-1 down to t_a (t_b - t_a <= K); steps at or beyond the
-// shot's record length leave its state unchanged
+// adjoint, steps t_b-1 down to t_a (t_b - t_a <= K, all inside one interval of
+// K steps from hist_base); steps at or beyond the shot's record length leave
+// its state unchanged
 
 template <bool PSV, bool SH, int TX, int TZ, int NT, int K, class S>
 __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int tiles_x,
@@ -386,15 +429,19 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
     constexpr int RX = TX + 2 * K, RZ = TZ + 2 * K, RN = RX * RZ;
     constexpr int NC = (RN + NT - 1) / NT;
     constexpr int NP = PSV ? RN : 1, NS = SH ? RN : 1;
-    using H = HF<PSV, SH>;
     using W = HistWindow<PSV, SH, TX, TZ>;
+    using HV = HC<PSV, SH>;
+    constexpr int NSL = adj_stress_slots<K>();
+    constexpr int FW_ = W::field;
     static_assert(adj_lds_bytes<PSV, SH, TX, TZ, K>() <= kLdsLimit, "LDS");
     __shared__ float qX[NP], qZ[NP], qS[NP], bU[NP], bW[NP];
     __shared__ float qY[NS], qV[NS], bV[NS];
-    // history windows [field][chunk * 64 + lane] of alternate steps (two
+    // velocity windows [field][chunk * 64 + lane] of alternate steps (two
     // arrays, so the LDS reads of one are not ordered after the loads into the
-    // other)
-    __shared__ float hwA[W::floats / 2], hwB[W::floats / 2];
+    // other), and the interval's pre-step stress windows: slot s - alpha - 1
+    // for step s > alpha, slot 1 for step alpha (taken once step alpha + 2 is done)
+    __shared__ float hwA[W::vfloats], hwB[W::vfloats];
+    __shared__ float hsw[NSL * W::sfloats];
 
     const FB& a = src.get();
     const int tiles = tiles_x * tiles_z;
@@ -407,6 +454,13 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
     const int t = threadIdx.x;
     const bool has_src = a.src_map != nullptr, has_rec = a.rec_map != nullptr;
     float* const* in = a.flip ? a.alt : a.st;
+    const size_t n = (size_t)nx * nz;
+    // the launch's steps t_a .. top lie in the interval of K steps from alpha,
+    // whose pre-step stresses are stored
+    const int alpha = a.hist_base + ((t_a - a.hist_base) / K) * K;
+    const int top = (t_b < nt ? t_b : nt) - 1;
+    const float* const hstr =
+        a.hist + ((size_t)a.hist_len * HV::nv + (size_t)((alpha - a.hist_base) / K) * HV::ns) * n;
 
     // The gradient and history loads of a region cell outside the tile (or
     // off the interior) read a nearby tile cell (an interior one) instead:
@@ -422,7 +476,9 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
         return ti * nz + tj;
     };
     int gc[NC], sg[NC], rg[NC];
-    unsigned fl[NC];  // bit0 in grid, bit1 interior, bit2 in tile, bit3 phase 1 ok, bit4 phase 2 ok
+    // bit0 in grid, bit1 interior, bit2 in tile, bit3 phase 1 ok, bit4 phase 2
+    // ok, bit5 in the stress window (the tile and its low ring, in the grid)
+    unsigned fl[NC];
     float A[NC], c0[NC], c1[NC], c2[NC], c3[NC], c4[NC], d0[NC], d1[NC], d2[NC];
     float pu[NC], pw[NC], pxx[NC], pzz[NC], pxz[NC], pv[NC], pyx[NC], pyz[NC];
     float gl[NC], gm[NC], gx[NC], gu[NC], gw[NC], gyx[NC], gyz[NC], gr[NC];
@@ -434,6 +490,9 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
         gc[k] = rc.c;
         fl[k] = rc.f;
         slot[k] = rc.slot;
+        const int r = e / RZ, z = e - r * RZ;
+        if ((rc.f & 1u) && r >= K - 1 && r <= K - 1 + TX && z >= K - 1 && z <= K - 1 + TZ)
+            fl[k] |= 32u;
         if (!(rc.f & 1u) && e < RN) {
             // outside the grid: the transposes read zero here (not interior)
             if (PSV) qX[e] = qZ[e] = qS[e] = bU[e] = bW[e] = 0.0f;
@@ -482,38 +541,49 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
         }
     }
 
-    const size_t n = (size_t)nx * nz;
     const float idx = a.idx, idz = a.idz, dt = a.dt, factor = a.factor;
     const gptr<const int> sorder = G(a.src_order), sstart = G(a.src_start);
     const gptr<const int> rorder = G(a.rec_order), rstart = G(a.rec_start);
     const gptr<const float> wu = G(a.wav_u), wv = G(a.wav_v), ww = G(a.wav_w);
     const gptr<const float> res_u = G(a.res_u), res_v = G(a.res_v), res_w = G(a.res_w);
 
-    // The history window of step it goes to buffer b asynchronously (global
-    // to LDS): issued one step ahead, waited for before the step's phase 1b.
-    // Wave w copies chunks q = w, w + NW, ... of the window (q = field *
-    // chunks + chunk); the source offset of this lane in each of them is the
-    // same every step, so it is computed once.
-    constexpr int NW = NT / 64, NQ = W::fields * W::chunks, QPW = (NQ + NW - 1) / NW;
+    // History windows go global -> LDS asynchronously.  Wave w copies chunks
+    // q = w, w + NW, ... of a window (q = field * chunks + chunk); the source
+    // offset of this lane in each velocity chunk is the same every step, so
+    // it is computed once.
+    constexpr int NW = NT / 64;
+    constexpr int NQV = HV::nv * W::chunks, QPV = (NQV + NW - 1) / NW;
+    constexpr int NQS = HV::ns * W::chunks, QPS = (NQS + NW - 1) / NW;
     const int lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
-    unsigned poff[QPW];  // byte offset from the step's history slot
-#pragma unroll
-    for (int m = 0; m < QPW; m++) {
-        const int q = wave + NW * m;
+    auto win_off = [&](int q, int nq) {
         const int f = q / W::chunks, ch = q - f * W::chunks;
         const int w = ch * 64 + lane;
         const int r = w / W::HZ, z = w - r * W::HZ;
         const int i = clampi(i0 - 1 + r, 0, nx - 1), j = clampi(j0 - 1 + z, 0, nz - 1);
-        poff[m] = (unsigned)((q < NQ ? f : 0) * (int)n + i * nz + j) * 4u;
-    }
-    auto prefetch = [&](int it, float* dst) {
-        const char* base = (const char*)(a.hist + (size_t)(it - a.hist_base) * (H::n * n));
+        return (unsigned)((q < nq ? f : 0) * (int)n + i * nz + j) * 4u;
+    };
+    unsigned voff[QPV];  // byte offset from the step's velocity slot
 #pragma unroll
-        for (int m = 0; m < QPW; m++) {
+    for (int m = 0; m < QPV; m++) voff[m] = win_off(wave + NW * m, NQV);
+    auto fetch_v = [&](int it, float* dst) {
+        const char* base = (const char*)(a.hist + (size_t)(it - a.hist_base) * (HV::nv * n));
+#pragma unroll
+        for (int m = 0; m < QPV; m++) {
             const int q = wave + NW * m;  // wave-uniform
-            if (q < NQ)
+            if (q < NQV)
                 __builtin_amdgcn_global_load_lds(
-                    (const void*)(base + poff[m]),
+                    (const void*)(base + voff[m]),
+                    (__attribute__((address_space(3))) void*)&dst[q * 64], 4, 0, 0);
+        }
+    };
+    auto fetch_s = [&](float* dst) {  // the stored stresses of step alpha
+        const char* base = (const char*)hstr;
+#pragma unroll
+        for (int m = 0; m < QPS; m++) {
+            const int q = wave + NW * m;
+            if (q < NQS)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(base + win_off(q, NQS)),
                     (__attribute__((address_space(3))) void*)&dst[q * 64], 4, 0, 0);
         }
     };
@@ -525,21 +595,91 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
         const int r = e / RZ, z = e - r * RZ;
         wo[k] = (r - K + 1) * W::HZ + (z - K + 1);
     }
-    auto hv = [&](const float* hb, int f, int w) { return hb[f * W::chunks * 64 + w]; };
+    // velocity window of step s (the step taken first, top, uses hwA)
+    auto vwin = [&](int s) { return ((top - s) & 1) ? hwB : hwA; };
 
-    // step t_b-1-s uses window s & 1; the first step taken is below nt
-    const int first = (t_b < nt ? t_b : nt) - 1;
-    if ((t_b - 1 - first) & 1)
-        prefetch(first, hwB);
-    else
-        prefetch(first, hwA);
+    // ---- pre-step stresses of steps alpha + 1 .. top over the stress window:
+    // the forward stress update and sponge (k_fwi_fwd_blk) from the stored
+    // stresses of step alpha and the velocities of steps alpha .. top - 1
+    if (top > alpha) {
+        float rxx[NC], rzz[NC], rxz[NC], ryx[NC], ryz[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            rxx[k] = rzz[k] = rxz[k] = ryx[k] = ryz[k] = 0.0f;
+            if (!(fl[k] & 32u)) continue;
+            const int c = gc[k];
+            if (PSV) {
+                rxx[k] = ld(hstr + HV::sxx * n, c);
+                rzz[k] = ld(hstr + HV::szz * n, c);
+                rxz[k] = ld(hstr + HV::sxz * n, c);
+            }
+            if (SH) {
+                ryx[k] = ld(hstr + HV::syx * n, c);
+                ryz[k] = ld(hstr + HV::syz * n, c);
+            }
+        }
+        fetch_v(alpha, vwin(alpha));
+#pragma unroll
+        for (int r = 0; r < K - 1; r++) {
+            const int s = alpha + r;  // stresses of step s + 1 from step s
+            if (s >= top) break;
+            const float* const buf = vwin(s);
+            vm_wait();
+            lds_barrier();
+            fetch_v(s + 1, vwin(s + 1));  // its window was last read a step ago
+            float* const dst = hsw + r * W::sfloats;
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                if (!(fl[k] & 32u)) continue;
+                const int w = wo[k];
+                if (PSV) {
+                    float sxx = rxx[k], szz = rzz[k], sxz = rxz[k];
+                    if (fl[k] & 2u) {
+                        const float u0 = buf[HV::u * FW_ + w], w0 = buf[HV::w * FW_ + w];
+                        const float u_x = (buf[HV::u * FW_ + w + W::HZ] - u0) * idx;
+                        const float u_z = (buf[HV::u * FW_ + w + 1] - u0) * idz;
+                        const float w_x = (buf[HV::w * FW_ + w + W::HZ] - w0) * idx;
+                        const float w_z = (buf[HV::w * FW_ + w + 1] - w0) * idz;
+                        const float div_uw = u_x + w_z;
+                        sxx = sxx + dt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
+                        szz = szz + dt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
+                        sxz = sxz + (dt * c4[k]) * (u_z + w_x);
+                    }
+                    rxx[k] = sxx * A[k];
+                    rzz[k] = szz * A[k];
+                    rxz[k] = sxz * A[k];
+                    dst[HV::sxx * FW_ + w] = rxx[k];
+                    dst[HV::szz * FW_ + w] = rzz[k];
+                    dst[HV::sxz * FW_ + w] = rxz[k];
+                }
+                if (SH) {
+                    float syx = ryx[k], syz = ryz[k];
+                    if (fl[k] & 2u) {
+                        const float v0 = buf[HV::v * FW_ + w];
+                        const float v_x = (buf[HV::v * FW_ + w + W::HZ] - v0) * idx;
+                        const float v_z = (buf[HV::v * FW_ + w + 1] - v0) * idz;
+                        syx = syx + (dt * d1[k]) * v_x;
+                        syz = syz + (dt * d2[k]) * v_z;
+                    }
+                    ryx[k] = syx * A[k];
+                    ryz[k] = syz * A[k];
+                    dst[HV::syx * FW_ + w] = ryx[k];
+                    dst[HV::syz * FW_ + w] = ryz[k];
+                }
+            }
+        }
+    } else {
+        fetch_v(top, vwin(top));
+        if (t_a == alpha) fetch_s(hsw + W::sfloats);
+    }
+
 #pragma unroll
     for (int s = 0; s < K; s++) {
-        const int it = t_b - 1 - s;
+        const int it = top - s;
         if (it < t_a) break;
-        if (it >= nt) continue;
-        float* const buf = (s & 1) ? hwB : hwA;
+        const float* const buf = (s & 1) ? hwB : hwA;  // = vwin(it)
         float* const nxt = (s & 1) ? hwA : hwB;
+        const float* const sw = hsw + (it == alpha ? 1 : it - alpha - 1) * W::sfloats;
 
         // ---- phase 1a: data adjoint at the receivers; stress transposes q
         // of every region cell from its own sponged stress adjoint
@@ -570,9 +710,13 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                 qV[e] = in_ ? (d2[k] * dt) * (A[k] * pyz[k]) : 0.0f;
             }
         }
-        vm_wait();  // this step's history window (and the residuals) arrived
+        vm_wait();  // this step's windows (and the residuals) arrived
         lds_barrier();
-        if (it - 1 >= t_a) prefetch(it - 1, nxt);  // window last read a step ago
+        // the next step's velocity window (the one of top - 1 came with the
+        // recomputation); the window it replaces was last read a step ago
+        if (s > 0 && it - 1 >= t_a) fetch_v(it - 1, nxt);
+        // the stored stresses of step alpha, into slot 1 (last read by step alpha + 2)
+        if (t_a == alpha && it == alpha + 1) fetch_s(hsw + W::sfloats);
 
         // ---- phase 1b: new velocity adjoint; lam / mu / mxz (myx / myz) and
         // source-term gradients of the tile; b = dt * inv_rho * p for phase 2
@@ -580,7 +724,6 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
         for (int k = 0; k < NC; k++) {
             if (!(fl[k] & 1u)) continue;
             const int e = t + NT * k;
-            const int c = gc[k];
             const bool in_ = fl[k] & 2u;
             const bool own_in = (fl[k] & 6u) == 6u;
             if (fl[k] & 8u) {
@@ -593,11 +736,11 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                         const float axx = dt * A[k] * pxx[k], azz = dt * A[k] * pzz[k];
                         const float axz = dt * A[k] * pxz[k];
                         const int w = wo[k];
-                        const float u0 = hv(buf, H::u, w), w0 = hv(buf, H::w, w);
-                        const float ux = (hv(buf, H::u, w + W::HZ) - u0) * idx;
-                        const float uz = (hv(buf, H::u, w + 1) - u0) * idz;
-                        const float wx = (hv(buf, H::w, w + W::HZ) - w0) * idx;
-                        const float wz = (hv(buf, H::w, w + 1) - w0) * idz;
+                        const float u0 = buf[HV::u * FW_ + w], w0 = buf[HV::w * FW_ + w];
+                        const float ux = (buf[HV::u * FW_ + w + W::HZ] - u0) * idx;
+                        const float uz = (buf[HV::u * FW_ + w + 1] - u0) * idz;
+                        const float wx = (buf[HV::w * FW_ + w + W::HZ] - w0) * idx;
+                        const float wz = (buf[HV::w * FW_ + w + 1] - w0) * idz;
                         gl[k] += (axx + azz) * (ux + wz);
                         gm[k] += 2.0f * (axx * ux + azz * wz);
                         gx[k] += axz * (uz + wx);
@@ -610,9 +753,9 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                         A[k] * pv[k] + idx * (qY[e - RZ] - qY[e]) + idz * (qV[e - 1] - qV[e]);
                     if (own_in) {
                         const int w = wo[k];
-                        const float v0 = hv(buf, H::v, w);
-                        const float vx = (hv(buf, H::v, w + W::HZ) - v0) * idx;
-                        const float vz = (hv(buf, H::v, w + 1) - v0) * idz;
+                        const float v0 = buf[HV::v * FW_ + w];
+                        const float vx = (buf[HV::v * FW_ + w + W::HZ] - v0) * idx;
+                        const float vz = (buf[HV::v * FW_ + w + 1] - v0) * idz;
                         gyx[k] += dt * A[k] * pyx[k] * vx;
                         gyz[k] += dt * A[k] * pyz[k] * vz;
                     }
@@ -642,7 +785,6 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
         for (int k = 0; k < NC; k++) {
             if ((fl[k] & 17u) != 17u) continue;
             const int e = t + NT * k;
-            const int c = gc[k];
             const bool own_in = (fl[k] & 6u) == 6u;
             if (PSV) {
                 const float bu0 = bU[e], bw0 = bW[e];
@@ -651,12 +793,12 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                 pzz[k] = A[k] * pzz[k] + idz * (bw0 - bW[e + 1]);
                 if (own_in) {
                     const int w = wo[k];
-                    const float xx = hv(buf, H::sxx, w), xz = hv(buf, H::sxz, w);
-                    const float zz = hv(buf, H::szz, w);
-                    const float sxx_x = (xx - hv(buf, H::sxx, w - W::HZ)) * idx;
-                    const float sxz_z = (xz - hv(buf, H::sxz, w - 1)) * idz;
-                    const float sxz_x = (xz - hv(buf, H::sxz, w - W::HZ)) * idx;
-                    const float szz_z = (zz - hv(buf, H::szz, w - 1)) * idz;
+                    const float xx = sw[HV::sxx * FW_ + w], xz = sw[HV::sxz * FW_ + w];
+                    const float zz = sw[HV::szz * FW_ + w];
+                    const float sxx_x = (xx - sw[HV::sxx * FW_ + w - W::HZ]) * idx;
+                    const float sxz_z = (xz - sw[HV::sxz * FW_ + w - 1]) * idz;
+                    const float sxz_x = (xz - sw[HV::sxz * FW_ + w - W::HZ]) * idx;
+                    const float szz_z = (zz - sw[HV::szz * FW_ + w - 1]) * idz;
                     gu[k] += pu[k] * dt * (sxx_x + sxz_z);
                     gw[k] += pw[k] * dt * (sxz_x + szz_z);
                 }
@@ -667,9 +809,9 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                 pyz[k] = A[k] * pyz[k] + idz * (bv0 - bV[e + 1]);
                 if (own_in) {
                     const int w = wo[k];
-                    const float yx = hv(buf, H::syx, w), yz = hv(buf, H::syz, w);
-                    const float syx_x = (yx - hv(buf, H::syx, w - W::HZ)) * idx;
-                    const float syz_z = (yz - hv(buf, H::syz, w - 1)) * idz;
+                    const float yx = sw[HV::syx * FW_ + w], yz = sw[HV::syz * FW_ + w];
+                    const float syx_x = (yx - sw[HV::syx * FW_ + w - W::HZ]) * idx;
+                    const float syz_z = (yz - sw[HV::syz * FW_ + w - 1]) * idz;
                     gr[k] += pv[k] * dt * (syx_x + syz_z);
                 }
             }
@@ -1037,9 +1179,8 @@ __global__ __launch_bounds__(NT) void k_fwi_scat_blk(S src, int t_a, int t_b, in
 // host side
 
 template <bool PSV, bool SH, int TX, int TZ, int K>
-constexpr bool fwi_fits() {
-    constexpr size_t fwd = 4 * (size_t)(TX + 2 * K) * (TZ + 2 * K) * ((PSV ? 5 : 0) + (SH ? 3 : 0));
-    return fwd <= kLdsLimit && adj_lds_bytes<PSV, SH, TX, TZ, K>() <= kLdsLimit;
+constexpr bool adj_fits() {
+    return adj_lds_bytes<PSV, SH, TX, TZ, K>() <= kLdsLimit;
 }
 
 template <bool PSV, bool SH, int TX, int TZ, int K>
@@ -1052,11 +1193,15 @@ static bool fwd_cfg_fits(int systems, const BlockCfg& c) {
     return 4 * (size_t)(c.tx + 2 * c.k) * (c.tz + 2 * c.k) * nf <= kLdsLimit;
 }
 
-static bool fwi_cfg_fits(int systems, const BlockCfg& c) {
-    const int nf = ((systems & TRTM_PSV) ? 5 : 0) + ((systems & TRTM_SH) ? 3 : 0);
+// adj_lds_bytes for a runtime shape
+static bool adj_cfg_fits(int systems, const BlockCfg& c) {
+    const bool psv = systems & TRTM_PSV, sh = systems & TRTM_SH;
+    const size_t nf = (psv ? 5 : 0) + (sh ? 3 : 0), nv = (psv ? 2 : 0) + (sh ? 1 : 0);
+    const size_t ns = (psv ? 3 : 0) + (sh ? 2 : 0);
     const size_t rn = (size_t)(c.tx + 2 * c.k) * (c.tz + 2 * c.k);
-    const size_t hist = 2 * (size_t)nf * (((c.tx + 2) * (c.tz + 2) + 63) / 64) * 64;
-    return 4 * rn * nf <= kLdsLimit && 4 * (hist + rn * nf) <= kLdsLimit;
+    const size_t field = (((size_t)(c.tx + 2) * (c.tz + 2) + 63) / 64) * 64;
+    const size_t slots = c.k - 1 > 2 ? c.k - 1 : 2;
+    return 4 * (2 * nv * field + slots * ns * field + rn * nf) <= kLdsLimit;
 }
 
 // compiled shapes (tile rows, tile columns, threads, steps per launch)
@@ -1065,13 +1210,13 @@ static const BlockCfg kFwiCfg[] = {{24, 64, 768, 4},  {32, 64, 1024, 4}, {16, 64
                                    {16, 48, 512, 4},  {24, 32, 512, 4}};
 // (the 512-thread 16 x 48 and 24 x 32 shapes keep the P-SV adjoint within half
 // the LDS of a CU, so two workgroups share one)
-// defaults, the fastest measured on MI355X for 8 P-SV shots of 600 x 300: the
-// forward passes {24, 64, 768, 4}, the adjoint {32, 64, 1024, 4}; the first
-// listed shape that fits otherwise
-static const BlockCfg kFwiDefaultFwd = {24, 64, 768, 4}, kFwiDefaultAdj = {32, 64, 1024, 4};
-// forward-only shapes (their adjoint would not fit the LDS): 8 steps per
-// launch halve the state and coefficient traffic per step of the
-// checkpointing and history passes
+// defaults: the forward passes {24, 64, 768, 4} (the fastest measured on
+// MI355X for 8 P-SV shots of 600 x 300), the adjoint {16, 48, 512, 4} (its
+// LDS fits every wave system); the first listed shape that fits otherwise
+static const BlockCfg kFwiDefaultFwd = {24, 64, 768, 4}, kFwiDefaultAdj = {16, 48, 512, 4};
+// forward-only shapes (not listed for the adjoint pass): 8 steps per launch
+// halve the state and coefficient traffic per step of the checkpointing and
+// history passes
 static const BlockCfg kFwiFwdOnly[] = {{32, 64, 1024, 8}, {24, 64, 768, 8}, {48, 64, 1024, 8}};
 
 static bool scat_cfg_fits(int systems, const BlockCfg& c);
@@ -1090,7 +1235,9 @@ int fwi_block_configs(int systems, int pass, int32_t* out, int max_n) {
         n++;
     };
     for (const BlockCfg& c : kFwiCfg)
-        if (fwi_cfg_fits(systems, c) && (pass != 2 || scat_cfg_fits(systems, c))) add(c);
+        if (pass == 0 ? fwd_cfg_fits(systems, c)
+                      : pass == 1 ? adj_cfg_fits(systems, c) : scat_cfg_fits(systems, c))
+            add(c);
     if (pass != 1)
         for (const BlockCfg& c : kFwiFwdOnly)
             if (pass == 0 ? fwd_cfg_fits(systems, c) : scat_cfg_fits(systems, c)) add(c);
@@ -1101,7 +1248,10 @@ int fwi_block_configs(int systems, int pass, int32_t* out, int max_n) {
 // matches every nonzero request
 static int fwi_choose(const trtm_fwi_desc* d, bool adjoint, BlockCfg* out) {
     const BlockCfg& def = adjoint ? kFwiDefaultAdj : kFwiDefaultFwd;
-    if (d->tile_x <= 0 && d->tile_z <= 0 && d->block_steps <= 0 && fwi_cfg_fits(d->systems, def)) {
+    auto fits = [&](const BlockCfg& c) {
+        return adjoint ? adj_cfg_fits(d->systems, c) : fwd_cfg_fits(d->systems, c);
+    };
+    if (d->tile_x <= 0 && d->tile_z <= 0 && d->block_steps <= 0 && fits(def)) {
         *out = def;
         return TRTM_OK;
     }
@@ -1110,7 +1260,7 @@ static int fwi_choose(const trtm_fwi_desc* d, bool adjoint, BlockCfg* out) {
                (d->block_steps <= 0 || c.k == d->block_steps);
     };
     for (const BlockCfg& c : kFwiCfg)
-        if (fwi_cfg_fits(d->systems, c) && match(c)) {
+        if (fits(c) && match(c)) {
             *out = c;
             return TRTM_OK;
         }
@@ -1129,6 +1279,7 @@ static FB make_fb(const trtm_fwi_desc* d, bool adjoint) {
     FB a;
     a.nx = d->nx; a.nz = d->nz; a.nt = d->nt; a.fw = d->fw;
     a.n_rec = d->n_rec; a.hist_base = d->hist_base; a.hist_len = d->hist_len; a.flip = 0;
+    a.hist_every = d->hist_every;
     a.dt = (float)d->dt; a.idx = (float)d->inv_dx; a.idz = (float)d->inv_dz;
     a.factor = (float)d->src_factor;
     a.tab = (const float*)d->sponge_table;
@@ -1160,6 +1311,19 @@ static FB make_fb(const trtm_fwi_desc* d, bool adjoint) {
 }
 
 template <bool PSV, bool SH, int TX, int TZ, int NT, int K, class S>
+static void launch_fwd(bool hist, const S& src, int shots, int t_a, int t_b, int tiles_x,
+                       int tiles_z, hipStream_t s) {
+    const int tiles = tiles_x * tiles_z;
+    const dim3 grid(8 * ((tiles + 7) / 8), 1, shots);
+    if (hist)
+        hipLaunchKernelGGL((k_fwi_fwd_blk<PSV, SH, TX, TZ, NT, K, true, S>), grid, dim3(NT), 0, s,
+                           src, t_a, t_b, tiles_x, tiles_z);
+    else
+        hipLaunchKernelGGL((k_fwi_fwd_blk<PSV, SH, TX, TZ, NT, K, false, S>), grid, dim3(NT), 0,
+                           s, src, t_a, t_b, tiles_x, tiles_z);
+}
+
+template <bool PSV, bool SH, int TX, int TZ, int NT, int K, class S>
 static void launch_fb(bool adjoint, bool hist, const S& src, int shots, int t_a, int t_b,
                       int tiles_x, int tiles_z, hipStream_t s) {
     const int tiles = tiles_x * tiles_z;
@@ -1180,11 +1344,16 @@ static int launch_cfg(const BlockCfg& c, bool adjoint, bool hist, const S& src, 
                       int t_a, int t_b, int nx, int nz, hipStream_t s) {
     const int tx_ = (nx + c.tx - 1) / c.tx, tz_ = (nz + c.tz - 1) / c.tz;
 #define TRTM_FB(TX, TZ, NT, K)                                                                  \
-    if constexpr (fwi_fits<PSV, SH, TX, TZ, K>()) {                                             \
-        if (c.tx == TX && c.tz == TZ && c.nt == NT && c.k == K) {                               \
+    if (c.tx == TX && c.tz == TZ && c.nt == NT && c.k == K) {                                   \
+        if constexpr (adj_fits<PSV, SH, TX, TZ, K>() && fwd_fits<PSV, SH, TX, TZ, K>()) {       \
             launch_fb<PSV, SH, TX, TZ, NT, K, S>(adjoint, hist, src, shots, t_a, t_b, tx_, tz_, \
                                                  s);                                            \
             return TRTM_OK;                                                                     \
+        } else if constexpr (fwd_fits<PSV, SH, TX, TZ, K>()) {                                  \
+            if (!adjoint) {                                                                     \
+                launch_fwd<PSV, SH, TX, TZ, NT, K, S>(hist, src, shots, t_a, t_b, tx_, tz_, s); \
+                return TRTM_OK;                                                                 \
+            }                                                                                   \
         }                                                                                       \
     }
     TRTM_FB(16, 64, 512, 4)
@@ -1243,17 +1412,28 @@ static int fwi_block_run(const trtm_fwi_desc* d, int n, int it_begin, int it_end
         if (!d[q].workspace)
             return fail(TRTM_ERR_ARG, "shot %d: blocked FWI kernel needs the workspace", q);
     const bool hist = !adjoint && d->hist != nullptr;
+    const int K = c.k, groups = (n + kShotsPerLaunch - 1) / kShotsPerLaunch;
+    if (adjoint && d->hist_every != K)
+        return fail(TRTM_ERR_ARG, "the blocked FWI adjoint of %d steps per launch reads the "
+                    "compact history with hist_every = %d (got %d)", K, K, d->hist_every);
+    if (hist && d->hist_every < 0)
+        return fail(TRTM_ERR_ARG, "hist_every must be >= 0");
     std::vector<FB> fb(n);
     for (int q = 0; q < n; q++) fb[q] = make_fb(&d[q], adjoint);
-    const int K = c.k, groups = (n + kShotsPerLaunch - 1) / kShotsPerLaunch;
-    // launches of up to K steps in stepping order; an even number of them (the
-    // first one split in two when needed) leaves the state in the caller's
-    // buffers for every shot that runs to the end of the range
+    // launches of up to K steps in stepping order (adjoint: each inside one
+    // interval of K steps from hist_base, whose first step's stresses are
+    // stored); an even number of them (the first one split in two when
+    // needed) leaves the state in the caller's buffers for every shot that
+    // runs to the end of the range
     std::vector<std::pair<int, int>> chunks;
-    for (int l = 0; l * K < it_end - it_begin; l++) {
-        const int t_a = adjoint ? std::max(it_end - (l + 1) * K, it_begin) : it_begin + l * K;
-        const int t_b = adjoint ? it_end - l * K : std::min(t_a + K, it_end);
-        chunks.push_back({t_a, t_b});
+    if (adjoint) {
+        for (int hi = it_end; hi > it_begin;) {
+            const int lo = std::max(d->hist_base + ((hi - 1 - d->hist_base) / K) * K, it_begin);
+            chunks.push_back({lo, hi});
+            hi = lo;
+        }
+    } else {
+        for (int t_a = it_begin; t_a < it_end; t_a += K) chunks.push_back({t_a, std::min(t_a + K, it_end)});
     }
     if (chunks.size() % 2 == 1 && chunks[0].second - chunks[0].first > 1) {
         const auto first = chunks[0];

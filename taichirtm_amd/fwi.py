@@ -32,7 +32,11 @@ give the same records and gradients bit for bit; ``"auto"`` is ``"blocked"``
 with the library's shape per pass in float32 and ``"step"`` in float64;
 ``"tuned"`` times every compiled blocked shape on a window of this problem's
 own steps, per pass (checkpointing forward, history forward, adjoint), and
-keeps the fastest.
+keeps the fastest.  The blocked kernels keep a compact history: the
+post-injection velocities of every step, but the pre-step stresses only of
+every K-th step (K = the adjoint's steps per launch, 4 by default); the
+adjoint recomputes the others at the start of each launch with the forward
+stress update, bit for bit.
 """
 from __future__ import annotations
 
@@ -52,6 +56,9 @@ _REAL = {"float32": (torch.float32, 4), "float64": (torch.float64, 8)}
 _KERNELS = {"auto": _lib.KERNEL_AUTO, "step": _lib.KERNEL_SIMPLE, "blocked": _lib.KERNEL_BLOCKED,
             "tuned": _lib.KERNEL_BLOCKED}
 _WINDOW = 64  # steps per autotuning trial
+# steps between the stored stresses of the compact history of the blocked
+# kernels (and the blocked adjoint's steps per launch) unless a shape is given
+HIST_EVERY = 4
 GRAD_NAMES = ("g_lam", "g_mu", "g_mxz", "g_myx", "g_myz", "g_ir", "g_iru", "g_irw")
 
 
@@ -225,6 +232,7 @@ class ElasticFWI:
         d.cell_map = _ptr(g["map"])
         d.kernel = _KERNELS[self.kernel]
         d.workspace = _ptr(g["ws"]) if g["ws"] is not None else None
+        d.hist_every = getattr(self, "hist_every", 0)
         return d
 
     def _systems(self):
@@ -240,6 +248,45 @@ class ElasticFWI:
         if self.systems & _lib.SH:
             names += ["v", "syx", "syz"]
         return names
+
+    # -- history layout -------------------------------------------------------
+    # The blocked kernels keep the compact history: the post-injection
+    # velocities of every step, the pre-step stresses of every hist_every-th
+    # step (the blocked adjoint's steps per launch; it recomputes the others).
+    # The one-step-per-launch kernels keep both of every step (hist_every 0).
+
+    def _choose_hist_every(self):
+        if not self.blocked:
+            return 0
+        return int(self.shapes["adjoint"][2]) or HIST_EVERY
+
+    def _hist_counts(self):
+        nv = (2 if self.systems & _lib.PSV else 0) + (1 if self.systems & _lib.SH else 0)
+        ns = (3 if self.systems & _lib.PSV else 0) + (2 if self.systems & _lib.SH else 0)
+        return nv, ns
+
+    def history_fields(self, L):
+        """Fields (each nx x nz) of the history of L steps."""
+        nv, ns = self._hist_counts()
+        if self.hist_every:
+            return L * nv + -(-L // self.hist_every) * ns
+        return L * (nv + ns)
+
+    def history_fields_per_step(self):
+        """Average fields per step of the history (the checkpoint spacing's
+        memory model)."""
+        nv, ns = self._hist_counts()
+        return nv + ns / self.hist_every if self.hist_every else nv + ns
+
+    def history_velocity_layout(self):
+        """(stride, {component: offset}) of the post-injection velocities in a
+        segment's history buffer: step a + j's component c is field
+        j * stride + offset[c]."""
+        nv, ns = self._hist_counts()
+        psv = bool(self.systems & _lib.PSV)
+        if self.hist_every:
+            return nv, {"u": 0, "w": 1, "v": 2 if psv else 0}
+        return nv + ns, {"u": 3, "w": 4, "v": 7 if psv else 2}
 
     # -- forward --------------------------------------------------------------
 
@@ -260,8 +307,13 @@ class ElasticFWI:
             self._call_batch(fn, descs, a, b, shape[1:])
             reset()
 
-        shape = autotune.tune(key, self.systems, trial,
-                              shapes=autotune.fwi_candidates(self.systems, name))
+        shapes = autotune.fwi_candidates(self.systems, name)
+        if name == "adjoint" and self.hist_every:
+            # the blocked adjoint takes as many steps per launch as the
+            # history's stress interval
+            shapes = [c for c in shapes if c[3] == self.hist_every]
+            key = key + (self.hist_every,)
+        shape = autotune.tune(key, self.systems, trial, shapes=shapes)
         self.tuned[name] = shape[1:]
         self.shapes[name] = shape[1:]
         reset()
@@ -292,17 +344,19 @@ class ElasticFWI:
             if self.blocked and g["ws"] is None:
                 g["ws"] = torch.empty(8 * self.nx * self.nz, dtype=self.dtype, device=dev)
         self.systems = self._systems()
+        self.hist_every = self._choose_hist_every()
         self.med = self._medium(vp, vs, rho)
         names = self._fields()
         nt_max = max(g["nt"] for g in self.geo)
-        n_hist = (5 if self.systems & _lib.PSV else 0) + (3 if self.systems & _lib.SH else 0)
-        self.L = self.checkpoint_every or checkpoint_interval(nt_max, len(names), n_hist)
+        self.L = self.checkpoint_every or checkpoint_interval(nt_max, len(names),
+                                                              self.history_fields_per_step())
         shape = (self.nx, self.nz)
         self.edges = list(range(0, nt_max, self.L)) + [nt_max]
         nseg = len(self.edges) - 1
         S = len(self.geo)
+        n_hist = self.history_fields(self.L)
         # trailing segments whose history the forward pass keeps
-        seg_bytes = self.L * n_hist * self.nx * self.nz * self.rb * S
+        seg_bytes = n_hist * self.nx * self.nz * self.rb * S
         self._kept = []  # an earlier forward's history is released first
         self._ck = None
         budget = self.history_bytes
@@ -315,7 +369,7 @@ class ElasticFWI:
         self.state = [{k: self.state_all[q, m] for m, k in enumerate(names)} for q in range(S)]
         self._ck = torch.empty((first_kept,) + tuple(self.state_all.shape), dtype=self.dtype,
                                device=dev)
-        self._kept = [torch.empty((S, self.L, n_hist) + shape, dtype=self.dtype, device=dev)
+        self._kept = [torch.empty((S, n_hist) + shape, dtype=self.dtype, device=dev)
                       for _ in range(self.kept_segments)]
         recs, descs = [], []
         for g, st in zip(self.geo, self.state):
@@ -374,10 +428,11 @@ class ElasticFWI:
             a, b = m64[k], r64[k]
             dif[k] = torch.where(a == b, torch.zeros_like(a), a - b).to(self.dtype).contiguous()
         del m64, r64
+        self.hist_every = self._choose_hist_every()
         names = self._fields()
         nt_max = max(g["nt"] for g in self.geo)
-        n_hist = (5 if self.systems & _lib.PSV else 0) + (3 if self.systems & _lib.SH else 0)
-        self.L = self.checkpoint_every or checkpoint_interval(nt_max, len(names), n_hist)
+        self.L = self.checkpoint_every or checkpoint_interval(nt_max, len(names),
+                                                              self.history_fields_per_step())
         shape = (self.nx, self.nz)
         self.edges = list(range(0, nt_max, self.L)) + [nt_max]
         nseg = len(self.edges) - 1
@@ -429,10 +484,10 @@ class ElasticFWI:
         shots in one batched call each; each
         shot accumulates its own coefficient-space gradient, summed in shot
         order at the end.  ``on_segment(a, b, hists)``, when given, is called
-        after the adjoint of steps [a, b) with each shot's history of those
-        steps, (b - a, fields, nx, nz) with the fields sxx, szz, sxz, u, w, syx,
-        syz, v of the active systems: the post-injection velocities there
-        times the sponge are the forward wavefield after each step."""
+        after the adjoint of steps [a, b) with each shot's history buffer of
+        those steps, (fields, nx, nz): the post-injection velocities there
+        (at the fields ``history_velocity_layout`` gives) times the sponge are
+        the forward wavefield after each step."""
         if self._ck is None:
             raise RuntimeError("call forward first")
         if self.kept_segments and not self._kept:
@@ -442,7 +497,7 @@ class ElasticFWI:
         names = self._fields()
         shape = (self.nx, self.nz)
         gnames = GRAD_NAMES
-        n_hist = (5 if self.systems & _lib.PSV else 0) + (3 if self.systems & _lib.SH else 0)
+        n_hist = self.history_fields(self.L)
         keep, descs = [], []
         S = len(self.geo)
         nseg = len(self.edges) - 1
@@ -450,7 +505,7 @@ class ElasticFWI:
         adj_all = torch.zeros((S, len(names)) + shape, dtype=self.dtype, device=dev)
         grad_all = torch.zeros((S, len(gnames)) + shape, dtype=self.dtype, device=dev)
         # recompute buffer, for the segments the forward pass did not keep
-        rebuf = (torch.empty((S, self.L, n_hist) + shape, dtype=self.dtype, device=dev)
+        rebuf = (torch.empty((S, n_hist) + shape, dtype=self.dtype, device=dev)
                  if first_kept > 0 else None)
         for q, (g, st, da) in enumerate(zip(self.geo, self.state, data_adjoint)):
             adj = {"p" + k: adj_all[q, m] for m, k in enumerate(names)}

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _lib.load().trtm_abi_version() == _lib.ABI_VERSION == 6
+    assert _lib.load().trtm_abi_version() == _lib.ABI_VERSION == 7
 
 
 @pytest.mark.parametrize("cname,pystruct", [("trtm_step_desc", _lib.StepDesc),

@@ -79,7 +79,8 @@ def test_salt_full_grid_within_budget():
     _case("budget salt 4000x2000x240, 2 shots")
     true, start, shots, obs, with_obs = _survey(4000, 2000, 240, 2, (1000, 3000))
     p = budget.plan(start.nx, start.nz, with_obs, 1 << 40)
-    b = p.shared_bytes + int(1.2 * max(p.shot_bytes_fwi, p.shot_bytes_rtm))
+    # sharing the forward history, a shot holds its FWI and RTM-backward memory
+    b = p.shared_bytes + int(1.2 * (p.shot_bytes_fwi + p.shot_bytes_rtm))
     job = budget.GradientAndImage(start.nx, start.nz, start.dx, start.dz, start.fs, start.fw,
                                   with_obs, b)
     assert job.plan.group == 1

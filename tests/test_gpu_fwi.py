@@ -373,9 +373,11 @@ def _uneven(shots, nts, spreads):
     return out
 
 
-def _records_and_gradient(grid, shots, m0, mt, **kw):
+def _records_and_gradient(grid, shots, m0, mt, adjoint_shape=None, **kw):
     kw.setdefault("history_bytes", 0)  # every segment recomputed unless asked otherwise
     p = ElasticFWI(*grid, shots, precision="float32", **kw)
+    if adjoint_shape is not None:
+        p.shapes["adjoint"] = adjoint_shape
     obs = [o.clone() for o in p.forward(*[_t(x, torch.float32) for x in mt])]
     recs = [r.clone() for r in p.forward(*[_t(x, torch.float32) for x in m0])]
     res = [r - o for r, o in zip(recs, obs)]
@@ -423,17 +425,21 @@ def test_blocked_kernels_bitwise_equal_step_kernels(system, shape):
     bit, for 3 shots of uneven record length and receiver spread, a checkpoint
     interval that is no multiple of K, and a source next to the grid edge."""
     _case("FWI blocked %s %s" % (system, shape))
-    if system == "both":
-        from taichirtm_amd import _lib, autotune
-        if ("fwi",) + tuple(shape) not in autotune.fwi_candidates(_lib.PSV | _lib.SH, "adjoint"):
-            pytest.skip("shape exceeds the LDS of a workgroup for both wave systems")
+    from taichirtm_amd import _lib, autotune
+    systems = {"psv": _lib.PSV, "sh": _lib.SH, "both": _lib.PSV | _lib.SH}[system]
+    if ("fwi",) + tuple(shape) not in autotune.fwi_candidates(systems, "forward"):
+        pytest.skip("shape exceeds the LDS of a workgroup for these wave systems")
+    # a shape whose adjoint (with its stress windows) exceeds the LDS runs the
+    # forward passes; the adjoint takes the library default
+    adj = None if ("fwi",) + tuple(shape) in autotune.fwi_candidates(systems, "adjoint") \
+        else (0, 0, 0)
     grid, shots, m0, mt = problem(system, nt=130, n_shots=3)
     shots = _uneven(shots, (130, 97, 121), ((0, 52), (5, 30), (20, 45)))
     shots[2] = ShotSpec(np.array([[1, 1]], np.int32), shots[2].wavelets, shots[2].rec_loc)
     tx, tz, k = shape
     ref = _records_and_gradient(grid, shots, m0, mt, kernel="step", checkpoint_every=29)
     got = _records_and_gradient(grid, shots, m0, mt, kernel="blocked", tile_x=tx, tile_z=tz,
-                                block_steps=k, checkpoint_every=29)
+                                block_steps=k, checkpoint_every=29, adjoint_shape=adj)
     for a, b in zip(got[0], ref[0]):
         assert torch.equal(a, b)
     for a, b in zip(got[1], ref[1]):
@@ -482,10 +488,15 @@ def test_blocked_inner_tiles_bitwise_equal_step_kernels(system, shape):
     shots = [ShotSpec(np.array([[50, 70]], np.int32), shots[0].wavelets, rec),
              ShotSpec(np.array([[150, 110]], np.int32), shots[1].wavelets, rec[10:40])]
     assert _inner_tiles(nx, nz, grid[5], shape, [(50, 70)] + [tuple(r) for r in rec]) > 0
+    from taichirtm_amd import _lib, autotune
+    systems = _lib.PSV if system == "psv" else _lib.SH
+    # a shape whose adjoint exceeds the LDS runs the forward passes only
+    adj = None if ("fwi",) + tuple(shape) in autotune.fwi_candidates(systems, "adjoint") \
+        else (0, 0, 0)
     tx, tz, k = shape
     ref = _records_and_gradient(grid, shots, m0, mt, kernel="step", checkpoint_every=60)
     got = _records_and_gradient(grid, shots, m0, mt, kernel="blocked", tile_x=tx, tile_z=tz,
-                                block_steps=k, checkpoint_every=60)
+                                block_steps=k, checkpoint_every=60, adjoint_shape=adj)
     assert max(float(x.abs().max()) for x in ref[1]) > 0
     for a, b in zip(got[0], ref[0]):
         assert torch.equal(a, b)
@@ -537,7 +548,11 @@ def test_history_kept_by_forward_bitwise(kernel):
     shots[2] = ShotSpec(np.array([[45, 2]], np.int32), shots[2].wavelets, shots[2].rec_loc)
     nx, nz = grid[0], grid[1]
     ref = _records_and_gradient(grid, shots, m0, mt, kernel=kernel, checkpoint_every=40)
-    seg = 40 * 5 * nx * nz * 4 * 3
+    # history of one 40-step segment of the 3 shots: every step's 5 fields
+    # (one step per launch), or the blocked kernels' compact layout (2
+    # velocities per step, 3 stresses per 4 steps)
+    fields = 40 * 5 if kernel == "step" else 40 * 2 + (40 // 4) * 3
+    seg = fields * nx * nz * 4 * 3
     for kept, nbytes in ((4, 10 * seg), (2, 2 * seg + seg // 2)):
         p = ElasticFWI(*grid, shots, precision="float32", kernel=kernel, checkpoint_every=40,
                        history_bytes=nbytes)
