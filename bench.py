@@ -41,6 +41,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -419,8 +420,10 @@ def job_main(args):
         else:
             # sqrt(N) checkpoints, every segment recomputed; the residual
             # against the true model's data propagated as a scattered field
+            nt_max = max(s.nt for s in shots)
             prob = ElasticFWI(cfg.nx, cfg.nz, cfg.dx, cfg.dz, cfg.fs, cfg.fw, shots,
-                              precision="float32", kernel=fwi_kernel, history_bytes=0)
+                              precision="float32", kernel=fwi_kernel, history_bytes=0,
+                              checkpoint_every=int(round(math.sqrt(nt_max))))
             run = lambda: l2_misfit_and_gradient(prob, *m0, reference=mt)
             how = ("checkpoints every sqrt(N) steps and every segment recomputed; residual "
                    "against the true model's data as a float32 scattered field")

@@ -115,6 +115,29 @@ __device__ __forceinline__ void lds_barrier() {
 }
 // wait for every vector-memory load of this wave (vmcnt 0)
 __device__ __forceinline__ void vm_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// wait until at most N vector-memory loads of this wave are in flight
+// (vmcnt holds 6 bits: a larger N waits for 63, which only waits longer)
+template <int N>
+__device__ __forceinline__ void vm_wait_n() {
+    constexpr int M = N < 63 ? N : 63;
+    __builtin_amdgcn_s_waitcnt(0x0F70 | (M & 15) | ((M >> 4) << 14));
+}
+// wait until at most n * Q loads are in flight (n = 0 .. 8; n is a constant
+// once the caller's loop is unrolled)
+template <int Q>
+__device__ __forceinline__ void vm_wait_windows(int n) {
+    switch (n) {
+    case 0: vm_wait_n<0>(); break;
+    case 1: vm_wait_n<Q>(); break;
+    case 2: vm_wait_n<2 * Q>(); break;
+    case 3: vm_wait_n<3 * Q>(); break;
+    case 4: vm_wait_n<4 * Q>(); break;
+    case 5: vm_wait_n<5 * Q>(); break;
+    case 6: vm_wait_n<6 * Q>(); break;
+    case 7: vm_wait_n<7 * Q>(); break;
+    default: vm_wait_n<8 * Q>(); break;
+    }
+}
 
 // Adjoint history window of a tile: rows i0-1 .. i0+TX, columns j0-1 .. j0+TZ,
 // in 64-cell chunks per field.  The velocity windows come in two buffers (the
@@ -565,15 +588,21 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
     unsigned voff[QPV];  // byte offset from the step's velocity slot
 #pragma unroll
     for (int m = 0; m < QPV; m++) voff[m] = win_off(wave + NW * m, NQV);
+    // A wave with fewer than QPV chunks of a window repeats its previous
+    // chunk (the same bytes to the same place), so every wave issues exactly
+    // QPV loads per window and can wait for a window by count.
+    static_assert(NQV >= NW, "every wave copies a chunk of each velocity window");
     auto fetch_v = [&](int it, float* dst) {
         const char* base = (const char*)(a.hist + (size_t)(it - a.hist_base) * (HV::nv * n));
 #pragma unroll
         for (int m = 0; m < QPV; m++) {
             const int q = wave + NW * m;  // wave-uniform
-            if (q < NQV)
-                __builtin_amdgcn_global_load_lds(
-                    (const void*)(base + voff[m]),
-                    (__attribute__((address_space(3))) void*)&dst[q * 64], 4, 0, 0);
+            const bool own = m == 0 || q < NQV;
+            const unsigned off = own ? voff[m] : voff[m > 0 ? m - 1 : 0];
+            const int qd = own ? q : q - NW;
+            __builtin_amdgcn_global_load_lds(
+                (const void*)(base + off),
+                (__attribute__((address_space(3))) void*)&dst[qd * 64], 4, 0, 0);
         }
     };
     auto fetch_s = [&](float* dst) {  // the stored stresses of step alpha
@@ -595,12 +624,14 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
         const int r = e / RZ, z = e - r * RZ;
         wo[k] = (r - K + 1) * W::HZ + (z - K + 1);
     }
-    // velocity window of step s (the step taken first, top, uses hwA)
-    auto vwin = [&](int s) { return ((top - s) & 1) ? hwB : hwA; };
-
     // ---- pre-step stresses of steps alpha + 1 .. top over the stress window:
     // the forward stress update and sponge (k_fwi_fwd_blk) from the stored
-    // stresses of step alpha and the velocities of steps alpha .. top - 1
+    // stresses of step alpha and the velocities of steps alpha .. top - 1.
+    // Every velocity window of the launch is requested first: v(alpha + r),
+    // r < top - 1 - alpha, parked in stress slot r + 1 (read by recomputation
+    // step r, overwritten by step r + 1), v(top - 1) in hwB and v(top) in hwA,
+    // where the first two adjoint steps read them.  Requests past top - 1
+    // repeat v(top), so every wave issues K windows and waits by count.
     if (top > alpha) {
         float rxx[NC], rzz[NC], rxz[NC], ryx[NC], ryz[NC];
 #pragma unroll
@@ -618,15 +649,20 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                 ryz[k] = ld(hstr + HV::syz * n, c);
             }
         }
-        fetch_v(alpha, vwin(alpha));
 #pragma unroll
         for (int r = 0; r < K - 1; r++) {
-            const int s = alpha + r;  // stresses of step s + 1 from step s
-            if (s >= top) break;
-            const float* const buf = vwin(s);
-            vm_wait();
-            lds_barrier();
-            fetch_v(s + 1, vwin(s + 1));  // its window was last read a step ago
+            const int s = alpha + r;
+            if (s < top - 1)
+                fetch_v(s, hsw + (r + 1) * W::sfloats);
+            else if (s == top - 1)
+                fetch_v(s, hwB);
+            else
+                fetch_v(top, hwA);
+        }
+        fetch_v(top, hwA);
+        vm_wait_windows<QPV>(K);  // the loads before the windows
+        // stresses of step s + 1 = alpha + r + 1 from step s, into slot r
+        auto recompute = [&](int r, const float* buf) {
             float* const dst = hsw + r * W::sfloats;
 #pragma unroll
             for (int k = 0; k < NC; k++) {
@@ -667,17 +703,36 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
                     dst[HV::syz * FW_ + w] = ryz[k];
                 }
             }
+        };
+#pragma unroll
+        for (int r = 0; r < K - 1; r++) {
+            const int s = alpha + r;
+            if (s >= top) break;
+            vm_wait_windows<QPV>(K - 1 - r);  // window r arrived
+            lds_barrier();
+            if (s == top - 1)
+                recompute(r, hwB);
+            else
+                recompute(r, hsw + (r + 1) * W::sfloats);
         }
     } else {
-        fetch_v(top, vwin(top));
+        fetch_v(top, hwA);
         if (t_a == alpha) fetch_s(hsw + W::sfloats);
     }
+    // the stored stresses of step alpha for its adjoint step (when the launch
+    // takes it after recomputed ones): loaded at step min(top, alpha + 2),
+    // written to slot 1 at step alpha, once slot 1's last reader is done
+    const bool late_alpha = t_a == alpha && top > alpha;
+    const int alpha_load = top < alpha + 2 ? top : alpha + 2;
+    float mxx[NC], mzz[NC], mxz[NC], myx_[NC], myz_[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) mxx[k] = mzz[k] = mxz[k] = myx_[k] = myz_[k] = 0.0f;
 
 #pragma unroll
     for (int s = 0; s < K; s++) {
         const int it = top - s;
         if (it < t_a) break;
-        const float* const buf = (s & 1) ? hwB : hwA;  // = vwin(it)
+        const float* const buf = (s & 1) ? hwB : hwA;  // v(top - s)
         float* const nxt = (s & 1) ? hwA : hwB;
         const float* const sw = hsw + (it == alpha ? 1 : it - alpha - 1) * W::sfloats;
 
@@ -715,8 +770,39 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
         // the next step's velocity window (the one of top - 1 came with the
         // recomputation); the window it replaces was last read a step ago
         if (s > 0 && it - 1 >= t_a) fetch_v(it - 1, nxt);
-        // the stored stresses of step alpha, into slot 1 (last read by step alpha + 2)
-        if (t_a == alpha && it == alpha + 1) fetch_s(hsw + W::sfloats);
+        if (late_alpha && it == alpha_load) {
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                if (!(fl[k] & 32u)) continue;
+                const int c = gc[k];
+                if (PSV) {
+                    mxx[k] = ld(hstr + HV::sxx * n, c);
+                    mzz[k] = ld(hstr + HV::szz * n, c);
+                    mxz[k] = ld(hstr + HV::sxz * n, c);
+                }
+                if (SH) {
+                    myx_[k] = ld(hstr + HV::syx * n, c);
+                    myz_[k] = ld(hstr + HV::syz * n, c);
+                }
+            }
+        }
+        if (late_alpha && it == alpha) {
+            float* const dst = hsw + W::sfloats;
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                if (!(fl[k] & 32u)) continue;
+                const int w = wo[k];
+                if (PSV) {
+                    dst[HV::sxx * FW_ + w] = mxx[k];
+                    dst[HV::szz * FW_ + w] = mzz[k];
+                    dst[HV::sxz * FW_ + w] = mxz[k];
+                }
+                if (SH) {
+                    dst[HV::syx * FW_ + w] = myx_[k];
+                    dst[HV::syz * FW_ + w] = myz_[k];
+                }
+            }
+        }
 
         // ---- phase 1b: new velocity adjoint; lam / mu / mxz (myx / myz) and
         // source-term gradients of the tile; b = dt * inv_rho * p for phase 2
