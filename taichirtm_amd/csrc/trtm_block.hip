@@ -158,8 +158,6 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
     const float sdt = FWD ? m.dt : -m.dt;
     const float idx = m.idx, idz = m.idz, dt = m.dt;
     const bool has_sm = m.sm != nullptr;
-    // stress read with the free-surface / topography mask of the cell
-    auto msk = [&](float x, int c) { return has_sm ? x * sm[c] : x; };
     const int save = sh->save, isnap = sh->isnap, num_snaps = sh->num_snaps;
     const int snap_base = sh->snap_base, cint = sh->check_interval;
     const int n_groups = sh->n_groups, n_rec = sh->n_rec;
@@ -410,7 +408,14 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                 sV[e] = v;
             }
         };
-        if constexpr (PSV) {  // per-cell branches, the original loop
+        // P-SV: per-cell branches; the surface matrix is a template choice,
+        // so the stress reads of the update issue together
+        auto velocity_psv = [&](auto sm_tag) {
+            constexpr bool SM = decltype(sm_tag)::value;
+            auto msk = [&](float x, int c) {
+                if constexpr (SM) return x * sm[c];
+                else return x;
+            };
 #pragma unroll
             for (int k = 0; k < NC; k++) {
                 if (!(fl[k] & 1u)) continue;
@@ -464,6 +469,12 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                     sV[e] = v;
                 }
             }
+        };
+        if constexpr (PSV) {
+            if (has_sm)
+                velocity_psv(std::true_type{});
+            else
+                velocity_psv(std::false_type{});
         } else if (has_sm) {
             velocity_phase(std::true_type{});
         } else {
@@ -553,7 +564,12 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                 }
             }
         };
-        if constexpr (PSV) {  // per-cell branches, the original loop
+        auto stress_psv = [&](auto sm_tag) {
+            constexpr bool SM = decltype(sm_tag)::value;
+            auto msk = [&](float x, int c) {
+                if constexpr (SM) return x * sm[c];
+                else return x;
+            };
 #pragma unroll
             for (int k = 0; k < NC; k++) {
                 if (!(fl[k] & 1u)) continue;
@@ -639,6 +655,12 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                     }
                 }
             }
+        };
+        if constexpr (PSV) {
+            if (has_sm)
+                stress_psv(std::true_type{});
+            else
+                stress_psv(std::false_type{});
         } else if (has_sm) {
             stress_phase(std::true_type{});
         } else {
