@@ -36,6 +36,8 @@ __global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict
                                               int tiles_x, int tiles_z) {
     constexpr int RX = TX + 2 * K, RZ = TZ + 2 * K, RN = RX * RZ;
     constexpr int NC = (RN + NT - 1) / NT;
+    static_assert(2 * K <= TX && 2 * K <= TZ,
+                  "the per-tile injection lists hold a cell in at most 2 x 2 regions");
     // Region arrays with RZ + 1 cells of padding on each side and room for
     // every thread slot: without a surface matrix the updates run branch-free
     // on every slot (neighbour reads of edge and spare slots stay inside the
@@ -746,7 +748,8 @@ static const BlockCfg kCfgBoth[] = {{16, 64, 256, 2}, {16, 64, 256, 4}, {16, 128
                                     {16, 32, 1024, 4}};
 // (the 16 x 32 and 16 x 16 tiles with 1024 threads give every thread one or
 // two region cells: the shortest step on grids too small to fill the GPU; 16
-// x 16 with 576 threads and K = 4, and 8 x 16 with 384 threads and K = 4, were
+// x 16 with 576 threads and K = 4, 8 x 16 with 384 threads and K = 4, and 16 x
+// 16 with K = 8 and 512 or 256 threads (two or four cells per thread) were
 // measured slower on the 200 x 200 grid)
 
 static void cfg_list(int systems, const BlockCfg** list, int* n) {
