@@ -309,7 +309,8 @@ typedef struct trtm_fwi_desc {
      * u, w) + 1 (SH: v), followed by the pre-step stresses of the steps
      * hist_base + m K, REAL (ceil(hist_len / K), n_str, nx, nz), n_str = 3 (P-SV:
      * sxx, szz, sxz) + 2 (SH: syx, syz); read by the blocked adjoint, whose
-     * block_steps must equal K (it recomputes the other steps' stresses). */
+     * block_steps B must give min(B, 4) = K (it recomputes the other steps'
+     * stresses). */
     int32_t hist_len, hist_every;
     void *hist;
     /* gradient accumulators in coefficient space, REAL (nx, nz), += */

@@ -152,14 +152,18 @@ struct HistWindow {
     static constexpr int vfloats = HC<PSV, SH>::nv * field;  // one velocity window
     static constexpr int sfloats = HC<PSV, SH>::ns * field;  // one stress window
 };
-template <int K>
-constexpr int adj_stress_slots() { return K - 1 > 2 ? K - 1 : 2; }
+// steps between stored stresses read by the adjoint of K steps per launch (a
+// launch of 8 steps spans two such intervals)
+constexpr int adj_stress_every(int K) { return K < 4 ? K : 4; }
+template <int KS>
+constexpr int adj_stress_slots() { return KS - 1 > 2 ? KS - 1 : 2; }
 
 template <bool PSV, bool SH, int TX, int TZ, int K>
 constexpr size_t adj_lds_bytes() {
     using W = HistWindow<PSV, SH, TX, TZ>;
     constexpr int RN = (TX + 2 * K) * (TZ + 2 * K);
-    return 4 * (2 * (size_t)W::vfloats + (size_t)adj_stress_slots<K>() * W::sfloats +
+    return 4 * (2 * (size_t)W::vfloats +
+                (size_t)adj_stress_slots<adj_stress_every(K)>() * W::sfloats +
                 (size_t)((PSV ? 5 : 0) + (SH ? 3 : 0)) * RN);
 }
 constexpr size_t kLdsLimit = 160 * 1024;
@@ -442,9 +446,9 @@ __global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int
 }
 
 // ---------------------------------------------------------------------------
-// adjoint, steps t_b-1 down to t_a (t_b - t_a <= K, all inside one interval of
-// K steps from hist_base); steps at or beyond the shot's record length leave
-// its state unchanged
+// adjoint, steps t_b-1 down to t_a (t_b - t_a <= K, inside K / KS intervals of
+// KS steps from hist_base, KS = min(K, 4)), interval by interval from the top;
+// steps at or beyond the shot's record length leave its state unchanged
 
 template <bool PSV, bool SH, int TX, int TZ, int NT, int K, class S>
 __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int tiles_x,
@@ -454,7 +458,8 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
     constexpr int NP = PSV ? RN : 1, NS = SH ? RN : 1;
     using W = HistWindow<PSV, SH, TX, TZ>;
     using HV = HC<PSV, SH>;
-    constexpr int NSL = adj_stress_slots<K>();
+    constexpr int KS = adj_stress_every(K);
+    constexpr int NSL = adj_stress_slots<KS>();
     constexpr int FW_ = W::field;
     static_assert(adj_lds_bytes<PSV, SH, TX, TZ, K>() <= kLdsLimit, "LDS");
     __shared__ float qX[NP], qZ[NP], qS[NP], bU[NP], bW[NP];
@@ -478,12 +483,6 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
     const bool has_src = a.src_map != nullptr, has_rec = a.rec_map != nullptr;
     float* const* in = a.flip ? a.alt : a.st;
     const size_t n = (size_t)nx * nz;
-    // the launch's steps t_a .. top lie in the interval of K steps from alpha,
-    // whose pre-step stresses are stored
-    const int alpha = a.hist_base + ((t_a - a.hist_base) / K) * K;
-    const int top = (t_b < nt ? t_b : nt) - 1;
-    const float* const hstr =
-        a.hist + ((size_t)a.hist_len * HV::nv + (size_t)((alpha - a.hist_base) / K) * HV::ns) * n;
 
     // The gradient and history loads of a region cell outside the tile (or
     // off the interior) read a nearby tile cell (an interior one) instead:
@@ -585,27 +584,26 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
         const int i = clampi(i0 - 1 + r, 0, nx - 1), j = clampi(j0 - 1 + z, 0, nz - 1);
         return (unsigned)((q < nq ? f : 0) * (int)n + i * nz + j) * 4u;
     };
+    // A wave's m-th load of a velocity window copies chunk q = wave + NW m,
+    // or, past the last chunk, chunk q mod NQV again (the same bytes to the
+    // same place), so every wave issues exactly QPV loads per window and can
+    // wait for a window by count.
+    auto vchunk = [&](int m) {
+        const int q = wave + NW * m;  // wave-uniform
+        return q < NQV ? q : q % NQV;
+    };
     unsigned voff[QPV];  // byte offset from the step's velocity slot
 #pragma unroll
-    for (int m = 0; m < QPV; m++) voff[m] = win_off(wave + NW * m, NQV);
-    // A wave with fewer than QPV chunks of a window repeats its previous
-    // chunk (the same bytes to the same place), so every wave issues exactly
-    // QPV loads per window and can wait for a window by count.
-    static_assert(NQV >= NW, "every wave copies a chunk of each velocity window");
+    for (int m = 0; m < QPV; m++) voff[m] = win_off(vchunk(m), NQV);
     auto fetch_v = [&](int it, float* dst) {
         const char* base = (const char*)(a.hist + (size_t)(it - a.hist_base) * (HV::nv * n));
 #pragma unroll
-        for (int m = 0; m < QPV; m++) {
-            const int q = wave + NW * m;  // wave-uniform
-            const bool own = m == 0 || q < NQV;
-            const unsigned off = own ? voff[m] : voff[m > 0 ? m - 1 : 0];
-            const int qd = own ? q : q - NW;
+        for (int m = 0; m < QPV; m++)
             __builtin_amdgcn_global_load_lds(
-                (const void*)(base + off),
-                (__attribute__((address_space(3))) void*)&dst[qd * 64], 4, 0, 0);
-        }
+                (const void*)(base + voff[m]),
+                (__attribute__((address_space(3))) void*)&dst[vchunk(m) * 64], 4, 0, 0);
     };
-    auto fetch_s = [&](float* dst) {  // the stored stresses of step alpha
+    auto fetch_s = [&](const float* hstr, float* dst) {  // stored stresses of an interval
         const char* base = (const char*)hstr;
 #pragma unroll
         for (int m = 0; m < QPS; m++) {
@@ -624,286 +622,302 @@ __global__ __launch_bounds__(NT) void k_fwi_adj_blk(S src, int t_a, int t_b, int
         const int r = e / RZ, z = e - r * RZ;
         wo[k] = (r - K + 1) * W::HZ + (z - K + 1);
     }
-    // ---- pre-step stresses of steps alpha + 1 .. top over the stress window:
-    // the forward stress update and sponge (k_fwi_fwd_blk) from the stored
-    // stresses of step alpha and the velocities of steps alpha .. top - 1.
-    // Every velocity window of the launch is requested first: v(alpha + r),
-    // r < top - 1 - alpha, parked in stress slot r + 1 (read by recomputation
-    // step r, overwritten by step r + 1), v(top - 1) in hwB and v(top) in hwA,
-    // where the first two adjoint steps read them.  Requests past top - 1
-    // repeat v(top), so every wave issues K windows and waits by count.
-    if (top > alpha) {
-        float rxx[NC], rzz[NC], rxz[NC], ryx[NC], ryz[NC];
+    // Intervals from the top: interval iv takes steps top_i down to t_lo,
+    // its pre-step stresses stored at step alpha_i.
+    int top_i = (t_b < nt ? t_b : nt) - 1;
 #pragma unroll
-        for (int k = 0; k < NC; k++) {
-            rxx[k] = rzz[k] = rxz[k] = ryx[k] = ryz[k] = 0.0f;
-            if (!(fl[k] & 32u)) continue;
-            const int c = gc[k];
-            if (PSV) {
-                rxx[k] = ld(hstr + HV::sxx * n, c);
-                rzz[k] = ld(hstr + HV::szz * n, c);
-                rxz[k] = ld(hstr + HV::sxz * n, c);
-            }
-            if (SH) {
-                ryx[k] = ld(hstr + HV::syx * n, c);
-                ryz[k] = ld(hstr + HV::syz * n, c);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < K - 1; r++) {
-            const int s = alpha + r;
-            if (s < top - 1)
-                fetch_v(s, hsw + (r + 1) * W::sfloats);
-            else if (s == top - 1)
-                fetch_v(s, hwB);
-            else
-                fetch_v(top, hwA);
-        }
-        fetch_v(top, hwA);
-        vm_wait_windows<QPV>(K);  // the loads before the windows
-        // stresses of step s + 1 = alpha + r + 1 from step s, into slot r
-        auto recompute = [&](int r, const float* buf) {
-            float* const dst = hsw + r * W::sfloats;
+    for (int iv = 0; iv < K / KS; iv++) {
+        if (top_i < t_a) break;
+        const int alpha_i = a.hist_base + ((top_i - a.hist_base) / KS) * KS;
+        const int t_lo = alpha_i > t_a ? alpha_i : t_a;
+        const float* const hstr =
+            a.hist + ((size_t)a.hist_len * HV::nv +
+                      (size_t)((alpha_i - a.hist_base) / KS) * HV::ns) * n;
+        // the previous interval's last step read the windows and slots reused here
+        if (iv > 0) lds_barrier();
+        // ---- pre-step stresses of steps alpha_i + 1 .. top_i over the stress
+        // window: the forward stress update and sponge (k_fwi_fwd_blk) from the
+        // stored stresses of step alpha_i and the velocities of steps alpha_i ..
+        // top_i - 1.  Every velocity window of the interval is requested first:
+        // v(alpha_i + r), r < top_i - 1 - alpha_i, parked in stress slot r + 1
+        // (read by recomputation step r, overwritten by step r + 1), v(top_i - 1)
+        // in hwB and v(top_i) in hwA, where the first two adjoint steps read
+        // them.  Requests past top_i - 1 repeat v(top_i), so every wave issues
+        // KS windows and waits by count.
+        if (top_i > alpha_i) {
+            float rxx[NC], rzz[NC], rxz[NC], ryx[NC], ryz[NC];
 #pragma unroll
             for (int k = 0; k < NC; k++) {
-                if (!(fl[k] & 32u)) continue;
-                const int w = wo[k];
-                if (PSV) {
-                    float sxx = rxx[k], szz = rzz[k], sxz = rxz[k];
-                    if (fl[k] & 2u) {
-                        const float u0 = buf[HV::u * FW_ + w], w0 = buf[HV::w * FW_ + w];
-                        const float u_x = (buf[HV::u * FW_ + w + W::HZ] - u0) * idx;
-                        const float u_z = (buf[HV::u * FW_ + w + 1] - u0) * idz;
-                        const float w_x = (buf[HV::w * FW_ + w + W::HZ] - w0) * idx;
-                        const float w_z = (buf[HV::w * FW_ + w + 1] - w0) * idz;
-                        const float div_uw = u_x + w_z;
-                        sxx = sxx + dt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
-                        szz = szz + dt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
-                        sxz = sxz + (dt * c4[k]) * (u_z + w_x);
-                    }
-                    rxx[k] = sxx * A[k];
-                    rzz[k] = szz * A[k];
-                    rxz[k] = sxz * A[k];
-                    dst[HV::sxx * FW_ + w] = rxx[k];
-                    dst[HV::szz * FW_ + w] = rzz[k];
-                    dst[HV::sxz * FW_ + w] = rxz[k];
-                }
-                if (SH) {
-                    float syx = ryx[k], syz = ryz[k];
-                    if (fl[k] & 2u) {
-                        const float v0 = buf[HV::v * FW_ + w];
-                        const float v_x = (buf[HV::v * FW_ + w + W::HZ] - v0) * idx;
-                        const float v_z = (buf[HV::v * FW_ + w + 1] - v0) * idz;
-                        syx = syx + (dt * d1[k]) * v_x;
-                        syz = syz + (dt * d2[k]) * v_z;
-                    }
-                    ryx[k] = syx * A[k];
-                    ryz[k] = syz * A[k];
-                    dst[HV::syx * FW_ + w] = ryx[k];
-                    dst[HV::syz * FW_ + w] = ryz[k];
-                }
-            }
-        };
-#pragma unroll
-        for (int r = 0; r < K - 1; r++) {
-            const int s = alpha + r;
-            if (s >= top) break;
-            vm_wait_windows<QPV>(K - 1 - r);  // window r arrived
-            lds_barrier();
-            if (s == top - 1)
-                recompute(r, hwB);
-            else
-                recompute(r, hsw + (r + 1) * W::sfloats);
-        }
-    } else {
-        fetch_v(top, hwA);
-        if (t_a == alpha) fetch_s(hsw + W::sfloats);
-    }
-    // the stored stresses of step alpha for its adjoint step (when the launch
-    // takes it after recomputed ones): loaded at step min(top, alpha + 2),
-    // written to slot 1 at step alpha, once slot 1's last reader is done
-    const bool late_alpha = t_a == alpha && top > alpha;
-    const int alpha_load = top < alpha + 2 ? top : alpha + 2;
-    float mxx[NC], mzz[NC], mxz[NC], myx_[NC], myz_[NC];
-#pragma unroll
-    for (int k = 0; k < NC; k++) mxx[k] = mzz[k] = mxz[k] = myx_[k] = myz_[k] = 0.0f;
-
-#pragma unroll
-    for (int s = 0; s < K; s++) {
-        const int it = top - s;
-        if (it < t_a) break;
-        const float* const buf = (s & 1) ? hwB : hwA;  // v(top - s)
-        float* const nxt = (s & 1) ? hwA : hwB;
-        const float* const sw = hsw + (it == alpha ? 1 : it - alpha - 1) * W::sfloats;
-
-        // ---- phase 1a: data adjoint at the receivers; stress transposes q
-        // of every region cell from its own sponged stress adjoint
-#pragma unroll
-        for (int k = 0; k < NC; k++) {
-            if (!(fl[k] & 1u)) continue;
-            const int e = t + NT * k;
-            if (rg[k] >= 0) {
-                const size_t row = (size_t)it * a.n_rec;
-                for (int q = rstart[rg[k]]; q < rstart[rg[k] + 1]; q++) {
-                    const int l = rorder[q];
-                    if (PSV) {
-                        pu[k] += res_u[row + l];
-                        pw[k] += res_w[row + l];
-                    }
-                    if (SH) pv[k] += res_v[row + l];
-                }
-            }
-            const bool in_ = fl[k] & 2u;
-            if (PSV) {
-                const float axx = dt * (A[k] * pxx[k]), azz = dt * (A[k] * pzz[k]);
-                qX[e] = in_ ? (c2[k] + 2.0f * c3[k]) * axx + c2[k] * azz : 0.0f;
-                qZ[e] = in_ ? c2[k] * axx + (c2[k] + 2.0f * c3[k]) * azz : 0.0f;
-                qS[e] = in_ ? (c4[k] * dt) * (A[k] * pxz[k]) : 0.0f;
-            }
-            if (SH) {
-                qY[e] = in_ ? (d1[k] * dt) * (A[k] * pyx[k]) : 0.0f;
-                qV[e] = in_ ? (d2[k] * dt) * (A[k] * pyz[k]) : 0.0f;
-            }
-        }
-        vm_wait();  // this step's windows (and the residuals) arrived
-        lds_barrier();
-        // the next step's velocity window (the one of top - 1 came with the
-        // recomputation); the window it replaces was last read a step ago
-        if (s > 0 && it - 1 >= t_a) fetch_v(it - 1, nxt);
-        if (late_alpha && it == alpha_load) {
-#pragma unroll
-            for (int k = 0; k < NC; k++) {
+                rxx[k] = rzz[k] = rxz[k] = ryx[k] = ryz[k] = 0.0f;
                 if (!(fl[k] & 32u)) continue;
                 const int c = gc[k];
                 if (PSV) {
-                    mxx[k] = ld(hstr + HV::sxx * n, c);
-                    mzz[k] = ld(hstr + HV::szz * n, c);
-                    mxz[k] = ld(hstr + HV::sxz * n, c);
+                    rxx[k] = ld(hstr + HV::sxx * n, c);
+                    rzz[k] = ld(hstr + HV::szz * n, c);
+                    rxz[k] = ld(hstr + HV::sxz * n, c);
                 }
                 if (SH) {
-                    myx_[k] = ld(hstr + HV::syx * n, c);
-                    myz_[k] = ld(hstr + HV::syz * n, c);
+                    ryx[k] = ld(hstr + HV::syx * n, c);
+                    ryz[k] = ld(hstr + HV::syz * n, c);
                 }
             }
+#pragma unroll
+            for (int r = 0; r < KS - 1; r++) {
+                const int s = alpha_i + r;
+                if (s < top_i - 1)
+                    fetch_v(s, hsw + (r + 1) * W::sfloats);
+                else if (s == top_i - 1)
+                    fetch_v(s, hwB);
+                else
+                    fetch_v(top_i, hwA);
+            }
+            fetch_v(top_i, hwA);
+            vm_wait_windows<QPV>(KS);  // the loads before the windows
+            // stresses of step s + 1 = alpha_i + r + 1 from step s, into slot r
+            auto recompute = [&](int r, const float* buf) {
+                float* const dst = hsw + r * W::sfloats;
+#pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    if (!(fl[k] & 32u)) continue;
+                    const int w = wo[k];
+                    if (PSV) {
+                        float sxx = rxx[k], szz = rzz[k], sxz = rxz[k];
+                        if (fl[k] & 2u) {
+                            const float u0 = buf[HV::u * FW_ + w], w0 = buf[HV::w * FW_ + w];
+                            const float u_x = (buf[HV::u * FW_ + w + W::HZ] - u0) * idx;
+                            const float u_z = (buf[HV::u * FW_ + w + 1] - u0) * idz;
+                            const float w_x = (buf[HV::w * FW_ + w + W::HZ] - w0) * idx;
+                            const float w_z = (buf[HV::w * FW_ + w + 1] - w0) * idz;
+                            const float div_uw = u_x + w_z;
+                            sxx = sxx + dt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
+                            szz = szz + dt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
+                            sxz = sxz + (dt * c4[k]) * (u_z + w_x);
+                        }
+                        rxx[k] = sxx * A[k];
+                        rzz[k] = szz * A[k];
+                        rxz[k] = sxz * A[k];
+                        dst[HV::sxx * FW_ + w] = rxx[k];
+                        dst[HV::szz * FW_ + w] = rzz[k];
+                        dst[HV::sxz * FW_ + w] = rxz[k];
+                    }
+                    if (SH) {
+                        float syx = ryx[k], syz = ryz[k];
+                        if (fl[k] & 2u) {
+                            const float v0 = buf[HV::v * FW_ + w];
+                            const float v_x = (buf[HV::v * FW_ + w + W::HZ] - v0) * idx;
+                            const float v_z = (buf[HV::v * FW_ + w + 1] - v0) * idz;
+                            syx = syx + (dt * d1[k]) * v_x;
+                            syz = syz + (dt * d2[k]) * v_z;
+                        }
+                        ryx[k] = syx * A[k];
+                        ryz[k] = syz * A[k];
+                        dst[HV::syx * FW_ + w] = ryx[k];
+                        dst[HV::syz * FW_ + w] = ryz[k];
+                    }
+                }
+            };
+#pragma unroll
+            for (int r = 0; r < KS - 1; r++) {
+                const int s = alpha_i + r;
+                if (s >= top_i) break;
+                vm_wait_windows<QPV>(KS - 1 - r);  // window r arrived
+                lds_barrier();
+                if (s == top_i - 1)
+                    recompute(r, hwB);
+                else
+                    recompute(r, hsw + (r + 1) * W::sfloats);
+            }
+        } else {
+            fetch_v(top_i, hwA);
+            if (t_lo == alpha_i) fetch_s(hstr, hsw + W::sfloats);
         }
-        if (late_alpha && it == alpha) {
-            float* const dst = hsw + W::sfloats;
+        // the stored stresses of step alpha_i for its adjoint step (when the launch
+        // takes it after recomputed ones): loaded at step min(top_i, alpha_i + 2),
+        // written to slot 1 at step alpha_i, once slot 1's last reader is done
+        const bool late_alpha = t_lo == alpha_i && top_i > alpha_i;
+        const int alpha_load = top_i < alpha_i + 2 ? top_i : alpha_i + 2;
+        float mxx[NC], mzz[NC], mxz[NC], myx_[NC], myz_[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) mxx[k] = mzz[k] = mxz[k] = myx_[k] = myz_[k] = 0.0f;
+
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int it = top_i - s;
+            if (it < t_lo) break;
+            const float* const buf = (s & 1) ? hwB : hwA;  // v(top_i - s)
+            float* const nxt = (s & 1) ? hwA : hwB;
+            const float* const sw = hsw + (it == alpha_i ? 1 : it - alpha_i - 1) * W::sfloats;
+
+            // ---- phase 1a: data adjoint at the receivers; stress transposes q
+            // of every region cell from its own sponged stress adjoint
 #pragma unroll
             for (int k = 0; k < NC; k++) {
-                if (!(fl[k] & 32u)) continue;
-                const int w = wo[k];
-                if (PSV) {
-                    dst[HV::sxx * FW_ + w] = mxx[k];
-                    dst[HV::szz * FW_ + w] = mzz[k];
-                    dst[HV::sxz * FW_ + w] = mxz[k];
-                }
-                if (SH) {
-                    dst[HV::syx * FW_ + w] = myx_[k];
-                    dst[HV::syz * FW_ + w] = myz_[k];
-                }
-            }
-        }
-
-        // ---- phase 1b: new velocity adjoint; lam / mu / mxz (myx / myz) and
-        // source-term gradients of the tile; b = dt * inv_rho * p for phase 2
-#pragma unroll
-        for (int k = 0; k < NC; k++) {
-            if (!(fl[k] & 1u)) continue;
-            const int e = t + NT * k;
-            const bool in_ = fl[k] & 2u;
-            const bool own_in = (fl[k] & 6u) == 6u;
-            if (fl[k] & 8u) {
-                if (PSV) {
-                    const float nu =
-                        A[k] * pu[k] + idx * (qX[e - RZ] - qX[e]) + idz * (qS[e - 1] - qS[e]);
-                    const float nw =
-                        A[k] * pw[k] + idx * (qS[e - RZ] - qS[e]) + idz * (qZ[e - 1] - qZ[e]);
-                    if (own_in) {
-                        const float axx = dt * A[k] * pxx[k], azz = dt * A[k] * pzz[k];
-                        const float axz = dt * A[k] * pxz[k];
-                        const int w = wo[k];
-                        const float u0 = buf[HV::u * FW_ + w], w0 = buf[HV::w * FW_ + w];
-                        const float ux = (buf[HV::u * FW_ + w + W::HZ] - u0) * idx;
-                        const float uz = (buf[HV::u * FW_ + w + 1] - u0) * idz;
-                        const float wx = (buf[HV::w * FW_ + w + W::HZ] - w0) * idx;
-                        const float wz = (buf[HV::w * FW_ + w + 1] - w0) * idz;
-                        gl[k] += (axx + azz) * (ux + wz);
-                        gm[k] += 2.0f * (axx * ux + azz * wz);
-                        gx[k] += axz * (uz + wx);
-                    }
-                    pu[k] = nu;
-                    pw[k] = nw;
-                }
-                if (SH) {
-                    const float nv =
-                        A[k] * pv[k] + idx * (qY[e - RZ] - qY[e]) + idz * (qV[e - 1] - qV[e]);
-                    if (own_in) {
-                        const int w = wo[k];
-                        const float v0 = buf[HV::v * FW_ + w];
-                        const float vx = (buf[HV::v * FW_ + w + W::HZ] - v0) * idx;
-                        const float vz = (buf[HV::v * FW_ + w + 1] - v0) * idz;
-                        gyx[k] += dt * A[k] * pyx[k] * vx;
-                        gyz[k] += dt * A[k] * pyz[k] * vz;
-                    }
-                    pv[k] = nv;
-                }
-                if (sg[k] >= 0)
-                    for (int q = sstart[sg[k]]; q < sstart[sg[k] + 1]; q++) {
-                        const size_t o = (size_t)sorder[q] * nt + it;
+                if (!(fl[k] & 1u)) continue;
+                const int e = t + NT * k;
+                if (rg[k] >= 0) {
+                    const size_t row = (size_t)it * a.n_rec;
+                    for (int q = rstart[rg[k]]; q < rstart[rg[k] + 1]; q++) {
+                        const int l = rorder[q];
                         if (PSV) {
-                            gu[k] += pu[k] * wu[o] * factor;
-                            gw[k] += pw[k] * ww[o] * factor;
+                            pu[k] += res_u[row + l];
+                            pw[k] += res_w[row + l];
                         }
-                        if (SH) gr[k] += pv[k] * wv[o] * factor;
+                        if (SH) pv[k] += res_v[row + l];
                     }
+                }
+                const bool in_ = fl[k] & 2u;
+                if (PSV) {
+                    const float axx = dt * (A[k] * pxx[k]), azz = dt * (A[k] * pzz[k]);
+                    qX[e] = in_ ? (c2[k] + 2.0f * c3[k]) * axx + c2[k] * azz : 0.0f;
+                    qZ[e] = in_ ? c2[k] * axx + (c2[k] + 2.0f * c3[k]) * azz : 0.0f;
+                    qS[e] = in_ ? (c4[k] * dt) * (A[k] * pxz[k]) : 0.0f;
+                }
+                if (SH) {
+                    qY[e] = in_ ? (d1[k] * dt) * (A[k] * pyx[k]) : 0.0f;
+                    qV[e] = in_ ? (d2[k] * dt) * (A[k] * pyz[k]) : 0.0f;
+                }
             }
-            if (PSV) {
-                bU[e] = in_ ? dt * c0[k] * pu[k] : 0.0f;
-                bW[e] = in_ ? dt * c1[k] * pw[k] : 0.0f;
-            }
-            if (SH) bV[e] = in_ ? dt * d0[k] * pv[k] : 0.0f;
-        }
-        lds_barrier();
-
-        // ---- phase 2: new stress adjoint; inv_rho_u / inv_rho_w (inv_rho)
-        // gradients of the tile
+            vm_wait();  // this step's windows (and the residuals) arrived
+            lds_barrier();
+            // the next step's velocity window (the one of top_i - 1 came with the
+            // recomputation); the window it replaces was last read a step ago
+            if (s > 0 && it - 1 >= t_lo) fetch_v(it - 1, nxt);
+            if (late_alpha && it == alpha_load) {
 #pragma unroll
-        for (int k = 0; k < NC; k++) {
-            if ((fl[k] & 17u) != 17u) continue;
-            const int e = t + NT * k;
-            const bool own_in = (fl[k] & 6u) == 6u;
-            if (PSV) {
-                const float bu0 = bU[e], bw0 = bW[e];
-                pxx[k] = A[k] * pxx[k] + idx * (bu0 - bU[e + RZ]);
-                pxz[k] = A[k] * pxz[k] + idz * (bu0 - bU[e + 1]) + idx * (bw0 - bW[e + RZ]);
-                pzz[k] = A[k] * pzz[k] + idz * (bw0 - bW[e + 1]);
-                if (own_in) {
-                    const int w = wo[k];
-                    const float xx = sw[HV::sxx * FW_ + w], xz = sw[HV::sxz * FW_ + w];
-                    const float zz = sw[HV::szz * FW_ + w];
-                    const float sxx_x = (xx - sw[HV::sxx * FW_ + w - W::HZ]) * idx;
-                    const float sxz_z = (xz - sw[HV::sxz * FW_ + w - 1]) * idz;
-                    const float sxz_x = (xz - sw[HV::sxz * FW_ + w - W::HZ]) * idx;
-                    const float szz_z = (zz - sw[HV::szz * FW_ + w - 1]) * idz;
-                    gu[k] += pu[k] * dt * (sxx_x + sxz_z);
-                    gw[k] += pw[k] * dt * (sxz_x + szz_z);
+                for (int k = 0; k < NC; k++) {
+                    if (!(fl[k] & 32u)) continue;
+                    const int c = gc[k];
+                    if (PSV) {
+                        mxx[k] = ld(hstr + HV::sxx * n, c);
+                        mzz[k] = ld(hstr + HV::szz * n, c);
+                        mxz[k] = ld(hstr + HV::sxz * n, c);
+                    }
+                    if (SH) {
+                        myx_[k] = ld(hstr + HV::syx * n, c);
+                        myz_[k] = ld(hstr + HV::syz * n, c);
+                    }
                 }
             }
-            if (SH) {
-                const float bv0 = bV[e];
-                pyx[k] = A[k] * pyx[k] + idx * (bv0 - bV[e + RZ]);
-                pyz[k] = A[k] * pyz[k] + idz * (bv0 - bV[e + 1]);
-                if (own_in) {
+            if (late_alpha && it == alpha_i) {
+                float* const dst = hsw + W::sfloats;
+#pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    if (!(fl[k] & 32u)) continue;
                     const int w = wo[k];
-                    const float yx = sw[HV::syx * FW_ + w], yz = sw[HV::syz * FW_ + w];
-                    const float syx_x = (yx - sw[HV::syx * FW_ + w - W::HZ]) * idx;
-                    const float syz_z = (yz - sw[HV::syz * FW_ + w - 1]) * idz;
-                    gr[k] += pv[k] * dt * (syx_x + syz_z);
+                    if (PSV) {
+                        dst[HV::sxx * FW_ + w] = mxx[k];
+                        dst[HV::szz * FW_ + w] = mzz[k];
+                        dst[HV::sxz * FW_ + w] = mxz[k];
+                    }
+                    if (SH) {
+                        dst[HV::syx * FW_ + w] = myx_[k];
+                        dst[HV::syz * FW_ + w] = myz_[k];
+                    }
+                }
+            }
+
+            // ---- phase 1b: new velocity adjoint; lam / mu / mxz (myx / myz) and
+            // source-term gradients of the tile; b = dt * inv_rho * p for phase 2
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                if (!(fl[k] & 1u)) continue;
+                const int e = t + NT * k;
+                const bool in_ = fl[k] & 2u;
+                const bool own_in = (fl[k] & 6u) == 6u;
+                if (fl[k] & 8u) {
+                    if (PSV) {
+                        const float nu =
+                            A[k] * pu[k] + idx * (qX[e - RZ] - qX[e]) + idz * (qS[e - 1] - qS[e]);
+                        const float nw =
+                            A[k] * pw[k] + idx * (qS[e - RZ] - qS[e]) + idz * (qZ[e - 1] - qZ[e]);
+                        if (own_in) {
+                            const float axx = dt * A[k] * pxx[k], azz = dt * A[k] * pzz[k];
+                            const float axz = dt * A[k] * pxz[k];
+                            const int w = wo[k];
+                            const float u0 = buf[HV::u * FW_ + w], w0 = buf[HV::w * FW_ + w];
+                            const float ux = (buf[HV::u * FW_ + w + W::HZ] - u0) * idx;
+                            const float uz = (buf[HV::u * FW_ + w + 1] - u0) * idz;
+                            const float wx = (buf[HV::w * FW_ + w + W::HZ] - w0) * idx;
+                            const float wz = (buf[HV::w * FW_ + w + 1] - w0) * idz;
+                            gl[k] += (axx + azz) * (ux + wz);
+                            gm[k] += 2.0f * (axx * ux + azz * wz);
+                            gx[k] += axz * (uz + wx);
+                        }
+                        pu[k] = nu;
+                        pw[k] = nw;
+                    }
+                    if (SH) {
+                        const float nv =
+                            A[k] * pv[k] + idx * (qY[e - RZ] - qY[e]) + idz * (qV[e - 1] - qV[e]);
+                        if (own_in) {
+                            const int w = wo[k];
+                            const float v0 = buf[HV::v * FW_ + w];
+                            const float vx = (buf[HV::v * FW_ + w + W::HZ] - v0) * idx;
+                            const float vz = (buf[HV::v * FW_ + w + 1] - v0) * idz;
+                            gyx[k] += dt * A[k] * pyx[k] * vx;
+                            gyz[k] += dt * A[k] * pyz[k] * vz;
+                        }
+                        pv[k] = nv;
+                    }
+                    if (sg[k] >= 0)
+                        for (int q = sstart[sg[k]]; q < sstart[sg[k] + 1]; q++) {
+                            const size_t o = (size_t)sorder[q] * nt + it;
+                            if (PSV) {
+                                gu[k] += pu[k] * wu[o] * factor;
+                                gw[k] += pw[k] * ww[o] * factor;
+                            }
+                            if (SH) gr[k] += pv[k] * wv[o] * factor;
+                        }
+                }
+                if (PSV) {
+                    bU[e] = in_ ? dt * c0[k] * pu[k] : 0.0f;
+                    bW[e] = in_ ? dt * c1[k] * pw[k] : 0.0f;
+                }
+                if (SH) bV[e] = in_ ? dt * d0[k] * pv[k] : 0.0f;
+            }
+            lds_barrier();
+
+            // ---- phase 2: new stress adjoint; inv_rho_u / inv_rho_w (inv_rho)
+            // gradients of the tile
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                if ((fl[k] & 17u) != 17u) continue;
+                const int e = t + NT * k;
+                const bool own_in = (fl[k] & 6u) == 6u;
+                if (PSV) {
+                    const float bu0 = bU[e], bw0 = bW[e];
+                    pxx[k] = A[k] * pxx[k] + idx * (bu0 - bU[e + RZ]);
+                    pxz[k] = A[k] * pxz[k] + idz * (bu0 - bU[e + 1]) + idx * (bw0 - bW[e + RZ]);
+                    pzz[k] = A[k] * pzz[k] + idz * (bw0 - bW[e + 1]);
+                    if (own_in) {
+                        const int w = wo[k];
+                        const float xx = sw[HV::sxx * FW_ + w], xz = sw[HV::sxz * FW_ + w];
+                        const float zz = sw[HV::szz * FW_ + w];
+                        const float sxx_x = (xx - sw[HV::sxx * FW_ + w - W::HZ]) * idx;
+                        const float sxz_z = (xz - sw[HV::sxz * FW_ + w - 1]) * idz;
+                        const float sxz_x = (xz - sw[HV::sxz * FW_ + w - W::HZ]) * idx;
+                        const float szz_z = (zz - sw[HV::szz * FW_ + w - 1]) * idz;
+                        gu[k] += pu[k] * dt * (sxx_x + sxz_z);
+                        gw[k] += pw[k] * dt * (sxz_x + szz_z);
+                    }
+                }
+                if (SH) {
+                    const float bv0 = bV[e];
+                    pyx[k] = A[k] * pyx[k] + idx * (bv0 - bV[e + RZ]);
+                    pyz[k] = A[k] * pyz[k] + idz * (bv0 - bV[e + 1]);
+                    if (own_in) {
+                        const int w = wo[k];
+                        const float yx = sw[HV::syx * FW_ + w], yz = sw[HV::syz * FW_ + w];
+                        const float syx_x = (yx - sw[HV::syx * FW_ + w - W::HZ]) * idx;
+                        const float syz_z = (yz - sw[HV::syz * FW_ + w - 1]) * idz;
+                        gr[k] += pv[k] * dt * (syx_x + syz_z);
+                    }
                 }
             }
         }
-    }
 
+        top_i = alpha_i - 1;
+    }
     // ---- the tile's adjoint state and gradients
     float* const* out = a.flip ? a.st : a.alt;
 #pragma unroll
@@ -1286,7 +1300,8 @@ static bool adj_cfg_fits(int systems, const BlockCfg& c) {
     const size_t ns = (psv ? 3 : 0) + (sh ? 2 : 0);
     const size_t rn = (size_t)(c.tx + 2 * c.k) * (c.tz + 2 * c.k);
     const size_t field = (((size_t)(c.tx + 2) * (c.tz + 2) + 63) / 64) * 64;
-    const size_t slots = c.k - 1 > 2 ? c.k - 1 : 2;
+    const int ks = adj_stress_every(c.k);
+    const size_t slots = ks - 1 > 2 ? ks - 1 : 2;
     return 4 * (2 * nv * field + slots * ns * field + rn * nf) <= kLdsLimit;
 }
 
@@ -1295,7 +1310,11 @@ static const BlockCfg kFwiCfg[] = {{24, 64, 768, 4},  {32, 64, 1024, 4}, {16, 64
                                    {16, 64, 512, 8},  {32, 64, 1024, 2}, {16, 64, 256, 2},
                                    {16, 48, 512, 4},  {24, 32, 512, 4}};
 // (the 512-thread 16 x 48 and 24 x 32 shapes keep the P-SV adjoint within half
-// the LDS of a CU, so two workgroups share one)
+// the LDS of a CU, so two workgroups share one and each one's barrier waits
+// overlap the other's work.  Measured slower on the 4000 x 2000 grid, 5 shots:
+// 16 x 48 and 24 x 32 with 8 steps per launch and 1024 threads, which read the
+// state, medium and gradients half as often per step but leave one workgroup
+// per CU (+22-25 %), and 16 x 32 with 320 threads, three per CU (+45 %))
 // defaults: the forward passes {24, 64, 768, 4} (the fastest measured on
 // MI355X for 8 P-SV shots of 600 x 300), the adjoint {16, 48, 512, 4} (its
 // LDS fits every wave system); the first listed shape that fits otherwise
@@ -1499,22 +1518,25 @@ static int fwi_block_run(const trtm_fwi_desc* d, int n, int it_begin, int it_end
             return fail(TRTM_ERR_ARG, "shot %d: blocked FWI kernel needs the workspace", q);
     const bool hist = !adjoint && d->hist != nullptr;
     const int K = c.k, groups = (n + kShotsPerLaunch - 1) / kShotsPerLaunch;
-    if (adjoint && d->hist_every != K)
+    const int KS = adj_stress_every(K);
+    if (adjoint && d->hist_every != KS)
         return fail(TRTM_ERR_ARG, "the blocked FWI adjoint of %d steps per launch reads the "
-                    "compact history with hist_every = %d (got %d)", K, K, d->hist_every);
+                    "compact history with hist_every = %d (got %d)", K, KS, d->hist_every);
     if (hist && d->hist_every < 0)
         return fail(TRTM_ERR_ARG, "hist_every must be >= 0");
     std::vector<FB> fb(n);
     for (int q = 0; q < n; q++) fb[q] = make_fb(&d[q], adjoint);
-    // launches of up to K steps in stepping order (adjoint: each inside one
-    // interval of K steps from hist_base, whose first step's stresses are
-    // stored); an even number of them (the first one split in two when
+    // launches of up to K steps in stepping order (adjoint: whole intervals of
+    // KS steps from hist_base, whose first steps' stresses are stored, at most
+    // K / KS of them); an even number of them (the first one split in two when
     // needed) leaves the state in the caller's buffers for every shot that
     // runs to the end of the range
     std::vector<std::pair<int, int>> chunks;
     if (adjoint) {
         for (int hi = it_end; hi > it_begin;) {
-            const int lo = std::max(d->hist_base + ((hi - 1 - d->hist_base) / K) * K, it_begin);
+            const int q = (hi - 1 - d->hist_base) / KS;  // interval of step hi - 1
+            const int q_lo = std::max(q - (K / KS - 1), 0);
+            const int lo = std::max(d->hist_base + q_lo * KS, it_begin);
             chunks.push_back({lo, hi});
             hi = lo;
         }

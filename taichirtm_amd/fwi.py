@@ -57,7 +57,8 @@ _KERNELS = {"auto": _lib.KERNEL_AUTO, "step": _lib.KERNEL_SIMPLE, "blocked": _li
             "tuned": _lib.KERNEL_BLOCKED}
 _WINDOW = 64  # steps per autotuning trial
 # steps between the stored stresses of the compact history of the blocked
-# kernels (and the blocked adjoint's steps per launch) unless a shape is given
+# kernels: the blocked adjoint of K steps per launch reads stresses stored every
+# min(K, HIST_EVERY) steps
 HIST_EVERY = 4
 GRAD_NAMES = ("g_lam", "g_mu", "g_mxz", "g_myx", "g_myz", "g_ir", "g_iru", "g_irw")
 
@@ -256,9 +257,11 @@ class ElasticFWI:
     # The one-step-per-launch kernels keep both of every step (hist_every 0).
 
     def _choose_hist_every(self):
+        # the blocked adjoint of K steps per launch reads stresses stored every
+        # min(K, HIST_EVERY) steps
         if not self.blocked:
             return 0
-        return int(self.shapes["adjoint"][2]) or HIST_EVERY
+        return min(int(self.shapes["adjoint"][2]) or HIST_EVERY, HIST_EVERY)
 
     def _hist_counts(self):
         nv = (2 if self.systems & _lib.PSV else 0) + (1 if self.systems & _lib.SH else 0)
@@ -309,9 +312,8 @@ class ElasticFWI:
 
         shapes = autotune.fwi_candidates(self.systems, name)
         if name == "adjoint" and self.hist_every:
-            # the blocked adjoint takes as many steps per launch as the
-            # history's stress interval
-            shapes = [c for c in shapes if c[3] == self.hist_every]
+            # adjoint shapes that read the history's stress interval
+            shapes = [c for c in shapes if min(c[3], HIST_EVERY) == self.hist_every]
             key = key + (self.hist_every,)
         shape = autotune.tune(key, self.systems, trial, shapes=shapes)
         self.tuned[name] = shape[1:]
