@@ -31,9 +31,13 @@ namespace trtm {
 
 int fail(int code, const char* fmt, ...);
 
+// (512-thread shapes: at least 4 waves per SIMD, i.e. at most 128 VGPRs, so
+// two workgroups share a CU)
 template <bool FWD, bool PSV, bool SH, int TX, int TZ, int NT, int K>
-__global__ __launch_bounds__(NT) void k_block(KArgs m, const ShotDev* __restrict__ shots, int it0,
-                                              int tiles_x, int tiles_z) {
+__global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
+                                                                 const ShotDev* __restrict__ shots,
+                                                                 int it0, int tiles_x,
+                                                                 int tiles_z) {
     constexpr int RX = TX + 2 * K, RZ = TZ + 2 * K, RN = RX * RZ;
     constexpr int NC = (RN + NT - 1) / NT;
     static_assert(2 * K <= TX && 2 * K <= TZ,
