@@ -40,7 +40,8 @@ import torch
 from . import _lib
 from ._propagator import Medium, PSV_STATE, SH_STATE, _ptr
 from .backward_modeling import BackwardModeling
-from .field import SnapshotField, ZeroField, as_host, device, free_device_bytes, stream_handle
+from .field import (Field, SnapshotField, ZeroField, as_host, device, free_device_bytes,
+                    stream_handle)
 from .forward_modeling import ForwardModeling
 
 
@@ -208,6 +209,13 @@ class BatchRTM:
         names = [k for k in PSV_STATE if self.systems & _lib.PSV] + \
                 [k for k in SH_STATE if self.systems & _lib.SH]
         self._state_names = names
+        # every shot's state in one (shots, fields, nx, nz) tensor, so saving
+        # or restoring a checkpoint is one copy (or gather) for the batch
+        self._fstate = torch.zeros((len(self.fw), len(names), self.nx, self.nz),
+                                   dtype=torch.float32, device=dev)
+        for q, f in enumerate(self.fw):
+            for m, name in enumerate(names):
+                setattr(f, name, Field(tensor=self._fstate[q, m]))
         if not snapshot:
             n_snap = (2 if self.systems & _lib.PSV else 0) + (1 if self.systems & _lib.SH else 0)
             L = self.checkpoint_every or checkpoint_interval(nt_max, len(names), n_snap,
@@ -288,19 +296,23 @@ class BatchRTM:
                               int(self.snapshot_bytes), kept_bytes, nseg - first_kept, nseg,
                               free), RuntimeWarning)
         self.kept = {}
-        self.ckpt = []
-        for f in self.fw:
-            ncp = min((f.nt + L - 1) // L, first_kept)
-            self.ckpt.append(torch.empty((ncp, len(names), self.nx, self.nz), dtype=torch.float32,
-                                         device=dev))
+        # checkpoint k holds the shots still running at its first step
+        # (ckpt_idx[k]; None: every shot)
+        self.ckpt, self.ckpt_idx = [], []
+        for k in range(first_kept):
+            act = [q for q, f in enumerate(self.fw) if self.edges[k] < f.nt]
+            self.ckpt.append(torch.empty((len(act), len(names), self.nx, self.nz),
+                                         dtype=torch.float32, device=dev))
+            self.ckpt_idx.append(None if len(act) == len(self.fw) else
+                                 torch.tensor(act, dtype=torch.int64, device=dev))
         spare = None  # snapshot target of shots whose record ended before the segment
         for k in range(nseg):
             a, b = self.edges[k], self.edges[k + 1]
             if k < first_kept:
-                for q, f in enumerate(self.fw):
-                    if a < f.nt:
-                        for m, name in enumerate(names):
-                            self.ckpt[q][k, m].copy_(getattr(f, name).tensor)
+                if self.ckpt_idx[k] is None:
+                    self.ckpt[k].copy_(self._fstate)
+                else:
+                    torch.index_select(self._fstate, 0, self.ckpt_idx[k], out=self.ckpt[k])
                 self._batch("trtm_forward_steps_batch", descs, a, b, 0)
                 continue
             bufs = []
@@ -324,13 +336,18 @@ class BatchRTM:
 
     # -- backward -----------------------------------------------------------
 
-    def _backward(self):
+    def _backward_propagators(self):
+        """The backward propagators (their set-up reads the data on the host
+        side, which waits for the device): built before the forward pass is
+        queued when every shot carries its observed data, so the device does
+        not idle between the two passes."""
         dev = device()
         self.bw = []
-        for s, f in zip(self.shots, self.fw):
+        for q, s in enumerate(self.shots):
             if s.observed is not None:
                 obs = observed_arrays(s.observed)
             else:
+                f = self.fw[q]
                 seis = f.seismogram_tensors()
                 obs = []
                 for c in ("u", "v", "w"):
@@ -348,6 +365,11 @@ class BatchRTM:
             raise ValueError("the observed data of every shot of a batch must excite the same "
                              "wave systems (a shot without data migrates to a zero image on "
                              "its own)")
+
+    def _backward(self):
+        dev = device()
+        if self.bw is None:
+            self._backward_propagators()
         self.flags_b = []
         descs, keep, self._isn = [], [], []
         nt_max = max(s.nt for s in self.shots)
@@ -407,10 +429,12 @@ class BatchRTM:
                 seg = self.kept.pop(k)  # written by the forward pass
             else:
                 seg = bufs
-                for q, f in enumerate(self.fw):
-                    if a < f.nt:
-                        for m, name in enumerate(self._state_names):
-                            getattr(f, name).tensor.copy_(self.ckpt[q][k, m])
+                if self.ckpt_idx[k] is None:
+                    self._fstate.copy_(self.ckpt[k])
+                else:
+                    self._fstate.index_copy_(0, self.ckpt_idx[k], self.ckpt[k])
+                self.ckpt[k] = None  # used once: free it
+                for q in range(len(self.fw)):
                     fdescs[q].snap_base = base
                 self._batch("trtm_forward_steps_batch", fdescs, a, b, 1)
             isns = []
@@ -556,6 +580,9 @@ class BatchRTM:
     def run(self) -> List[int]:
         """Forward then backward for every shot; returns per-shot codes
         (0, 1-3 forward, 4-6 backward; the first failing check step)."""
+        self.bw = None
+        if all(s.observed is not None for s in self.shots):
+            self._backward_propagators()
         self._forward()
         self._backward()
         self._close_plans()
