@@ -32,7 +32,8 @@ namespace trtm {
 int fail(int code, const char* fmt, ...);
 
 // (512-thread shapes: at least 4 waves per SIMD, i.e. at most 128 VGPRs, so
-// two workgroups share a CU)
+// two workgroups share a CU; a 32 x 64, 8-step P-SV shape with 1024 threads
+// held to 64 VGPRs for two per CU spilled 35-62 registers)
 template <bool FWD, bool PSV, bool SH, int TX, int TZ, int NT, int K>
 __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                                                                  const ShotDev* __restrict__ shots,

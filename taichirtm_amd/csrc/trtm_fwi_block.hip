@@ -198,9 +198,20 @@ __device__ __forceinline__ RegionCell region_cell(int e, int i0, int j0, int nx,
 // ---------------------------------------------------------------------------
 // forward, steps t_a .. t_b-1 (t_b - t_a <= K)
 
+// Waves per SIMD asked of the forward kernel: the 8-step shapes whose LDS
+// leaves room for two workgroups per CU get the registers that allow two
+// (24 x 64 with 768 threads: 6 waves, 85 VGPRs; 32 x 64 with 1024: 8 waves,
+// 64 VGPRs); the rest keep the compiler's choice.
+template <bool PSV, int TX, int TZ, int NT, int K>
+constexpr int fwd_waves_per_eu() {
+    return (PSV && K == 8 && TX == 24 && TZ == 64 && NT == 768)    ? 6
+           : (PSV && K == 8 && TX == 32 && TZ == 64 && NT == 1024) ? 8
+                                                                   : 1;
+}
+
 template <bool PSV, bool SH, int TX, int TZ, int NT, int K, bool HIST, class S>
-__global__ __launch_bounds__(NT) void k_fwi_fwd_blk(S src, int t_a, int t_b, int tiles_x,
-                                                    int tiles_z) {
+__global__ __launch_bounds__(NT, (fwd_waves_per_eu<PSV && !SH, TX, TZ, NT, K>())) void
+k_fwi_fwd_blk(S src, int t_a, int t_b, int tiles_x, int tiles_z) {
     constexpr int RX = TX + 2 * K, RZ = TZ + 2 * K, RN = RX * RZ;
     constexpr int NC = (RN + NT - 1) / NT;
     constexpr int NP = PSV ? RN : 1, NS = SH ? RN : 1;
