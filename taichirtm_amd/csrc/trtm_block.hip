@@ -261,26 +261,52 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
         }
     }
 
+    // it, tb = nt - 1 - it (both per isnap) and it per check interval as
+    // floor quotient and remainder, advanced by one per step instead of
+    // divided each step (the same slots as it / isnap, it % isnap, ...)
+    auto fdiv = [](int a, int b) { const int q = a / b; return (a % b != 0 && a < 0) ? q - 1 : q; };
+    const int it_first = it0 + it_off;
+    int sq = fdiv(it_first, isnap), sr = it_first - sq * isnap;
+    int tq = fdiv(nt - 1 - it_first, isnap), tr = nt - 1 - it_first - tq * isnap;
+    int cq = 0, cr = 0;
+    if (cint > 0) {
+        cq = fdiv(it_first, cint);
+        cr = it_first - cq * cint;
+    }
     bool started = false;  // a step ran in this launch: LDS velocities await the sponge
     for (int s = 0; s < K; s++) {
         const int it = it0 + it_off + s;
+        if (s > 0) {
+            if (++sr == isnap) {
+                sr = 0;
+                ++sq;
+            }
+            if (tr-- == 0) {
+                tr = isnap - 1;
+                --tq;
+            }
+            if (cint > 0 && ++cr == cint) {
+                cr = 0;
+                ++cq;
+            }
+        }
         if (it < it_lo) continue;
         if (it >= it_end) break;
         const int tb = nt - 1 - it;  // backward time index
         // this step's snapshot slot to write (forward) or correlate (backward),
         // finite-check slot; -1 none
         int snap = -1, img = -1, chk = -1;
-        if (FWD && save && it % isnap == 0 && it != 0) {
-            const int sl = it / isnap - 1 - snap_base;
+        if (FWD && save && sr == 0 && it != 0) {
+            const int sl = sq - 1 - snap_base;
             if (sl >= 0 && sl < num_snaps) snap = sl;
         }
         if constexpr (PIPE) {
             img = img_next;
-        } else if (imaging && tb > 0 && tb % isnap == 0) {
-            const int sl = tb / isnap - 1 - snap_base;
+        } else if (imaging && tb > 0 && tr == 0) {
+            const int sl = tq - 1 - snap_base;
             if (sl >= 0 && sl < num_snaps && C(sh->isnaps)[sl] == tb) img = sl;
         }
-        if (cint > 0 && it % cint == 0) chk = it / cint;
+        if (cint > 0 && cr == 0) chk = cq;
         // this step's injected values of the slots' single-entry groups,
         // loaded now and added after the velocity update
         constexpr int NG = HOIST ? NC : 1;
@@ -355,7 +381,13 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                 }
             }
             // the next step's forward wavefield, in flight during this step
-            img_next = (s + 1 < K && it + 1 < it_end) ? image_slot(it + 1) : -1;
+            // image_slot(it + 1), from tb - 1 = tq' * isnap + tr'
+            img_next = -1;
+            if (s + 1 < K && it + 1 < it_end && imaging && tb - 1 > 0 &&
+                (tr == 0 ? isnap == 1 : tr == 1)) {
+                const int sl = (tr == 0 ? tq - 1 : tq) - 1 - snap_base;
+                if (sl >= 0 && sl < num_snaps && C(sh->isnaps)[sl] == tb - 1) img_next = sl;
+            }
             if (img_next >= 0) load_forward(img_next);
         } else if constexpr (SH && !PSV) {
             if (img >= 0) {
