@@ -46,9 +46,13 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
     // Region arrays with RZ + 1 cells of padding on each side and room for
     // every thread slot: without a surface matrix the updates run branch-free
     // on every slot (neighbour reads of edge and spare slots stay inside the
-    // array) and the result is selected per cell.
-    constexpr int PAD = PSV ? 0 : RZ + 1;  // the branch-free updates run for SH only
-    constexpr int NA = PSV ? RN : (RN > NT * NC ? RN : NT * NC) + 2 * PAD;
+    // array) and the result is selected per cell.  SH-only kernels and the
+    // P-SV-only shapes with at most two cells per thread run them; the other
+    // P-SV kernels keep per-cell branches (their registers are the tighter
+    // limit).
+    constexpr bool BFP = PSV && !SH && NC <= 2;  // branch-free P-SV updates
+    constexpr int PAD = (!PSV || BFP) ? RZ + 1 : 0;
+    constexpr int NA = PAD == 0 ? RN : (RN > NT * NC ? RN : NT * NC) + 2 * PAD;
     constexpr int NP = PSV ? NA : 1, NS = SH ? NA : 1;
     __shared__ float sU_[NP], sW_[NP], sXX_[NP], sZZ_[NP], sXZ_[NP];
     __shared__ float sV_[NS], sYX_[NS], sYZ_[NS];
@@ -509,7 +513,46 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                 }
             }
         };
-        if constexpr (PSV) {
+        // P-SV, small shapes, flat surface: branch-free over every slot
+        auto velocity_psv_bf = [&]() {
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                const int e = t + NT * k;
+                const bool upd = (fl[k] & 10u) == 10u;
+                float u = sU[e], w = sW[e];
+                if (started) {
+                    u = u * A[k];
+                    w = w * A[k];
+                }
+                float un, wn;
+                if (FWD) {
+                    const float sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
+                    const float szz_z = (sZZ[e] - sZZ[e - 1]) * idz;
+                    const float sxz_x = (sXZ[e] - sXZ[e - RZ]) * idx;
+                    const float sxz_z = (sXZ[e] - sXZ[e - 1]) * idz;
+                    un = u + ((sxx_x + sxz_z) * dt) * c0[k];
+                    wn = w + ((sxz_x + szz_z) * dt) * c1[k];
+                } else {
+                    const float sxx_x = (sXX[e + RZ] - sXX[e]) * idx;
+                    const float szz_z = (sZZ[e + 1] - sZZ[e]) * idz;
+                    const float sxz_x = (sXZ[e + RZ] - sXZ[e]) * idx;
+                    const float sxz_z = (sXZ[e + 1] - sXZ[e]) * idz;
+                    un = u + ((-(sxx_x + sxz_z)) * dt) * c0[k];
+                    wn = w + ((-(sxz_x + szz_z)) * dt) * c1[k];
+                }
+                u = upd ? un : u;
+                w = upd ? wn : w;
+                inject_u(k, u, w);
+                sU[e] = u;
+                sW[e] = w;
+            }
+        };
+        if constexpr (BFP) {
+            if (has_sm)
+                velocity_psv(std::true_type{});
+            else
+                velocity_psv_bf();
+        } else if constexpr (PSV) {
             if (has_sm)
                 velocity_psv(std::true_type{});
             else
@@ -695,7 +738,62 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                 }
             }
         };
-        if constexpr (PSV) {
+        // P-SV, small shapes, flat surface: branch-free over every slot; the
+        // outputs (snapshots, finite check) only for the tile's cells
+        auto stress_psv_bf = [&]() {
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                const int e = t + NT * k;
+                const int c = gc[k];
+                const bool upd = (fl[k] & 18u) == 18u;
+                float sxx = sXX[e], szz = sZZ[e], sxz = sXZ[e];
+                float u_x, u_z, w_x, w_z;
+                if (FWD) {
+                    u_x = (sU[e + RZ] - sU[e]) * idx;
+                    u_z = (sU[e + 1] - sU[e]) * idz;
+                    w_x = (sW[e + RZ] - sW[e]) * idx;
+                    w_z = (sW[e + 1] - sW[e]) * idz;
+                } else {
+                    u_x = (sU[e] - sU[e - RZ]) * idx;
+                    u_z = (sU[e] - sU[e - 1]) * idz;
+                    w_x = (sW[e] - sW[e - RZ]) * idx;
+                    w_z = (sW[e] - sW[e - 1]) * idz;
+                }
+                const float div_uw = u_x + w_z;
+                const float xxn = sxx + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
+                const float zzn = szz + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
+                const float xzn = sxz + (sdt * c4[k]) * (u_z + w_x);
+                sxx = upd ? xxn : sxx;
+                szz = upd ? zzn : szz;
+                sxz = upd ? xzn : sxz;
+                sXX[e] = sxx * A[k];
+                sZZ[e] = szz * A[k];
+                sXZ[e] = sxz * A[k];
+                const float u = sU[e] * A[k], w = sW[e] * A[k];
+                if constexpr (PIPE) {  // the image of a slot outside the tile is never written
+                    if (img >= 0) {
+                        aU[k] = aU[k] + fimu[k] * u;
+                        aW[k] = aW[k] + fimw[k] * w;
+                    }
+                }
+                if (fl[k] & 4u) {
+                    if (snap >= 0) {
+                        snap_u[(size_t)snap * sstride + c] = u;
+                        snap_w[(size_t)snap * sstride + c] = w;
+                    }
+                    if (chk >= 0) {
+                        if (not_finite_or_big(u) && code < 1) code = 1;
+                        if (not_finite_or_big(w)) code = 3;
+                    }
+                }
+            }
+        };
+        if constexpr (BFP) {
+            if (has_sm)
+                stress_psv(std::true_type{});
+            else
+                stress_psv_bf();
+        } else if constexpr (PSV) {
             if (has_sm)
                 stress_psv(std::true_type{});
             else
