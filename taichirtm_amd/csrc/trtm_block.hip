@@ -66,9 +66,14 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
     float* const sYX = sYX_ + OS;
     float* const sYZ = sYZ_ + OS;
 
-    // small shapes: injection group of each region cell (see below)
+    // small shapes: injection group of each region cell, and the injected
+    // values of the launch's K steps for up to kInjTab single-entry groups of
+    // the tile (see below)
     constexpr bool HOIST = NC <= 2;
+    constexpr int kInjTab = 64;
     __shared__ int gmap[HOIST ? RN : 1];
+    __shared__ float tabU[HOIST && PSV ? kInjTab * K : 1], tabW[HOIST && PSV ? kInjTab * K : 1];
+    __shared__ float tabV[HOIST && SH ? kInjTab * K : 1];
 
     const ShotDev* __restrict__ sh = shots + blockIdx.y;
     const int it_end = sh->it_end, it_lo = sh->it_lo, it_off = sh->it_off;
@@ -82,6 +87,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
     const int nx = m.nx, nz = m.nz;
     const int t = threadIdx.x;
     const gptr<const float> sm = G(m.sm);
+    const int it_first = it0 + it_off;
 
     // per owned cell: global offset, flags, sponge factor, coefficients
     int gc[NC];
@@ -122,7 +128,11 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
         fl[k] = f;
     }
 
-    // load the region's state
+    // load the region's state (the small shapes hold it in registers until
+    // the lookups and the schedule below are issued, so that those overlap
+    // the loads)
+    constexpr int NR = HOIST ? NC : 1;
+    float st[NR][PSV && SH ? 8 : (PSV ? 5 : 3)];
     {
         const gptr<float> in_u = G(sh->in[0]), in_v = G(sh->in[1]), in_w = G(sh->in[2]);
         const gptr<float> in_xx = G(sh->in[3]), in_zz = G(sh->in[4]), in_xz = G(sh->in[5]);
@@ -133,17 +143,37 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
             if (e >= RN) continue;
             const bool g = fl[k] & 1u;
             const int c = gc[k];
+            float x[PSV && SH ? 8 : (PSV ? 5 : 3)];
+            int q = 0;
             if (PSV) {
-                sU[e] = g ? in_u[c] : 0.0f;
-                sW[e] = g ? in_w[c] : 0.0f;
-                sXX[e] = g ? in_xx[c] : 0.0f;
-                sZZ[e] = g ? in_zz[c] : 0.0f;
-                sXZ[e] = g ? in_xz[c] : 0.0f;
+                x[q++] = g ? in_u[c] : 0.0f;
+                x[q++] = g ? in_w[c] : 0.0f;
+                x[q++] = g ? in_xx[c] : 0.0f;
+                x[q++] = g ? in_zz[c] : 0.0f;
+                x[q++] = g ? in_xz[c] : 0.0f;
             }
             if (SH) {
-                sV[e] = g ? in_v[c] : 0.0f;
-                sYX[e] = g ? in_yx[c] : 0.0f;
-                sYZ[e] = g ? in_yz[c] : 0.0f;
+                x[q++] = g ? in_v[c] : 0.0f;
+                x[q++] = g ? in_yx[c] : 0.0f;
+                x[q++] = g ? in_yz[c] : 0.0f;
+            }
+            if constexpr (HOIST) {
+#pragma unroll
+                for (int f = 0; f < q; f++) st[k][f] = x[f];
+            } else {
+                q = 0;
+                if (PSV) {
+                    sU[e] = x[q++];
+                    sW[e] = x[q++];
+                    sXX[e] = x[q++];
+                    sZZ[e] = x[q++];
+                    sXZ[e] = x[q++];
+                }
+                if (SH) {
+                    sV[e] = x[q++];
+                    sYX[e] = x[q++];
+                    sYZ[e] = x[q++];
+                }
             }
         }
     }
@@ -163,29 +193,19 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
             if (SH) aV[k] = img_v[gc[k]];
         }
     }
-    __syncthreads();
-
-    const size_t n = (size_t)nx * nz;
-    const float sdt = FWD ? m.dt : -m.dt;
-    const float idx = m.idx, idz = m.idz, dt = m.dt;
-    const bool has_sm = m.sm != nullptr;
-    const int save = sh->save, isnap = sh->isnap, num_snaps = sh->num_snaps;
-    const int snap_base = sh->snap_base, cint = sh->check_interval;
-    const int n_groups = sh->n_groups, n_rec = sh->n_rec;
-    const gptr<float> snap_u = G(sh->snap_u), snap_v = G(sh->snap_v), snap_w = G(sh->snap_w);
-    const size_t sstride = (size_t)sh->snap_stride;
-    const bool ssponge = sh->snap_sponge != 0;
 
     // The tile's injection groups and receivers, looked up once per launch
-    // (lists no longer than the workgroup; a longer list keeps the per-step
-    // loops below): each region cell's thread slot learns its group (the data
-    // row of a single entry, or the group of several) and adds the injection
-    // right after its own velocity update, with no separate injection phase
-    // and barrier, from a value loaded at the start of the step; thread t
-    // records receiver rb0 + t.  Only for shapes with at most two cells per
-    // thread: their few workgroups per CU leave the per-step lookups' latency
+    // while the region's loads are in flight (lists no longer than kInjTab
+    // groups / the workgroup's receivers; longer lists keep the per-step
+    // loops below): each region cell's thread slot learns
+    // its group (a row of the table of injected values, or the group of
+    // several entries) and adds the injection right after its own velocity
+    // update, with no separate injection phase and barrier; thread t records
+    // receiver rb0 + t.  Only for shapes with at most two cells per thread:
+    // their few workgroups per CU leave per-step global loads' latency
     // exposed, and they have registers to spare; the larger shapes keep the
     // per-step lookups.
+    const int n_groups = sh->n_groups, n_rec = sh->n_rec;
     int ig0 = 0, ig1 = 0, rb0 = 0, rb1 = 0;
     if (n_groups > 0) {
         const cptr<int> tstart = C(sh->inj_tstart);
@@ -197,30 +217,49 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
         rb0 = rstart[tile];
         rb1 = rstart[tile + 1];
     }
-    const bool inj_one_pass = HOIST && ig1 - ig0 <= NT, rec_one_pass = HOIST && rb1 - rb0 <= NT;
-    // per slot: the data row of its cell's single-entry group, -1 - g for a
+    const bool inj_one_pass = HOIST && ig1 - ig0 <= kInjTab, rec_one_pass = HOIST && rb1 - rb0 <= NT;
+    // per slot: the table row of its cell's single-entry group, -1 - g for a
     // group g of several entries (summed per step in entry order), or kNone
     constexpr int kNone = -2147483647 - 1;
-    int gk[HOIST ? NC : 1];
+    int i_e = -1, i_code = kNone;  // this thread's group: region cell, code
+    if (HOIST && inj_one_pass && ig0 + t < ig1) {
+        const int g = C(sh->inj_titems)[ig0 + t];
+        const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
+        const int k0 = C(sh->order)[q0];
+        const int ci = C(sh->loc)[2 * k0], cj = C(sh->loc)[2 * k0 + 1];
+        i_e = (ci - ri0) * RZ + (cj - rj0);
+        i_code = q1 - q0 == 1 ? t : -1 - g;
+        if (q1 - q0 == 1) {  // the launch's K injected values of the entry
+            // (loaded together: the time index clamped to [0, nt), steps
+            // outside the shot's range never read theirs)
+            float vu[K], vw[K], vv[K];
 #pragma unroll
-    for (int k = 0; k < (HOIST ? NC : 1); k++) gk[k] = kNone;
+            for (int s = 0; s < K; s++) {
+                int col_ = FWD ? it_first + s : nt - 1 - (it_first + s);
+                col_ = col_ < 0 ? 0 : (col_ >= nt ? nt - 1 : col_);
+                const size_t o = (size_t)k0 * (size_t)nt + col_;
+                vu[s] = vw[s] = vv[s] = 0.0f;
+                if (PSV) {
+                    vu[s] = C(sh->su)[o];
+                    vw[s] = C(sh->sw)[o];
+                }
+                if (SH) vv[s] = C(sh->sv)[o];
+            }
+#pragma unroll
+            for (int s = 0; s < K; s++) {
+                if constexpr (HOIST && PSV) {
+                    tabU[t * K + s] = vu[s];
+                    tabW[t * K + s] = vw[s];
+                }
+                if constexpr (HOIST && SH) tabV[t * K + s] = vv[s];
+            }
+        }
+    }
     if constexpr (HOIST) {
         if (inj_one_pass && ig1 > ig0) {  // uniform
 #pragma unroll
             for (int k = 0; k < NC; k++)
                 if (t + NT * k < RN) gmap[t + NT * k] = kNone;
-            __syncthreads();
-            if (ig0 + t < ig1) {
-                const int g = C(sh->inj_titems)[ig0 + t];
-                const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
-                const int k0 = C(sh->order)[q0];
-                const int ci = C(sh->loc)[2 * k0], cj = C(sh->loc)[2 * k0 + 1];
-                gmap[(ci - ri0) * RZ + (cj - rj0)] = q1 - q0 == 1 ? k0 : -1 - g;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < NC; k++)
-                if (t + NT * k < RN) gk[k] = gmap[t + NT * k];
         }
     }
     int r_l = -1, r_e = 0;  // this thread's receiver
@@ -232,87 +271,158 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
         r_A = sponge_at(ci, cj, nx, nz, m.fw, m.tab);
     }
 
-    // Imaging slot of step it_ (-1: none).  The small shapes with one wave
-    // system load the forward wavefield a whole step before the image update
-    // uses it (the others a phase before): pf0 / pf1 hold the next step's u / w
-    // (or v) of every slot.
-    auto image_slot = [&](int it_) {
-        const int tb_ = nt - 1 - it_;
-        if (!(imaging && tb_ > 0 && tb_ % isnap == 0)) return -1;
-        const int sl = tb_ / isnap - 1 - snap_base;
-        return (sl >= 0 && sl < num_snaps && C(sh->isnaps)[sl] == tb_) ? sl : -1;
-    };
+    const int save = sh->save, isnap = sh->isnap, num_snaps = sh->num_snaps;
+    const int snap_base = sh->snap_base, cint = sh->check_interval;
+    const gptr<float> snap_u = G(sh->snap_u), snap_v = G(sh->snap_v), snap_w = G(sh->snap_w);
+    const size_t sstride = (size_t)sh->snap_stride;
+    const bool ssponge = sh->snap_sponge != 0;
+
+    // The launch's step schedule, worked out once by wave 0 (while the loads
+    // are in flight) and handed to the others through LDS: the launch runs
+    // steps s_lo .. s_hi - 1 of its K (the shot's [it_lo, it_end)).  Bit s of
+    // mult: the step's time index (it forward, tb = nt - 1 - it backward) is a
+    // multiple of isnap, where the snapshot slot q - 1 - snap_base moves on
+    // (up forward, down backward); omask: the forward writes that slot, or the
+    // backward images it (slot in range and its isnaps entry equal to tb);
+    // cmask: finite-check step (it a multiple of cint, index it / cint).  Per
+    // step the slot offset and the check index then follow from bit tests and
+    // running counters, with no division and no table read.
+    __shared__ int sched[8];
+    const int s_lo = it_lo > it_first ? it_lo - it_first : 0;
+    const int s_hi = it_end - it_first < K ? it_end - it_first : K;
+    if (t < 64) {
+        const int it_start = it_first + s_lo;                 // >= it_lo >= 0, < it_end <= nt
+        const int T = FWD ? it_start : nt - 1 - it_start;  // >= 0
+        const int qt = T / isnap, r = T - qt * isnap;
+        const int qn = qt + ((FWD && r != 0) ? 1 : 0);  // quotient at the first multiple
+        unsigned mult = 0, omask = 0, cmask = 0;
+        int sl = qn - 1 - snap_base;
+        for (int b = s_lo + (FWD ? (r != 0 ? isnap - r : 0) : r); b < s_hi;
+             b += isnap, sl += FWD ? 1 : -1) {
+            mult |= 1u << b;
+            const bool in = sl >= 0 && sl < num_snaps;
+            if (FWD && save && in && it_first + b != 0) omask |= 1u << b;
+            if (!FWD && imaging && in) {
+                const int tb_ = nt - 1 - (it_first + b);
+                if (tb_ > 0 && (sh->snaps_regular || C(sh->isnaps)[sl] == tb_)) omask |= 1u << b;
+            }
+        }
+        int cn = 0;
+        if (cint > 0) {
+            const int qc = it_start / cint, rc = it_start - qc * cint;
+            cn = qc + (rc != 0 ? 1 : 0);
+            for (int b = s_lo + (rc != 0 ? cint - rc : 0); b < s_hi; b += cint) cmask |= 1u << b;
+        }
+        const long long q_off = (long long)(qn - 1 - snap_base) * (long long)sstride;
+        if (t == 0) {
+            sched[0] = (int)mult;
+            sched[1] = (int)omask;
+            sched[2] = (int)cmask;
+            sched[3] = cn;
+            sched[4] = (int)(unsigned)(q_off & 0xffffffffll);
+            sched[5] = (int)(q_off >> 32);
+        }
+    }
+    if constexpr (HOIST) {
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            const int e = t + NT * k;
+            if (e >= RN) continue;
+            int q = 0;
+            if (PSV) {
+                sU[e] = st[k][q++];
+                sW[e] = st[k][q++];
+                sXX[e] = st[k][q++];
+                sZZ[e] = st[k][q++];
+                sXZ[e] = st[k][q++];
+            }
+            if (SH) {
+                sV[e] = st[k][q++];
+                sYX[e] = st[k][q++];
+                sYZ[e] = st[k][q++];
+            }
+        }
+    }
+    __syncthreads();
+    // the medium's coefficients count as defined here: computations the
+    // compiler hoists out of the step loop (2 mu, dt * c, ...) then cannot
+    // wait for their loads ahead of the state loads
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        if constexpr (PSV) asm volatile("" : "+v"(c0[k]), "+v"(c1[k]), "+v"(c2[k]), "+v"(c3[k]), "+v"(c4[k]));
+        if constexpr (SH) asm volatile("" : "+v"(d0[k]), "+v"(d1[k]), "+v"(d2[k]));
+    }
+    const unsigned mult = (unsigned)__builtin_amdgcn_readfirstlane(sched[0]);
+    const unsigned omask = (unsigned)__builtin_amdgcn_readfirstlane(sched[1]);
+    const unsigned cmask = (unsigned)__builtin_amdgcn_readfirstlane(sched[2]);
+    int cn = __builtin_amdgcn_readfirstlane(sched[3]);  // index of the next finite check
+    long long q_off = (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(sched[5]) << 32) |
+                                  (unsigned)__builtin_amdgcn_readfirstlane(sched[4]));
+
+    const size_t n = (size_t)nx * nz;
+    const float sdt = FWD ? m.dt : -m.dt;
+    const float idx = m.idx, idz = m.idz, dt = m.dt;
+    const bool has_sm = m.sm != nullptr;
+
+    int gk[HOIST ? NC : 1];
+#pragma unroll
+    for (int k = 0; k < (HOIST ? NC : 1); k++) gk[k] = kNone;
+    if constexpr (HOIST) {
+        if (inj_one_pass && ig1 > ig0) {  // uniform
+            if (i_e >= 0) gmap[i_e] = i_code;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < NC; k++)
+                if (t + NT * k < RN) gk[k] = gmap[t + NT * k];
+        }
+    }
+
+    const long long sstep = FWD ? (long long)sstride : -(long long)sstride;
+
+    // The small shapes with one wave system load the forward wavefield of the
+    // image a whole step before the image update uses it (the others a phase
+    // before): pf0 / pf1 hold the next step's u / w (or v) of every slot.
     constexpr bool PIPE = HOIST && (PSV != SH);
     constexpr int NF = PIPE ? NC : 1;
     float pf0[NF], pf1[NF];
-    int img_next = -1;
-    auto load_forward = [&](int sl) {
+    bool img_next = false;
+    auto load_forward = [&](size_t off) {
 #pragma unroll
         for (int k = 0; k < NF; k++) {
             if (PSV) {
-                pf0[k] = snap_u[(size_t)sl * sstride + gc[k]];
-                pf1[k] = snap_w[(size_t)sl * sstride + gc[k]];
+                pf0[k] = snap_u[off + gc[k]];
+                pf1[k] = snap_w[off + gc[k]];
             } else {
-                pf0[k] = snap_v[(size_t)sl * sstride + gc[k]];
+                pf0[k] = snap_v[off + gc[k]];
             }
         }
     };
     if constexpr (PIPE) {
-        const int first = it0 + it_off > it_lo ? it0 + it_off : it_lo;
-        if (first < it_end && first < it0 + it_off + K) {
-            img_next = image_slot(first);
-            if (img_next >= 0) load_forward(img_next);
+        if ((omask >> s_lo) & 1u) {
+            img_next = true;
+            load_forward((size_t)q_off);
         }
     }
 
-    // it, tb = nt - 1 - it (both per isnap) and it per check interval as
-    // floor quotient and remainder, advanced by one per step instead of
-    // divided each step (the same slots as it / isnap, it % isnap, ...)
-    auto fdiv = [](int a, int b) { const int q = a / b; return (a % b != 0 && a < 0) ? q - 1 : q; };
-    const int it_first = it0 + it_off;
-    int sq = fdiv(it_first, isnap), sr = it_first - sq * isnap;
-    int tq = fdiv(nt - 1 - it_first, isnap), tr = nt - 1 - it_first - tq * isnap;
-    int cq = 0, cr = 0;
-    if (cint > 0) {
-        cq = fdiv(it_first, cint);
-        cr = it_first - cq * cint;
-    }
     bool started = false;  // a step ran in this launch: LDS velocities await the sponge
-    for (int s = 0; s < K; s++) {
-        const int it = it0 + it_off + s;
-        if (s > 0) {
-            if (++sr == isnap) {
-                sr = 0;
-                ++sq;
-            }
-            if (tr-- == 0) {
-                tr = isnap - 1;
-                --tq;
-            }
-            if (cint > 0 && ++cr == cint) {
-                cr = 0;
-                ++cq;
-            }
-        }
-        if (it < it_lo) continue;
-        if (it >= it_end) break;
+    for (int s = s_lo; s < s_hi; s++) {
+        const int it = it_first + s;
         const int tb = nt - 1 - it;  // backward time index
-        // this step's snapshot slot to write (forward) or correlate (backward),
-        // finite-check slot; -1 none
-        int snap = -1, img = -1, chk = -1;
-        if (FWD && save && sr == 0 && it != 0) {
-            const int sl = sq - 1 - snap_base;
-            if (sl >= 0 && sl < num_snaps) snap = sl;
-        }
-        if constexpr (PIPE) {
+        const unsigned bit = 1u << s;
+        // this step's snapshot to write (forward) or correlate (backward) at
+        // slot offset s_off, finite-check index (-1: none)
+        const size_t s_off = (size_t)q_off;
+        if (mult & bit) q_off += sstep;
+        const bool snap = FWD && (omask & bit);
+        bool img = false;
+        if constexpr (PIPE)
             img = img_next;
-        } else if (imaging && tb > 0 && tr == 0) {
-            const int sl = tq - 1 - snap_base;
-            if (sl >= 0 && sl < num_snaps && C(sh->isnaps)[sl] == tb) img = sl;
-        }
-        if (cint > 0 && cr == 0) chk = cq;
-        // this step's injected values of the slots' single-entry groups,
-        // loaded now and added after the velocity update
+        else
+            img = !FWD && (omask & bit);
+        int chk = -1;
+        if (cmask & bit) chk = cn++;
+        // this step's injected values of the slots' single-entry groups, from
+        // the launch's table, added after the velocity update
         constexpr int NG = HOIST ? NC : 1;
         float gvu[NG], gvv[NG], gvw[NG];
         const int col = FWD ? it : tb;
@@ -320,12 +430,12 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
         for (int k = 0; k < NG; k++) {
             gvu[k] = gvv[k] = gvw[k] = 0.0f;
             if (HOIST && gk[k] >= 0) {
-                const size_t o = (size_t)gk[k] * (size_t)nt + col;
-                if (PSV) {
-                    gvu[k] = C(sh->su)[o];
-                    gvw[k] = C(sh->sw)[o];
+                const int o = gk[k] * K + s;
+                if constexpr (HOIST && PSV) {
+                    gvu[k] = tabU[o];
+                    gvw[k] = tabW[o];
                 }
-                if (SH) gvv[k] = C(sh->sv)[o];
+                if constexpr (HOIST && SH) gvv[k] = tabV[o];
             }
         }
         // the injection of slot k into its velocity (after the update, in
@@ -373,7 +483,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
         // read cell 0; only the tile's cells use the value)
         float fimg[SH && !PSV ? NC : 1], fimu[PSV && !SH ? NC : 1], fimw[PSV && !SH ? NC : 1];
         if constexpr (PIPE) {
-            if (img >= 0) {
+            if (img) {
 #pragma unroll
                 for (int k = 0; k < NF; k++) {
                     if constexpr (PSV) {
@@ -385,29 +495,24 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                 }
             }
             // the next step's forward wavefield, in flight during this step
-            // image_slot(it + 1), from tb - 1 = tq' * isnap + tr'
-            img_next = -1;
-            if (s + 1 < K && it + 1 < it_end && imaging && tb - 1 > 0 &&
-                (tr == 0 ? isnap == 1 : tr == 1)) {
-                const int sl = (tr == 0 ? tq - 1 : tq) - 1 - snap_base;
-                if (sl >= 0 && sl < num_snaps && C(sh->isnaps)[sl] == tb - 1) img_next = sl;
-            }
-            if (img_next >= 0) load_forward(img_next);
+            // (its slot offset is q_off, moved past this step)
+            img_next = s + 1 < K && ((omask >> (s + 1)) & 1u);
+            if (img_next) load_forward((size_t)q_off);
         } else if constexpr (SH && !PSV) {
-            if (img >= 0) {
+            if (img) {
 #pragma unroll
                 for (int k = 0; k < NC; k++) {
-                    fimg[k] = snap_v[(size_t)img * sstride + gc[k]];
+                    fimg[k] = snap_v[s_off + gc[k]];
                     if (ssponge) fimg[k] = fimg[k] * A[k];
                 }
             }
         }
         if constexpr (PSV && !SH && !PIPE) {
-            if (img >= 0) {
+            if (img) {
 #pragma unroll
                 for (int k = 0; k < NC; k++) {
-                    fimu[k] = snap_u[(size_t)img * sstride + gc[k]];
-                    fimw[k] = snap_w[(size_t)img * sstride + gc[k]];
+                    fimu[k] = snap_u[s_off + gc[k]];
+                    fimw[k] = snap_w[s_off + gc[k]];
                     if (ssponge) {
                         fimu[k] = fimu[k] * A[k];
                         fimw[k] = fimw[k] * A[k];
@@ -640,8 +745,8 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                 sYZ[e] = syz * A[k];
                 if (fl[k] & 4u) {  // the tile's cell: outputs
                     const float v = sV[e] * A[k];
-                    if (snap >= 0) snap_v[(size_t)snap * sstride + c] = v;
-                    if (img >= 0) aV[k] = aV[k] + fimg[k] * v;
+                    if (snap) snap_v[s_off + c] = v;
+                    if (img) aV[k] = aV[k] + fimg[k] * v;
                     if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
                 }
             }
@@ -684,17 +789,17 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                     sXZ[e] = sxz * A[k];
                     if (own) {
                         const float u = sU[e] * A[k], w = sW[e] * A[k];
-                        if (snap >= 0) {
-                            snap_u[(size_t)snap * sstride + c] = u;
-                            snap_w[(size_t)snap * sstride + c] = w;
+                        if (snap) {
+                            snap_u[s_off + c] = u;
+                            snap_w[s_off + c] = w;
                         }
-                        if (img >= 0) {
+                        if (img) {
                             if constexpr (!SH) {
                                 aU[k] = aU[k] + fimu[k] * u;
                                 aW[k] = aW[k] + fimw[k] * w;
                             } else {
-                                float fu = snap_u[(size_t)img * sstride + c];
-                                float fw = snap_w[(size_t)img * sstride + c];
+                                float fu = snap_u[s_off + c];
+                                float fw = snap_w[s_off + c];
                                 if (ssponge) {
                                     fu = fu * A[k];
                                     fw = fw * A[k];
@@ -727,9 +832,9 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                     sYZ[e] = syz * A[k];
                     if (own) {
                         const float v = sV[e] * A[k];
-                        if (snap >= 0) snap_v[(size_t)snap * sstride + c] = v;
-                        if (img >= 0) {
-                            float fv = snap_v[(size_t)img * sstride + c];
+                        if (snap) snap_v[s_off + c] = v;
+                        if (img) {
+                            float fv = snap_v[s_off + c];
                             if (ssponge) fv = fv * A[k];
                             aV[k] = aV[k] + fv * v;
                         }
@@ -771,15 +876,15 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                 sXZ[e] = sxz * A[k];
                 const float u = sU[e] * A[k], w = sW[e] * A[k];
                 if constexpr (PIPE) {  // the image of a slot outside the tile is never written
-                    if (img >= 0) {
+                    if (img) {
                         aU[k] = aU[k] + fimu[k] * u;
                         aW[k] = aW[k] + fimw[k] * w;
                     }
                 }
                 if (fl[k] & 4u) {
-                    if (snap >= 0) {
-                        snap_u[(size_t)snap * sstride + c] = u;
-                        snap_w[(size_t)snap * sstride + c] = w;
+                    if (snap) {
+                        snap_u[s_off + c] = u;
+                        snap_w[s_off + c] = w;
                     }
                     if (chk >= 0) {
                         if (not_finite_or_big(u) && code < 1) code = 1;
@@ -1206,7 +1311,7 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
     x.check_interval = d->check_interval;
     x.n_groups = ng;
     x.n_rec = nr;
-    x.pad2 = 0;
+    x.snaps_regular = 0;
     x.loc = d->inj_loc;
     x.order = d->inj_order;
     x.start = d->inj_group_start;
@@ -1240,6 +1345,12 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
         e = upload_async(dev, d->isnaps_host, sizeof(int) * d->num_snaps, s);
         if (e != hipSuccess) return fail(TRTM_ERR_HIP, "isnaps copy: %s", hipGetErrorString(e));
         x.isnaps = dev;
+        // the slots of the forward's own schedule: the kernel then needs no
+        // isnaps reads to find the imaging steps
+        bool regular = true;
+        for (int k = 0; k < x.num_snaps && regular; k++)
+            regular = (long long)d->isnaps_host[k] == (long long)(k + 1 + x.snap_base) * x.isnap;
+        x.snaps_regular = regular ? 1 : 0;
     }
     x.image_u = d->image_u;
     x.image_v = d->image_v;

@@ -12,7 +12,7 @@ struct ShotDev {
     float* out[8];  // written at launch end
     int nt, it_end, save, isnap;
     int num_snaps, snap_base, check_interval, n_groups;
-    int n_rec, it_lo, it_off, pad2;
+    int n_rec, it_lo, it_off, snaps_regular;  // isnaps[k] == (k + 1 + snap_base) * isnap
     const int *loc, *order, *start;
     const float *su, *sv, *sw;
     const int* rec_loc;
