@@ -223,13 +223,11 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
     constexpr int kNone = -2147483647 - 1;
     int i_e = -1, i_code = kNone;  // this thread's group: region cell, code
     if (HOIST && inj_one_pass && ig0 + t < ig1) {
-        const int g = C(sh->inj_titems)[ig0 + t];
-        const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
-        const int k0 = C(sh->order)[q0];
-        const int ci = C(sh->loc)[2 * k0], cj = C(sh->loc)[2 * k0 + 1];
-        i_e = (ci - ri0) * RZ + (cj - rj0);
-        i_code = q1 - q0 == 1 ? t : -1 - g;
-        if (q1 - q0 == 1) {  // the launch's K injected values of the entry
+        // the group's region cell and data row (single entry) or -1 - g
+        const int k0 = C(sh->inj_tcells)[2 * (ig0 + t) + 1];
+        i_e = C(sh->inj_tcells)[2 * (ig0 + t)];
+        i_code = k0 >= 0 ? t : k0;
+        if (k0 >= 0) {  // the launch's K injected values of the entry
             // (loaded together: the time index clamped to [0, nt), steps
             // outside the shot's range never read theirs)
             float vu[K], vw[K], vv[K];
@@ -265,10 +263,9 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
     int r_l = -1, r_e = 0;  // this thread's receiver
     float r_A = 0.0f;
     if (rec_one_pass && rb0 + t < rb1) {
-        r_l = C(sh->rec_titems)[rb0 + t];
-        const int ci = C(sh->rec_loc)[2 * r_l], cj = C(sh->rec_loc)[2 * r_l + 1];
-        r_e = (ci - ri0) * RZ + (cj - rj0);
-        r_A = sponge_at(ci, cj, nx, nz, m.fw, m.tab);
+        r_l = C(sh->rec_tcells)[2 * (rb0 + t)];
+        r_e = C(sh->rec_tcells)[2 * (rb0 + t) + 1];
+        r_A = sponge_at(ri0 + r_e / RZ, rj0 + r_e % RZ, nx, nz, m.fw, m.tab);
     }
 
     const int save = sh->save, isnap = sh->isnap, num_snaps = sh->num_snaps;
@@ -276,6 +273,41 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
     const gptr<float> snap_u = G(sh->snap_u), snap_v = G(sh->snap_v), snap_w = G(sh->snap_w);
     const size_t sstride = (size_t)sh->snap_stride;
     const bool ssponge = sh->snap_sponge != 0;
+
+    // The small shapes with one wave system load the forward wavefield of the
+    // image a whole step before the image update uses it (the others a phase
+    // before): pf0 / pf1 hold the next step's u / w (or v) of every slot.
+    constexpr bool PIPE = HOIST && (PSV != SH);
+    constexpr int NF = PIPE ? NC : 1;
+    float pf0[NF], pf1[NF];
+    bool img_next = false;
+    auto load_forward = [&](size_t off) {
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+            if (PSV) {
+                pf0[k] = snap_u[off + gc[k]];
+                pf1[k] = snap_w[off + gc[k]];
+            } else {
+                pf0[k] = snap_v[off + gc[k]];
+            }
+        }
+    };
+    const int s_lo = it_lo > it_first ? it_lo - it_first : 0;
+    const int s_hi = it_end - it_first < K ? it_end - it_first : K;
+    // backward with a snapshot every step in the forward's own slots: the
+    // first step's forward wavefield (the slot the schedule below finds) is
+    // requested before the barrier
+    bool pre = false;
+    if constexpr (PIPE && !FWD) {
+        if (imaging && isnap == 1 && sh->snaps_regular) {
+            const int T = nt - 1 - (it_first + s_lo);
+            const int sl = T - 1 - snap_base;
+            if (T > 0 && sl >= 0 && sl < num_snaps) {
+                pre = true;
+                load_forward((size_t)sl * sstride);
+            }
+        }
+    }
 
     // The launch's step schedule, worked out once by wave 0 (while the loads
     // are in flight) and handed to the others through LDS: the launch runs
@@ -288,8 +320,6 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
     // step the slot offset and the check index then follow from bit tests and
     // running counters, with no division and no table read.
     __shared__ int sched[8];
-    const int s_lo = it_lo > it_first ? it_lo - it_first : 0;
-    const int s_hi = it_end - it_first < K ? it_end - it_first : K;
     if (t < 64) {
         const int it_start = it_first + s_lo;                 // >= it_lo >= 0, < it_end <= nt
         const int T = FWD ? it_start : nt - 1 - it_start;  // >= 0
@@ -379,29 +409,9 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
 
     const long long sstep = FWD ? (long long)sstride : -(long long)sstride;
 
-    // The small shapes with one wave system load the forward wavefield of the
-    // image a whole step before the image update uses it (the others a phase
-    // before): pf0 / pf1 hold the next step's u / w (or v) of every slot.
-    constexpr bool PIPE = HOIST && (PSV != SH);
-    constexpr int NF = PIPE ? NC : 1;
-    float pf0[NF], pf1[NF];
-    bool img_next = false;
-    auto load_forward = [&](size_t off) {
-#pragma unroll
-        for (int k = 0; k < NF; k++) {
-            if (PSV) {
-                pf0[k] = snap_u[off + gc[k]];
-                pf1[k] = snap_w[off + gc[k]];
-            } else {
-                pf0[k] = snap_v[off + gc[k]];
-            }
-        }
-    };
     if constexpr (PIPE) {
-        if ((omask >> s_lo) & 1u) {
-            img_next = true;
-            load_forward((size_t)q_off);
-        }
+        img_next = (omask >> s_lo) & 1u;
+        if (img_next && !pre) load_forward((size_t)q_off);
     }
 
     bool started = false;  // a step ran in this launch: LDS velocities await the sponge
@@ -1052,7 +1062,8 @@ static size_t align256b(size_t x) { return (x + 255) & ~size_t(255); }
 // compiled configuration of both kernel families, device copy of isnaps, two
 // shot-table entries
 struct BlockLayout {
-    size_t alt, inj_start, inj_cursor, inj_items, rec_start, rec_cursor, rec_items, maps, isnaps,
+    size_t alt, inj_start, inj_cursor, inj_items, inj_cells, rec_start, rec_cursor, rec_items,
+        rec_cells, maps, isnaps,
         shot, total;
     int tiles_x, tiles_z;
 };
@@ -1095,9 +1106,11 @@ static BlockLayout block_layout(const trtm_step_desc* d, BlockCfg c) {
     L.inj_start = off;  off = align256b(off + ib * sizeof(int));
     L.inj_cursor = off; off = align256b(off + ib * sizeof(int));
     L.inj_items = off;  off = align256b(off + ii);
+    L.inj_cells = off;  off = align256b(off + 2 * ii);
     L.rec_start = off;  off = align256b(off + rb * sizeof(int));
     L.rec_cursor = off; off = align256b(off + rb * sizeof(int));
     L.rec_items = off;  off = align256b(off + ri);
+    L.rec_cells = off;  off = align256b(off + 2 * ri);
     L.maps = off;       off = align256b(off + mb);
     L.isnaps = off;     off = align256b(off + (size_t)std::max(d->num_snaps, 0) * sizeof(int));
     L.shot = off;       off = align256b(off + 2 * sizeof(ShotDev));
@@ -1111,10 +1124,14 @@ int64_t block_workspace_bytes(const trtm_step_desc* d) {
 
 // mode 0 counts, mode 1 fills.  Groups go to every tile whose region (tile +
 // halo K) holds their cell; receivers to the tile that holds their cell.
+// Mode 1 also writes per placed item its region cell e (row-major in the
+// tile's region, RZ = tz + 2K per row) with, for groups, the data row of a
+// single entry or -1 - g (icells), for receivers the receiver index (rcells).
 __global__ void k_block_lists(int mode, int tx, int tz, int K, int tiles_x, int tiles_z,
                               int n_groups, const int* loc, const int* order, const int* start,
-                              int* icount, const int* istart, int* iitems, int n_rec,
-                              const int* rloc, int* rcount, const int* rstart, int* ritems) {
+                              int* icount, const int* istart, int* iitems, int* icells, int n_rec,
+                              const int* rloc, int* rcount, const int* rstart, int* ritems,
+                              int* rcells) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n_groups) {
         const int k0 = order[start[q]];
@@ -1127,13 +1144,24 @@ __global__ void k_block_lists(int mode, int tx, int tz, int K, int tiles_x, int 
                     continue;
                 const int tile = ti * tiles_z + tk;
                 const int pos = atomicAdd(icount + tile, 1);
-                if (mode == 1) iitems[istart[tile] + pos] = q;
+                if (mode == 1) {
+                    const int w = istart[tile] + pos;
+                    iitems[w] = q;
+                    icells[2 * w] = (ci - ri0) * (tz + 2 * K) + (cj - rj0);
+                    icells[2 * w + 1] = start[q + 1] - start[q] == 1 ? k0 : -1 - q;
+                }
             }
     }
     if (q < n_rec) {
-        const int tile = (rloc[2 * q] / tx) * tiles_z + rloc[2 * q + 1] / tz;
+        const int ti = rloc[2 * q] / tx, tk = rloc[2 * q + 1] / tz;
+        const int tile = ti * tiles_z + tk;
         const int pos = atomicAdd(rcount + tile, 1);
-        if (mode == 1) ritems[rstart[tile] + pos] = q;
+        if (mode == 1) {
+            const int w = rstart[tile] + pos;
+            ritems[w] = q;
+            rcells[2 * w] = q;
+            rcells[2 * w + 1] = (rloc[2 * q] - (ti * tx - K)) * (tz + 2 * K) + (rloc[2 * q + 1] - (tk * tz - K));
+        }
     }
 }
 
@@ -1146,9 +1174,9 @@ __global__ __launch_bounds__(1024) void k_block_lists_one(int tx, int tz, int K,
                                                           int tiles_z, int n_groups,
                                                           const int* loc, const int* order,
                                                           const int* start, int* istart_g,
-                                                          int* iitems, int n_rec,
+                                                          int* iitems, int* icells, int n_rec,
                                                           const int* rloc, int* rstart_g,
-                                                          int* ritems) {
+                                                          int* ritems, int* rcells) {
     extern __shared__ int lds[];
     __shared__ int part[1024];
     const int t = threadIdx.x, tiles = tiles_x * tiles_z;
@@ -1175,13 +1203,25 @@ __global__ __launch_bounds__(1024) void k_block_lists_one(int tx, int tz, int K,
                             continue;
                         const int tile = ti * tiles_z + tk;
                         const int pos = atomicAdd(icnt + tile, 1);
-                        if (mode == 1) iitems[ist[tile] + pos] = q;
+                        if (mode == 1) {
+                            const int w = ist[tile] + pos;
+                            iitems[w] = q;
+                            icells[2 * w] = (ci - ri0) * (tz + 2 * K) + (cj - rj0);
+                            icells[2 * w + 1] = start[q + 1] - start[q] == 1 ? k0 : -1 - q;
+                        }
                     }
             }
             if (q < n_rec) {
-                const int tile = (rloc[2 * q] / tx) * tiles_z + rloc[2 * q + 1] / tz;
+                const int ti = rloc[2 * q] / tx, tk = rloc[2 * q + 1] / tz;
+                const int tile = ti * tiles_z + tk;
                 const int pos = atomicAdd(rcnt + tile, 1);
-                if (mode == 1) ritems[rst[tile] + pos] = q;
+                if (mode == 1) {
+                    const int w = rst[tile] + pos;
+                    ritems[w] = q;
+                    rcells[2 * w] = q;
+                    rcells[2 * w + 1] = (rloc[2 * q] - (ti * tx - K)) * (tz + 2 * K) +
+                                        (rloc[2 * q + 1] - (tk * tz - K));
+                }
             }
         }
         __syncthreads();
@@ -1253,6 +1293,8 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
     int* rstart = reinterpret_cast<int*>(ws + L.rec_start);
     int* rcur = reinterpret_cast<int*>(ws + L.rec_cursor);
     int* ritems = reinterpret_cast<int*>(ws + L.rec_items);
+    int* icells = reinterpret_cast<int*>(ws + L.inj_cells);
+    int* rcells = reinterpret_cast<int*>(ws + L.rec_cells);
     const int ng = d->n_inj_groups;
     const int nr = (d->rec_u || d->rec_v) ? d->n_rec : 0;
     hipError_t e;
@@ -1264,7 +1306,8 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
         const size_t lds = (4 * (size_t)tiles + 2) * sizeof(int);
         hipLaunchKernelGGL(k_block_lists_one, dim3(1), dim3(1024), lds, s, tx, cfg.tz, K,
                            L.tiles_x, L.tiles_z, ng, d->inj_loc, d->inj_order,
-                           d->inj_group_start, istart, iitems, nr, d->rec_loc, rstart, ritems);
+                           d->inj_group_start, istart, iitems, icells, nr, d->rec_loc, rstart,
+                           ritems, rcells);
         e = hipGetLastError();
         if (e != hipSuccess)
             return fail(TRTM_ERR_HIP, "blocked tile lists: %s", hipGetErrorString(e));
@@ -1275,8 +1318,8 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
         if (ng > 0 || nr > 0)
             hipLaunchKernelGGL(k_block_lists, lg, tb, 0, s, 0, tx, cfg.tz, K, L.tiles_x, L.tiles_z,
                                ng, d->inj_loc, d->inj_order, d->inj_group_start, icur,
-                               (const int*)nullptr, (int*)nullptr, nr, d->rec_loc, rcur,
-                               (const int*)nullptr, (int*)nullptr);
+                               (const int*)nullptr, (int*)nullptr, (int*)nullptr, nr, d->rec_loc,
+                               rcur, (const int*)nullptr, (int*)nullptr, (int*)nullptr);
         hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, s, icur, istart, tiles);
         hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, s, rcur, rstart, tiles);
         hipLaunchKernelGGL(k_fill_int_b, tg, tb, 0, s, icur, tiles, 0);
@@ -1284,8 +1327,8 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
         if (ng > 0 || nr > 0)
             hipLaunchKernelGGL(k_block_lists, lg, tb, 0, s, 1, tx, cfg.tz, K, L.tiles_x,
                                L.tiles_z, ng, d->inj_loc, d->inj_order, d->inj_group_start, icur,
-                               (const int*)istart, iitems, nr, d->rec_loc, rcur,
-                               (const int*)rstart, ritems);
+                               (const int*)istart, iitems, icells, nr, d->rec_loc, rcur,
+                               (const int*)rstart, ritems, rcells);
         e = hipGetLastError();
         if (e != hipSuccess)
             return fail(TRTM_ERR_HIP, "blocked tile lists: %s", hipGetErrorString(e));
@@ -1326,6 +1369,8 @@ static int build_shot(const trtm_step_desc* d, bool fwd, bool save, BlockCfg cfg
     x.inj_titems = iitems;
     x.rec_tstart = rstart;
     x.rec_titems = ritems;
+    x.inj_tcells = icells;
+    x.rec_tcells = rcells;
     x.imap_slot = x.imap = x.rmap_slot = x.rmap = nullptr;
     if (is_wave(cfg)) {
         const WaveMaps wm = wave_maps(d, cfg, ws + L.maps);

@@ -18,6 +18,8 @@ struct ShotDev {
     const int* rec_loc;
     float *rec_u, *rec_v, *rec_w;
     const int *inj_tstart, *inj_titems, *rec_tstart, *rec_titems;
+    // per list item (blocked kernel): region cell and group code / receiver
+    const int *inj_tcells, *rec_tcells;
     // wavefront kernel: per-tile slot (-1: none) of a per-lane map of the
     // single-entry injection cells (wavelet row) and of the receivers (index)
     const int *imap_slot, *imap, *rmap_slot, *rmap;
