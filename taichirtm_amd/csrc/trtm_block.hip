@@ -194,80 +194,6 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
         }
     }
 
-    // The tile's injection groups and receivers, looked up once per launch
-    // while the region's loads are in flight (lists no longer than kInjTab
-    // groups / the workgroup's receivers; longer lists keep the per-step
-    // loops below): each region cell's thread slot learns
-    // its group (a row of the table of injected values, or the group of
-    // several entries) and adds the injection right after its own velocity
-    // update, with no separate injection phase and barrier; thread t records
-    // receiver rb0 + t.  Only for shapes with at most two cells per thread:
-    // their few workgroups per CU leave per-step global loads' latency
-    // exposed, and they have registers to spare; the larger shapes keep the
-    // per-step lookups.
-    const int n_groups = sh->n_groups, n_rec = sh->n_rec;
-    int ig0 = 0, ig1 = 0, rb0 = 0, rb1 = 0;
-    if (n_groups > 0) {
-        const cptr<int> tstart = C(sh->inj_tstart);
-        ig0 = tstart[tile];
-        ig1 = tstart[tile + 1];
-    }
-    if (n_rec > 0) {
-        const cptr<int> rstart = C(sh->rec_tstart);
-        rb0 = rstart[tile];
-        rb1 = rstart[tile + 1];
-    }
-    const bool inj_one_pass = HOIST && ig1 - ig0 <= kInjTab, rec_one_pass = HOIST && rb1 - rb0 <= NT;
-    // per slot: the table row of its cell's single-entry group, -1 - g for a
-    // group g of several entries (summed per step in entry order), or kNone
-    constexpr int kNone = -2147483647 - 1;
-    int i_e = -1, i_code = kNone;  // this thread's group: region cell, code
-    if (HOIST && inj_one_pass && ig0 + t < ig1) {
-        // the group's region cell and data row (single entry) or -1 - g
-        const int k0 = C(sh->inj_tcells)[2 * (ig0 + t) + 1];
-        i_e = C(sh->inj_tcells)[2 * (ig0 + t)];
-        i_code = k0 >= 0 ? t : k0;
-        if (k0 >= 0) {  // the launch's K injected values of the entry
-            // (loaded together: the time index clamped to [0, nt), steps
-            // outside the shot's range never read theirs)
-            float vu[K], vw[K], vv[K];
-#pragma unroll
-            for (int s = 0; s < K; s++) {
-                int col_ = FWD ? it_first + s : nt - 1 - (it_first + s);
-                col_ = col_ < 0 ? 0 : (col_ >= nt ? nt - 1 : col_);
-                const size_t o = (size_t)k0 * (size_t)nt + col_;
-                vu[s] = vw[s] = vv[s] = 0.0f;
-                if (PSV) {
-                    vu[s] = C(sh->su)[o];
-                    vw[s] = C(sh->sw)[o];
-                }
-                if (SH) vv[s] = C(sh->sv)[o];
-            }
-#pragma unroll
-            for (int s = 0; s < K; s++) {
-                if constexpr (HOIST && PSV) {
-                    tabU[t * K + s] = vu[s];
-                    tabW[t * K + s] = vw[s];
-                }
-                if constexpr (HOIST && SH) tabV[t * K + s] = vv[s];
-            }
-        }
-    }
-    if constexpr (HOIST) {
-        if (inj_one_pass && ig1 > ig0) {  // uniform
-#pragma unroll
-            for (int k = 0; k < NC; k++)
-                if (t + NT * k < RN) gmap[t + NT * k] = kNone;
-        }
-    }
-    int r_l = -1, r_e = 0;  // this thread's receiver
-    float r_A = 0.0f;
-    if (rec_one_pass && rb0 + t < rb1) {
-        r_l = C(sh->rec_tcells)[2 * (rb0 + t)];
-        r_e = C(sh->rec_tcells)[2 * (rb0 + t) + 1];
-        r_A = sponge_at(ri0 + r_e / RZ, rj0 + r_e % RZ, nx, nz, m.fw, m.tab);
-    }
-
     const int save = sh->save, isnap = sh->isnap, num_snaps = sh->num_snaps;
     const int snap_base = sh->snap_base, cint = sh->check_interval;
     const gptr<float> snap_u = G(sh->snap_u), snap_v = G(sh->snap_v), snap_w = G(sh->snap_w);
@@ -308,6 +234,101 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
             }
         }
     }
+
+    // The tile's injection groups and receivers, looked up once per launch
+    // while the region's loads are in flight (lists no longer than kInjTab
+    // groups / the workgroup's receivers; longer lists keep the per-step
+    // loops below): each region cell's thread slot learns
+    // its group (a row of the table of injected values, or the group of
+    // several entries) and adds the injection right after its own velocity
+    // update, with no separate injection phase and barrier; thread t records
+    // receiver rb0 + t.  Only for shapes with at most two cells per thread:
+    // their few workgroups per CU leave per-step global loads' latency
+    // exposed, and they have registers to spare; the larger shapes keep the
+    // per-step lookups.
+    const int n_groups = sh->n_groups, n_rec = sh->n_rec;
+    int ig0 = 0, ig1 = 0, rb0 = 0, rb1 = 0;
+    if (n_groups > 0) {
+        const cptr<int> tstart = C(sh->inj_tstart);
+        ig0 = tstart[tile];
+        ig1 = tstart[tile + 1];
+    }
+    if (n_rec > 0) {
+        const cptr<int> rstart = C(sh->rec_tstart);
+        rb0 = rstart[tile];
+        rb1 = rstart[tile + 1];
+    }
+    const bool inj_one_pass = HOIST && ig1 - ig0 <= kInjTab, rec_one_pass = HOIST && rb1 - rb0 <= NT;
+    // per slot: the table row of its cell's single-entry group, -1 - g for a
+    // group g of several entries (summed per step in entry order), or kNone
+    constexpr int kNone = -2147483647 - 1;
+    // Wave w sets up items w, w + NW, ... of the list; their cells and codes
+    // with scalar loads, whose latency runs on the scalar counter beside the
+    // region's vector loads (indexes clamped into the list; the extra items
+    // fill table rows no slot reads).  An item's region cell and code go to
+    // the region map after the first barrier; a single entry's K injected
+    // values go to the item's row of the table, loaded by lanes 0 .. IPW K - 1
+    // of the wave (time index clamped to [0, nt) at the ends of the record:
+    // steps outside the shot's range never read theirs).
+    constexpr int NWV = NT / 64, IPW = (kInjTab + NWV - 1) / NWV;
+    static_assert(!HOIST || IPW * K <= 64, "one lane per injected value of a wave's items");
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6), ln = t & 63;
+    int q_e[HOIST ? IPW : 1], q_code[HOIST ? IPW : 1];
+    if constexpr (HOIST) {
+        const int n_items = inj_one_pass ? ig1 - ig0 : 0;
+        int k0s[IPW];
+#pragma unroll
+        for (int m2 = 0; m2 < IPW; m2++) {
+            q_e[m2] = -1;
+            q_code[m2] = kNone;
+        }
+        if (n_items > 0) {  // uniform
+            const cptr<int> cells = C(sh->inj_tcells);
+            int ce_e[IPW], ce_k[IPW];
+#pragma unroll
+            for (int m2 = 0; m2 < IPW; m2++) {
+                const int w2 = 2 * (ig0 + min(wv + NWV * m2, n_items - 1));
+                ce_e[m2] = cells[w2];
+                ce_k[m2] = cells[w2 + 1];
+            }
+#pragma unroll
+            for (int m2 = 0; m2 < IPW; m2++) {
+                const int q = wv + NWV * m2;
+                k0s[m2] = max(ce_k[m2], 0);
+                q_e[m2] = q < n_items ? ce_e[m2] : -1;
+                q_code[m2] = ce_k[m2] >= 0 ? q : ce_k[m2];
+            }
+            // lane l < IPW K loads value s = l % K of the wave's item l / K
+            if (ln < IPW * K) {
+                const int m2 = ln / K, s = ln - m2 * K;
+                int k0 = k0s[0];
+#pragma unroll
+                for (int m3 = 1; m3 < IPW; m3++) k0 = m2 == m3 ? k0s[m3] : k0;
+                const int col_ = min(max(FWD ? it_first + s : nt - 1 - (it_first + s), 0), nt - 1);
+                const size_t o = (size_t)k0 * (size_t)nt + col_;
+                const int r = (wv + NWV * m2) * K + s;
+                if constexpr (PSV) {
+                    tabU[r] = G(sh->su)[o];
+                    tabW[r] = G(sh->sw)[o];
+                }
+                if constexpr (SH) tabV[r] = G(sh->sv)[o];
+            }
+        }
+    }
+    if constexpr (HOIST) {
+        if (inj_one_pass && ig1 > ig0) {  // uniform
+#pragma unroll
+            for (int k = 0; k < NC; k++)
+                if (t + NT * k < RN) gmap[t + NT * k] = kNone;
+        }
+    }
+    int r_l = -1, r_e = 0;  // this thread's receiver (sponge factor after the barrier)
+    float r_A = 0.0f;
+    if (rec_one_pass && rb0 + t < rb1) {
+        r_l = C(sh->rec_tcells)[2 * (rb0 + t)];
+        r_e = C(sh->rec_tcells)[2 * (rb0 + t) + 1];
+    }
+
 
     // The launch's step schedule, worked out once by wave 0 (while the loads
     // are in flight) and handed to the others through LDS: the launch runs
@@ -382,6 +403,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
         if constexpr (PSV) asm volatile("" : "+v"(c0[k]), "+v"(c1[k]), "+v"(c2[k]), "+v"(c3[k]), "+v"(c4[k]));
         if constexpr (SH) asm volatile("" : "+v"(d0[k]), "+v"(d1[k]), "+v"(d2[k]));
     }
+    if (r_l >= 0) r_A = sponge_at(ri0 + r_e / RZ, rj0 + r_e % RZ, nx, nz, m.fw, m.tab);
     const unsigned mult = (unsigned)__builtin_amdgcn_readfirstlane(sched[0]);
     const unsigned omask = (unsigned)__builtin_amdgcn_readfirstlane(sched[1]);
     const unsigned cmask = (unsigned)__builtin_amdgcn_readfirstlane(sched[2]);
@@ -399,7 +421,9 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
     for (int k = 0; k < (HOIST ? NC : 1); k++) gk[k] = kNone;
     if constexpr (HOIST) {
         if (inj_one_pass && ig1 > ig0) {  // uniform
-            if (i_e >= 0) gmap[i_e] = i_code;
+#pragma unroll
+            for (int m2 = 0; m2 < IPW; m2++)
+                if (q_e[m2] >= 0 && ln == 0) gmap[q_e[m2]] = q_code[m2];
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < NC; k++)

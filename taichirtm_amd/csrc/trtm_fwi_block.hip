@@ -294,10 +294,26 @@ k_fwi_fwd_blk(S src, int t_a, int t_b, int tiles_x, int tiles_z) {
     const gptr<const float> wu = G(a.wav_u), wv = G(a.wav_v), ww = G(a.wav_w);
     const gptr<float> ru = G(a.rec_u), rv = G(a.rec_v), rw = G(a.rec_w);
 
+    // compact history: floor quotient and remainder of the step's slot
+    // (it - hist_base) by hist_every, advanced by one per step
+    int hq = 0, hr = 0;
+    if (HIST && a.hist_every > 0) {
+        const int s0 = t_a - a.hist_base;
+        hq = s0 / a.hist_every;
+        hr = s0 - hq * a.hist_every;
+        if (hr < 0) {
+            hr += a.hist_every;
+            hq--;
+        }
+    }
     bool started = false;  // a step ran: LDS velocities await the sponge
     for (int s = 0; s < K; s++) {
         const int it = t_a + s;
         if (it >= hi) break;
+        if (HIST && s > 0 && ++hr == a.hist_every) {
+            hr = 0;
+            ++hq;
+        }
         // this step's history: stress fields (hs) and velocity fields (hv)
         float *hs = nullptr, *hv = nullptr;
         if (HIST) {
@@ -305,9 +321,9 @@ k_fwi_fwd_blk(S src, int t_a, int t_b, int tiles_x, int tiles_z) {
             if (slot >= 0 && slot < a.hist_len) {
                 if (a.hist_every > 0) {
                     hv = a.hist + (size_t)slot * (HC<PSV, SH>::nv * n);
-                    if (slot % a.hist_every == 0)
+                    if (hr == 0)
                         hs = a.hist + ((size_t)a.hist_len * HC<PSV, SH>::nv +
-                                       (size_t)(slot / a.hist_every) * HC<PSV, SH>::ns) * n;
+                                       (size_t)hq * HC<PSV, SH>::ns) * n;
                 } else {
                     hs = hv = a.hist + (size_t)slot * (H::n * n);
                 }
