@@ -47,10 +47,10 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
     // every thread slot: without a surface matrix the updates run branch-free
     // on every slot (neighbour reads of edge and spare slots stay inside the
     // array) and the result is selected per cell.  SH-only kernels and the
-    // P-SV-only shapes with at most two cells per thread run them; the other
+    // P-SV-only shapes with at most four cells per thread run them; the other
     // P-SV kernels keep per-cell branches (their registers are the tighter
     // limit).
-    constexpr bool BFP = PSV && !SH && NC <= 2;  // branch-free P-SV updates
+    constexpr bool BFP = PSV && !SH && NC <= 4;  // branch-free P-SV updates
     constexpr int PAD = (!PSV || BFP) ? RZ + 1 : 0;
     constexpr int NA = PAD == 0 ? RN : (RN > NT * NC ? RN : NT * NC) + 2 * PAD;
     constexpr int NP = PSV ? NA : 1, NS = SH ? NA : 1;
@@ -652,7 +652,8 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                 }
             }
         };
-        // P-SV, small shapes, flat surface: branch-free over every slot
+        // P-SV, up to four cells per thread, flat surface: branch-free over
+        // every slot
         auto velocity_psv_bf = [&]() {
 #pragma unroll
             for (int k = 0; k < NC; k++) {
@@ -877,8 +878,9 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                 }
             }
         };
-        // P-SV, small shapes, flat surface: branch-free over every slot; the
-        // outputs (snapshots, finite check) only for the tile's cells
+        // P-SV, up to four cells per thread, flat surface: branch-free over
+        // every slot; the outputs (snapshots, image, finite check) only for the
+        // tile's cells
         auto stress_psv_bf = [&]() {
 #pragma unroll
             for (int k = 0; k < NC; k++) {
@@ -916,6 +918,12 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                     }
                 }
                 if (fl[k] & 4u) {
+                    if constexpr (!PIPE) {
+                        if (img) {
+                            aU[k] = aU[k] + fimu[k] * u;
+                            aW[k] = aW[k] + fimw[k] * w;
+                        }
+                    }
                     if (snap) {
                         snap_u[s_off + c] = u;
                         snap_w[s_off + c] = w;
