@@ -1024,8 +1024,8 @@ static const BlockCfg kCfgSH[] = {{16, 64, 256, 4},   {16, 64, 256, 8},    {16, 
                                   {16, 32, 1024, 8},  {16, 16, 1024, 8}};
 static const BlockCfg kCfgPSV[] = {{16, 64, 256, 2},  {16, 64, 256, 4},    {16, 128, 512, 4},
                                    {16, 128, 512, 8}, {32, 128, 1024, 4},  {48, 64, 1024, 4},
-                                   {32, 64, 512, 4},  {16, 32, 1024, 4},   {16, 32, 1024, 8},
-                                   {16, 16, 1024, 8}};
+                                   {32, 64, 512, 4},  {24, 56, 512, 4},    {16, 32, 1024, 4},
+                                   {16, 32, 1024, 8}, {16, 16, 1024, 8}};
 static const BlockCfg kCfgBoth[] = {{16, 64, 256, 2}, {16, 64, 256, 4}, {16, 128, 512, 4},
                                     {16, 32, 1024, 4}};
 // (the 16 x 32 and 16 x 16 tiles with 1024 threads give every thread one or
@@ -1459,8 +1459,8 @@ static bool launch_sys(BlockCfg c, const KArgs& m, const ShotDev* shots, int n_s
     } else if constexpr (PSV) {
         TRTM_CFG(16, 64, 256, 2) TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 128, 512, 4)
         TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4) TRTM_CFG(48, 64, 1024, 4)
-        TRTM_CFG(32, 64, 512, 4) TRTM_CFG(16, 32, 1024, 4) TRTM_CFG(16, 32, 1024, 8)
-        TRTM_CFG(16, 16, 1024, 8)
+        TRTM_CFG(32, 64, 512, 4) TRTM_CFG(24, 56, 512, 4) TRTM_CFG(16, 32, 1024, 4)
+        TRTM_CFG(16, 32, 1024, 8) TRTM_CFG(16, 16, 1024, 8)
     } else {
         TRTM_CFG(16, 64, 256, 4) TRTM_CFG(16, 64, 256, 8) TRTM_CFG(16, 128, 512, 4)
         TRTM_CFG(16, 128, 512, 8) TRTM_CFG(32, 128, 1024, 4) TRTM_CFG(32, 128, 1024, 8)

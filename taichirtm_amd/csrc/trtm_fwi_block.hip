@@ -306,6 +306,16 @@ k_fwi_fwd_blk(S src, int t_a, int t_b, int tiles_x, int tiles_z) {
             hq--;
         }
     }
+    // this wave holds a source or a receiver cell (wave-uniform tests ahead
+    // of the per-cell ones)
+    bool any_src = false, any_rec = false;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        any_src = any_src || sg[k] >= 0;
+        any_rec = any_rec || rg[k] >= 0;
+    }
+    const bool w_src = __builtin_amdgcn_ballot_w64(any_src) != 0;
+    const bool w_rec = __builtin_amdgcn_ballot_w64(any_rec) != 0;
     bool started = false;  // a step ran: LDS velocities await the sponge
     for (int s = 0; s < K; s++) {
         const int it = t_a + s;
@@ -351,11 +361,13 @@ k_fwi_fwd_blk(S src, int t_a, int t_b, int tiles_x, int tiles_z) {
                 }
             }
         }
-        // ---- velocity update + source injection (region)
+        // ---- velocity update + source injection (region); a slot outside
+        // the grid holds 0 in LDS and keeps it (no update, no source, sponge
+        // 1), so only spare slots past the region skip
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            if (!(fl[k] & 1u)) continue;
             const int e = t + NT * k;
+            if (NT * (k + 1) > RN && e >= RN) continue;
             const bool upd = (fl[k] & 2u) && (fl[k] & 8u);
             if (PSV) {
                 float u = sU[e], w = sW[e];
@@ -371,7 +383,7 @@ k_fwi_fwd_blk(S src, int t_a, int t_b, int tiles_x, int tiles_z) {
                     u = u + ((sxx_x + sxz_z) * dt) * c0[k];
                     w = w + ((sxz_x + szz_z) * dt) * c1[k];
                 }
-                if (sg[k] >= 0)
+                if (w_src && sg[k] >= 0)
                     for (int q = sstart[sg[k]]; q < sstart[sg[k] + 1]; q++) {
                         const size_t o = (size_t)sorder[q] * nt + it;
                         u = u + (wu[o] * factor) * c0[k];
@@ -392,7 +404,7 @@ k_fwi_fwd_blk(S src, int t_a, int t_b, int tiles_x, int tiles_z) {
                     const float syz_z = (sYZ[e] - sYZ[e - 1]) * idz;
                     v = v + ((syx_x + syz_z) * dt) * d0[k];
                 }
-                if (sg[k] >= 0)
+                if (w_src && sg[k] >= 0)
                     for (int q = sstart[sg[k]]; q < sstart[sg[k] + 1]; q++)
                         v = v + (wv[(size_t)sorder[q] * nt + it] * factor) * d0[k];
                 sV[e] = v;
@@ -401,11 +413,12 @@ k_fwi_fwd_blk(S src, int t_a, int t_b, int tiles_x, int tiles_z) {
         }
         __syncthreads();
 
-        // ---- stress update + sponge (region), records (tile)
+        // ---- stress update + sponge (region), records (tile); slots outside
+        // the grid keep their 0 as above
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            if (!(fl[k] & 1u)) continue;
             const int e = t + NT * k;
+            if (NT * (k + 1) > RN && e >= RN) continue;
             const bool upd = (fl[k] & 2u) && (fl[k] & 16u);
             if (PSV) {
                 float sxx = sXX[e], szz = sZZ[e], sxz = sXZ[e];
@@ -434,7 +447,7 @@ k_fwi_fwd_blk(S src, int t_a, int t_b, int tiles_x, int tiles_z) {
                 sYX[e] = syx * A[k];
                 sYZ[e] = syz * A[k];
             }
-            if (rg[k] >= 0) {
+            if (w_rec && rg[k] >= 0) {
                 const size_t row = (size_t)it * a.n_rec;
                 for (int q = rstart[rg[k]]; q < rstart[rg[k] + 1]; q++) {
                     const int l = rorder[q];

@@ -334,7 +334,8 @@ def test_sh_layered_full_length_image_error_against_float64():
 
 
 BLOCK_CFGS = {"psv": [(16, 64, 2), (16, 64, 4), (16, 128, 4), (16, 128, 8), (32, 128, 4),
-                      (48, 64, 4), (32, 64, 4), (16, 32, 4), (16, 32, 8), (16, 16, 8)],
+                      (48, 64, 4), (32, 64, 4), (24, 56, 4), (16, 32, 4), (16, 32, 8),
+                      (16, 16, 8)],
               "sh": [(16, 64, 4), (16, 64, 8), (16, 128, 4), (16, 128, 8), (32, 128, 4),
                      (32, 128, 8), (24, 128, 8), (32, 96, 8), (48, 64, 8), (32, 48, 8),
                      (24, 64, 8), (16, 32, 4), (16, 32, 8), (16, 16, 8)],
