@@ -1032,7 +1032,10 @@ static const BlockCfg kCfgBoth[] = {{16, 64, 256, 2}, {16, 64, 256, 4}, {16, 128
 // two region cells: the shortest step on grids too small to fill the GPU; 16
 // x 16 with 576 threads and K = 4, 8 x 16 with 384 threads and K = 4, and 16 x
 // 16 with K = 8 and 512 or 256 threads (two or four cells per thread) were
-// measured slower on the 200 x 200 grid)
+// measured slower on the 200 x 200 grid; 24 x 56 with 512 threads holds four
+// cells per thread, branch-free, two workgroups per CU: the batch and salt
+// jobs' choice; 32 x 64 with 1024 threads and K = 8 (four cells, one
+// workgroup per CU) was slower there)
 
 static void cfg_list(int systems, const BlockCfg** list, int* n) {
     if (systems == TRTM_SH) {
