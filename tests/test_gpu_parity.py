@@ -298,22 +298,24 @@ def test_sh_layered_reduced_parity_and_float64_error():
 
 def test_sh_layered_full_length_bench_kernels_bitwise():
     """The single-shot bench workload at full length, 1200 x 600 x 4000: the
-    blocked pair the autotuner picks for it on MI355X (forward 48 x 64 tiles,
-    backward 32 x 48 tiles, 8 steps per launch) against the oracle, bit for
-    bit: seismograms, final states, the image (isnap 10, so the oracle keeps
-    400 snapshots) and the codes."""
-    _case("sh_layered 1200x600x4000 blocked 48x64x8 / 32x48x8 vs oracle")
+    blocked shapes the autotuner picks for it on MI355X (48 x 64 tiles with
+    1024 threads and 8 steps per launch both ways; before the per-launch step
+    schedule the backward took 32 x 48 tiles with 512 threads) against the
+    oracle, bit for bit: seismograms, final states, the image (isnap 10, so
+    the oracle keeps 400 snapshots) and the codes."""
+    _case("sh_layered 1200x600x4000 blocked 48x64x8 / 48x64x8 and 32x48x8 vs oracle")
     cfg = configs.sh_layered()
     kw = cfg.kwargs(cfg.shots[0])
     wav = cfg.wavelets(cfg.shots[0])
-    h = run_hip(kw, wav, isnap=10, kernel="blocked",
-                fw_opts=dict(tile_x=48, tile_z=64, block_steps=8),
-                bw_opts=dict(tile_x=32, tile_z=48, block_steps=8))
-    assert h["fw"].block_config() == (48, 64, 1024, 8)
-    assert h["bw"].block_config() == (32, 48, 512, 8)
     o = run_oracle(kw, wav, isnap=10)
-    assert_bitwise(h, o)
-    assert np.abs(h["image"][1]).max() > 0 and np.abs(h["seis"][1]).max() > 0
+    for bw in ((48, 64, 1024, 8), (32, 48, 512, 8)):
+        h = run_hip(kw, wav, isnap=10, kernel="blocked",
+                    fw_opts=dict(tile_x=48, tile_z=64, block_steps=8),
+                    bw_opts=dict(tile_x=bw[0], tile_z=bw[1], block_steps=bw[3]))
+        assert h["fw"].block_config() == (48, 64, 1024, 8)
+        assert h["bw"].block_config() == bw
+        assert_bitwise(h, o)
+        assert np.abs(h["image"][1]).max() > 0 and np.abs(h["seis"][1]).max() > 0
 
 
 def test_sh_layered_full_length_image_error_against_float64():
