@@ -439,419 +439,329 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
     }
 
     bool started = false;  // a step ran in this launch: LDS velocities await the sponge
-    for (int s = s_lo; s < s_hi; s++) {
-        const int it = it_first + s;
-        const int tb = nt - 1 - it;  // backward time index
-        const unsigned bit = 1u << s;
-        // this step's snapshot to write (forward) or correlate (backward) at
-        // slot offset s_off, finite-check index (-1: none)
-        const size_t s_off = (size_t)q_off;
-        if (mult & bit) q_off += sstep;
-        const bool snap = FWD && (omask & bit);
-        bool img = false;
-        if constexpr (PIPE)
-            img = img_next;
-        else
-            img = !FWD && (omask & bit);
-        int chk = -1;
-        if (cmask & bit) chk = cn++;
-        // this step's injected values of the slots' single-entry groups, from
-        // the launch's table, added after the velocity update
-        constexpr int NG = HOIST ? NC : 1;
-        float gvu[NG], gvv[NG], gvw[NG];
-        const int col = FWD ? it : tb;
-#pragma unroll
-        for (int k = 0; k < NG; k++) {
-            gvu[k] = gvv[k] = gvw[k] = 0.0f;
-            if (HOIST && gk[k] >= 0) {
-                const int o = gk[k] * K + s;
-                if constexpr (HOIST && PSV) {
-                    gvu[k] = tabU[o];
-                    gvw[k] = tabW[o];
-                }
-                if constexpr (HOIST && SH) gvv[k] = tabV[o];
-            }
-        }
-        // the injection of slot k into its velocity (after the update, in
-        // entry order, the expressions of the injection phase below)
-        auto inject_u = [&](int k, float& u, float& w) {
-            if (!HOIST || gk[k] == kNone) return;
-            if (gk[k] >= 0) {
-                if (FWD) {
-                    u = u + (gvu[k] * m.factor) * c0[k];
-                    w = w + (gvw[k] * m.factor) * c1[k];
-                } else {
-                    u = u + gvu[k];
-                    w = w + gvw[k];
-                }
-                return;
-            }
-            const int g = -1 - gk[k];
-            const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
-            for (int q2 = q0; q2 < q1; q2++) {
-                const size_t o = (size_t)C(sh->order)[q2] * (size_t)nt + col;
-                if (FWD) {
-                    u = u + (C(sh->su)[o] * m.factor) * c0[k];
-                    w = w + (C(sh->sw)[o] * m.factor) * c1[k];
-                } else {
-                    u = u + C(sh->su)[o];
-                    w = w + C(sh->sw)[o];
-                }
-            }
-        };
-        auto inject_v = [&](int k, float& v) {
-            if (!HOIST || gk[k] == kNone) return;
-            if (gk[k] >= 0) {
-                v = FWD ? v + (gvv[k] * m.factor) * d0[k] : v + gvv[k];
-                return;
-            }
-            const int g = -1 - gk[k];
-            const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
-            for (int q2 = q0; q2 < q1; q2++) {
-                const size_t o = (size_t)C(sh->order)[q2] * (size_t)nt + col;
-                v = FWD ? v + (C(sh->sv)[o] * m.factor) * d0[k] : v + C(sh->sv)[o];
-            }
-        };
-        // SH imaging: the forward wavefield of every slot is loaded here, a
-        // whole phase before the image update uses it (cells outside the grid
-        // read cell 0; only the tile's cells use the value)
-        float fimg[SH && !PSV ? NC : 1], fimu[PSV && !SH ? NC : 1], fimw[PSV && !SH ? NC : 1];
-        if constexpr (PIPE) {
-            if (img) {
-#pragma unroll
-                for (int k = 0; k < NF; k++) {
-                    if constexpr (PSV) {
-                        fimu[k] = ssponge ? pf0[k] * A[k] : pf0[k];
-                        fimw[k] = ssponge ? pf1[k] * A[k] : pf1[k];
-                    } else {
-                        fimg[k] = ssponge ? pf0[k] * A[k] : pf0[k];
+    // the step loop, compiled once per surface-matrix choice so that the
+    // executed loop's code is contiguous
+    auto step_loop = [&](auto sm_loop) {
+        constexpr bool SML = decltype(sm_loop)::value;
+        for (int s = s_lo; s < s_hi; s++) {
+            const int it = it_first + s;
+            const int tb = nt - 1 - it;  // backward time index
+            const unsigned bit = 1u << s;
+            // this step's snapshot to write (forward) or correlate (backward) at
+            // slot offset s_off, finite-check index (-1: none)
+            const size_t s_off = (size_t)q_off;
+            if (mult & bit) q_off += sstep;
+            const bool snap = FWD && (omask & bit);
+            bool img = false;
+            if constexpr (PIPE)
+                img = img_next;
+            else
+                img = !FWD && (omask & bit);
+            int chk = -1;
+            if (cmask & bit) chk = cn++;
+            // this step's injected values of the slots' single-entry groups, from
+            // the launch's table, added after the velocity update
+            constexpr int NG = HOIST ? NC : 1;
+            float gvu[NG], gvv[NG], gvw[NG];
+            const int col = FWD ? it : tb;
+    #pragma unroll
+            for (int k = 0; k < NG; k++) {
+                gvu[k] = gvv[k] = gvw[k] = 0.0f;
+                if (HOIST && gk[k] >= 0) {
+                    const int o = gk[k] * K + s;
+                    if constexpr (HOIST && PSV) {
+                        gvu[k] = tabU[o];
+                        gvw[k] = tabW[o];
                     }
+                    if constexpr (HOIST && SH) gvv[k] = tabV[o];
                 }
             }
-            // the next step's forward wavefield, in flight during this step
-            // (its slot offset is q_off, moved past this step)
-            img_next = s + 1 < K && ((omask >> (s + 1)) & 1u);
-            if (img_next) load_forward((size_t)q_off);
-        } else if constexpr (SH && !PSV) {
-            if (img) {
-#pragma unroll
-                for (int k = 0; k < NC; k++) {
-                    fimg[k] = snap_v[s_off + gc[k]];
-                    if (ssponge) fimg[k] = fimg[k] * A[k];
-                }
-            }
-        }
-        if constexpr (PSV && !SH && !PIPE) {
-            if (img) {
-#pragma unroll
-                for (int k = 0; k < NC; k++) {
-                    fimu[k] = snap_u[s_off + gc[k]];
-                    fimw[k] = snap_w[s_off + gc[k]];
-                    if (ssponge) {
-                        fimu[k] = fimu[k] * A[k];
-                        fimw[k] = fimw[k] * A[k];
-                    }
-                }
-            }
-        }
-
-        // ---- velocity update (region)
-        // SH-only kernels, without a surface matrix branch-free over every
-        // slot (the result selected per cell); P-SV keeps per-cell branches
-        // (its registers are the tighter limit)
-        auto velocity_phase = [&](auto sm_tag) {
-            constexpr bool SM = decltype(sm_tag)::value;  // stresses read through the surface matrix
-            auto mk = [&](float x, int c) {
-                if constexpr (SM) return x * sm[c];
-                else return x;
-            };
-#pragma unroll
-            for (int k = 0; k < NC; k++) {
-                if (SM && !(fl[k] & 1u)) continue;
-                const int e = t + NT * k;
-                const int c = gc[k];
-                const bool upd = (fl[k] & 10u) == 10u;
-                float v = sV[e];
-                if (started) v = v * A[k];
-                if (!SM || upd) {
-                    float syx_x, syz_z, vn;
+            // the injection of slot k into its velocity (after the update, in
+            // entry order, the expressions of the injection phase below)
+            auto inject_u = [&](int k, float& u, float& w) {
+                if (!HOIST || gk[k] == kNone) return;
+                if (gk[k] >= 0) {
                     if (FWD) {
-                        syx_x = (sYX[e] - sYX[e - RZ]) * idx;
-                        syz_z = (mk(sYZ[e], c) - mk(sYZ[e - 1], c - 1)) * idz;
-                        vn = v + ((syx_x + syz_z) * dt) * d0[k];
+                        u = u + (gvu[k] * m.factor) * c0[k];
+                        w = w + (gvw[k] * m.factor) * c1[k];
                     } else {
-                        syx_x = (sYX[e + RZ] - sYX[e]) * idx;
-                        syz_z = (mk(sYZ[e + 1], c + 1) - mk(sYZ[e], c)) * idz;
-                        vn = v + ((-(syx_x + syz_z)) * dt) * d0[k];
+                        u = u + gvu[k];
+                        w = w + gvw[k];
                     }
-                    v = upd ? vn : v;
+                    return;
                 }
-                inject_v(k, v);
-                sV[e] = v;
-            }
-        };
-        // P-SV: per-cell branches; the surface matrix is a template choice,
-        // so the stress reads of the update issue together
-        auto velocity_psv = [&](auto sm_tag) {
-            constexpr bool SM = decltype(sm_tag)::value;
-            auto msk = [&](float x, int c) {
-                if constexpr (SM) return x * sm[c];
-                else return x;
+                const int g = -1 - gk[k];
+                const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
+                for (int q2 = q0; q2 < q1; q2++) {
+                    const size_t o = (size_t)C(sh->order)[q2] * (size_t)nt + col;
+                    if (FWD) {
+                        u = u + (C(sh->su)[o] * m.factor) * c0[k];
+                        w = w + (C(sh->sw)[o] * m.factor) * c1[k];
+                    } else {
+                        u = u + C(sh->su)[o];
+                        w = w + C(sh->sw)[o];
+                    }
+                }
             };
-#pragma unroll
-            for (int k = 0; k < NC; k++) {
-                if (!(fl[k] & 1u)) continue;
-                const int e = t + NT * k;
-                const int c = gc[k];
-                const bool upd = (fl[k] & 2u) && (fl[k] & 8u);
-                if (PSV) {
+            auto inject_v = [&](int k, float& v) {
+                if (!HOIST || gk[k] == kNone) return;
+                if (gk[k] >= 0) {
+                    v = FWD ? v + (gvv[k] * m.factor) * d0[k] : v + gvv[k];
+                    return;
+                }
+                const int g = -1 - gk[k];
+                const int q0 = C(sh->start)[g], q1 = C(sh->start)[g + 1];
+                for (int q2 = q0; q2 < q1; q2++) {
+                    const size_t o = (size_t)C(sh->order)[q2] * (size_t)nt + col;
+                    v = FWD ? v + (C(sh->sv)[o] * m.factor) * d0[k] : v + C(sh->sv)[o];
+                }
+            };
+            // SH imaging: the forward wavefield of every slot is loaded here, a
+            // whole phase before the image update uses it (cells outside the grid
+            // read cell 0; only the tile's cells use the value)
+            float fimg[SH && !PSV ? NC : 1], fimu[PSV && !SH ? NC : 1], fimw[PSV && !SH ? NC : 1];
+            if constexpr (PIPE) {
+                if (img) {
+    #pragma unroll
+                    for (int k = 0; k < NF; k++) {
+                        if constexpr (PSV) {
+                            fimu[k] = ssponge ? pf0[k] * A[k] : pf0[k];
+                            fimw[k] = ssponge ? pf1[k] * A[k] : pf1[k];
+                        } else {
+                            fimg[k] = ssponge ? pf0[k] * A[k] : pf0[k];
+                        }
+                    }
+                }
+                // the next step's forward wavefield, in flight during this step
+                // (its slot offset is q_off, moved past this step)
+                img_next = s + 1 < K && ((omask >> (s + 1)) & 1u);
+                if (img_next) load_forward((size_t)q_off);
+            } else if constexpr (SH && !PSV) {
+                if (img) {
+    #pragma unroll
+                    for (int k = 0; k < NC; k++) {
+                        fimg[k] = snap_v[s_off + gc[k]];
+                        if (ssponge) fimg[k] = fimg[k] * A[k];
+                    }
+                }
+            }
+            if constexpr (PSV && !SH && !PIPE) {
+                if (img) {
+    #pragma unroll
+                    for (int k = 0; k < NC; k++) {
+                        fimu[k] = snap_u[s_off + gc[k]];
+                        fimw[k] = snap_w[s_off + gc[k]];
+                        if (ssponge) {
+                            fimu[k] = fimu[k] * A[k];
+                            fimw[k] = fimw[k] * A[k];
+                        }
+                    }
+                }
+            }
+
+            // ---- velocity update (region)
+            // SH-only kernels, without a surface matrix branch-free over every
+            // slot (the result selected per cell); P-SV keeps per-cell branches
+            // (its registers are the tighter limit)
+            auto velocity_phase = [&](auto sm_tag) {
+                constexpr bool SM = decltype(sm_tag)::value;  // stresses read through the surface matrix
+                auto mk = [&](float x, int c) {
+                    if constexpr (SM) return x * sm[c];
+                    else return x;
+                };
+    #pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    if (SM && !(fl[k] & 1u)) continue;
+                    const int e = t + NT * k;
+                    const int c = gc[k];
+                    const bool upd = (fl[k] & 10u) == 10u;
+                    float v = sV[e];
+                    if (started) v = v * A[k];
+                    if (!SM || upd) {
+                        float syx_x, syz_z, vn;
+                        if (FWD) {
+                            syx_x = (sYX[e] - sYX[e - RZ]) * idx;
+                            syz_z = (mk(sYZ[e], c) - mk(sYZ[e - 1], c - 1)) * idz;
+                            vn = v + ((syx_x + syz_z) * dt) * d0[k];
+                        } else {
+                            syx_x = (sYX[e + RZ] - sYX[e]) * idx;
+                            syz_z = (mk(sYZ[e + 1], c + 1) - mk(sYZ[e], c)) * idz;
+                            vn = v + ((-(syx_x + syz_z)) * dt) * d0[k];
+                        }
+                        v = upd ? vn : v;
+                    }
+                    inject_v(k, v);
+                    sV[e] = v;
+                }
+            };
+            // P-SV: per-cell branches; the surface matrix is a template choice,
+            // so the stress reads of the update issue together
+            auto velocity_psv = [&](auto sm_tag) {
+                constexpr bool SM = decltype(sm_tag)::value;
+                auto msk = [&](float x, int c) {
+                    if constexpr (SM) return x * sm[c];
+                    else return x;
+                };
+    #pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    if (!(fl[k] & 1u)) continue;
+                    const int e = t + NT * k;
+                    const int c = gc[k];
+                    const bool upd = (fl[k] & 2u) && (fl[k] & 8u);
+                    if (PSV) {
+                        float u = sU[e], w = sW[e];
+                        if (started) {
+                            u = u * A[k];
+                            w = w * A[k];
+                        }
+                        if (upd) {
+                            float sxx_x, szz_z, sxz_x, sxz_z;
+                            if (FWD) {
+                                sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
+                                szz_z = (msk(sZZ[e], c) - msk(sZZ[e - 1], c - 1)) * idz;
+                                sxz_x = (msk(sXZ[e], c) - msk(sXZ[e - RZ], c - nz)) * idx;
+                                sxz_z = (msk(sXZ[e], c) - msk(sXZ[e - 1], c - 1)) * idz;
+                                u = u + ((sxx_x + sxz_z) * dt) * c0[k];
+                                w = w + ((sxz_x + szz_z) * dt) * c1[k];
+                            } else {
+                                sxx_x = (sXX[e + RZ] - sXX[e]) * idx;
+                                szz_z = (msk(sZZ[e + 1], c + 1) - msk(sZZ[e], c)) * idz;
+                                sxz_x = (msk(sXZ[e + RZ], c + nz) - msk(sXZ[e], c)) * idx;
+                                sxz_z = (msk(sXZ[e + 1], c + 1) - msk(sXZ[e], c)) * idz;
+                                u = u + ((-(sxx_x + sxz_z)) * dt) * c0[k];
+                                w = w + ((-(sxz_x + szz_z)) * dt) * c1[k];
+                            }
+                        }
+                        inject_u(k, u, w);
+                        sU[e] = u;
+                        sW[e] = w;
+                    }
+                    if (SH) {
+                        float v = sV[e];
+                        if (started) v = v * A[k];
+                        if (upd) {
+                            float syx_x, syz_z;
+                            if (FWD) {
+                                syx_x = (sYX[e] - sYX[e - RZ]) * idx;
+                                syz_z = (msk(sYZ[e], c) - msk(sYZ[e - 1], c - 1)) * idz;
+                                v = v + ((syx_x + syz_z) * dt) * d0[k];
+                            } else {
+                                syx_x = (sYX[e + RZ] - sYX[e]) * idx;
+                                syz_z = (msk(sYZ[e + 1], c + 1) - msk(sYZ[e], c)) * idz;
+                                v = v + ((-(syx_x + syz_z)) * dt) * d0[k];
+                            }
+                        }
+                        inject_v(k, v);
+                        sV[e] = v;
+                    }
+                }
+            };
+            // P-SV, up to four cells per thread, flat surface: branch-free over
+            // every slot
+            auto velocity_psv_bf = [&]() {
+    #pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    const int e = t + NT * k;
+                    const bool upd = (fl[k] & 10u) == 10u;
                     float u = sU[e], w = sW[e];
                     if (started) {
                         u = u * A[k];
                         w = w * A[k];
                     }
-                    if (upd) {
-                        float sxx_x, szz_z, sxz_x, sxz_z;
-                        if (FWD) {
-                            sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
-                            szz_z = (msk(sZZ[e], c) - msk(sZZ[e - 1], c - 1)) * idz;
-                            sxz_x = (msk(sXZ[e], c) - msk(sXZ[e - RZ], c - nz)) * idx;
-                            sxz_z = (msk(sXZ[e], c) - msk(sXZ[e - 1], c - 1)) * idz;
-                            u = u + ((sxx_x + sxz_z) * dt) * c0[k];
-                            w = w + ((sxz_x + szz_z) * dt) * c1[k];
-                        } else {
-                            sxx_x = (sXX[e + RZ] - sXX[e]) * idx;
-                            szz_z = (msk(sZZ[e + 1], c + 1) - msk(sZZ[e], c)) * idz;
-                            sxz_x = (msk(sXZ[e + RZ], c + nz) - msk(sXZ[e], c)) * idx;
-                            sxz_z = (msk(sXZ[e + 1], c + 1) - msk(sXZ[e], c)) * idz;
-                            u = u + ((-(sxx_x + sxz_z)) * dt) * c0[k];
-                            w = w + ((-(sxz_x + szz_z)) * dt) * c1[k];
-                        }
+                    float un, wn;
+                    if (FWD) {
+                        const float sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
+                        const float szz_z = (sZZ[e] - sZZ[e - 1]) * idz;
+                        const float sxz_x = (sXZ[e] - sXZ[e - RZ]) * idx;
+                        const float sxz_z = (sXZ[e] - sXZ[e - 1]) * idz;
+                        un = u + ((sxx_x + sxz_z) * dt) * c0[k];
+                        wn = w + ((sxz_x + szz_z) * dt) * c1[k];
+                    } else {
+                        const float sxx_x = (sXX[e + RZ] - sXX[e]) * idx;
+                        const float szz_z = (sZZ[e + 1] - sZZ[e]) * idz;
+                        const float sxz_x = (sXZ[e + RZ] - sXZ[e]) * idx;
+                        const float sxz_z = (sXZ[e + 1] - sXZ[e]) * idz;
+                        un = u + ((-(sxx_x + sxz_z)) * dt) * c0[k];
+                        wn = w + ((-(sxz_x + szz_z)) * dt) * c1[k];
                     }
+                    u = upd ? un : u;
+                    w = upd ? wn : w;
                     inject_u(k, u, w);
                     sU[e] = u;
                     sW[e] = w;
                 }
-                if (SH) {
-                    float v = sV[e];
-                    if (started) v = v * A[k];
-                    if (upd) {
-                        float syx_x, syz_z;
-                        if (FWD) {
-                            syx_x = (sYX[e] - sYX[e - RZ]) * idx;
-                            syz_z = (msk(sYZ[e], c) - msk(sYZ[e - 1], c - 1)) * idz;
-                            v = v + ((syx_x + syz_z) * dt) * d0[k];
-                        } else {
-                            syx_x = (sYX[e + RZ] - sYX[e]) * idx;
-                            syz_z = (msk(sYZ[e + 1], c + 1) - msk(sYZ[e], c)) * idz;
-                            v = v + ((-(syx_x + syz_z)) * dt) * d0[k];
-                        }
-                    }
-                    inject_v(k, v);
-                    sV[e] = v;
-                }
-            }
-        };
-        // P-SV, up to four cells per thread, flat surface: branch-free over
-        // every slot
-        auto velocity_psv_bf = [&]() {
-#pragma unroll
-            for (int k = 0; k < NC; k++) {
-                const int e = t + NT * k;
-                const bool upd = (fl[k] & 10u) == 10u;
-                float u = sU[e], w = sW[e];
-                if (started) {
-                    u = u * A[k];
-                    w = w * A[k];
-                }
-                float un, wn;
-                if (FWD) {
-                    const float sxx_x = (sXX[e] - sXX[e - RZ]) * idx;
-                    const float szz_z = (sZZ[e] - sZZ[e - 1]) * idz;
-                    const float sxz_x = (sXZ[e] - sXZ[e - RZ]) * idx;
-                    const float sxz_z = (sXZ[e] - sXZ[e - 1]) * idz;
-                    un = u + ((sxx_x + sxz_z) * dt) * c0[k];
-                    wn = w + ((sxz_x + szz_z) * dt) * c1[k];
-                } else {
-                    const float sxx_x = (sXX[e + RZ] - sXX[e]) * idx;
-                    const float szz_z = (sZZ[e + 1] - sZZ[e]) * idz;
-                    const float sxz_x = (sXZ[e + RZ] - sXZ[e]) * idx;
-                    const float sxz_z = (sXZ[e + 1] - sXZ[e]) * idz;
-                    un = u + ((-(sxx_x + sxz_z)) * dt) * c0[k];
-                    wn = w + ((-(sxz_x + szz_z)) * dt) * c1[k];
-                }
-                u = upd ? un : u;
-                w = upd ? wn : w;
-                inject_u(k, u, w);
-                sU[e] = u;
-                sW[e] = w;
-            }
-        };
-        if constexpr (BFP) {
-            if (has_sm)
-                velocity_psv(std::true_type{});
-            else
-                velocity_psv_bf();
-        } else if constexpr (PSV) {
-            if (has_sm)
-                velocity_psv(std::true_type{});
-            else
-                velocity_psv(std::false_type{});
-        } else if (has_sm) {
-            velocity_phase(std::true_type{});
-        } else {
-            velocity_phase(std::false_type{});
-        }
-        __syncthreads();
-
-        // ---- injection into the region (one thread per cell, entries in order;
-        // the small shapes inject in the velocity update above)
-        if (ig1 > ig0 && !inj_one_pass) {
-            const gptr<const float> su = G(sh->su), sv = G(sh->sv), sw = G(sh->sw);
-            const size_t ntt = (size_t)nt;
-            {
-                const gptr<const int> titems = G(sh->inj_titems), gstart = G(sh->start);
-                const gptr<const int> order = G(sh->order), loc = G(sh->loc);
-                for (int q = ig0 + t; q < ig1; q += NT) {
-                    const int g = titems[q];
-                    const int e0 = gstart[g], e1 = gstart[g + 1];
-                    const int k0 = order[e0];
-                    const int ci = loc[2 * k0], cj = loc[2 * k0 + 1];
-                    const int e = (ci - ri0) * RZ + (cj - rj0);
-                    const int c = ci * nz + cj;
-                    if (PSV) {
-                        float x = sU[e], zz = sW[e];
-                        for (int q2 = e0; q2 < e1; q2++) {
-                            const size_t o = (size_t)order[q2] * ntt + col;
-                            if (FWD) {
-                                x = x + (su[o] * m.factor) * G(m.iru)[c];
-                                zz = zz + (sw[o] * m.factor) * G(m.irw)[c];
-                            } else {
-                                x = x + su[o];
-                                zz = zz + sw[o];
-                            }
-                        }
-                        sU[e] = x;
-                        sW[e] = zz;
-                    }
-                    if (SH) {
-                        float y = sV[e];
-                        for (int q2 = e0; q2 < e1; q2++) {
-                            const size_t o = (size_t)order[q2] * ntt + col;
-                            y = FWD ? y + (sv[o] * m.factor) * G(m.ir)[c] : y + sv[o];
-                        }
-                        sV[e] = y;
-                    }
-                }
+            };
+            if constexpr (BFP) {
+                if constexpr (SML)
+                    velocity_psv(std::true_type{});
+                else
+                    velocity_psv_bf();
+            } else if constexpr (PSV) {
+                velocity_psv(std::integral_constant<bool, SML>{});
+            } else {
+                velocity_phase(std::integral_constant<bool, SML>{});
             }
             __syncthreads();
-        }
 
-        // ---- stress update + sponge (region), outputs (tile)
-        int code = 0;
-        auto stress_phase = [&](auto sm_tag) {
-            constexpr bool SM = decltype(sm_tag)::value;
-            auto mk = [&](float x, int c) {
-                if constexpr (SM) return x * sm[c];
-                else return x;
-            };
-#pragma unroll
-            for (int k = 0; k < NC; k++) {
-                if (SM && !(fl[k] & 1u)) continue;
-                const int e = t + NT * k;
-                const int c = gc[k];
-                const bool upd = (fl[k] & 18u) == 18u;
-                float syx = sYX[e], syz = mk(sYZ[e], c);
-                if (!SM || upd) {
-                    float v_x, v_z;
-                    if (FWD) {
-                        v_x = (sV[e + RZ] - sV[e]) * idx;
-                        v_z = (sV[e + 1] - sV[e]) * idz;
-                    } else {
-                        v_x = (sV[e] - sV[e - RZ]) * idx;
-                        v_z = (sV[e] - sV[e - 1]) * idz;
-                    }
-                    const float yxn = syx + (sdt * d1[k]) * v_x;
-                    const float yzn = syz + (sdt * d2[k]) * v_z;
-                    syx = upd ? yxn : syx;
-                    syz = upd ? yzn : syz;
-                }
-                sYX[e] = syx * A[k];
-                sYZ[e] = syz * A[k];
-                if (fl[k] & 4u) {  // the tile's cell: outputs
-                    const float v = sV[e] * A[k];
-                    if (snap) snap_v[s_off + c] = v;
-                    if (img) aV[k] = aV[k] + fimg[k] * v;
-                    if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
-                }
-            }
-        };
-        auto stress_psv = [&](auto sm_tag) {
-            constexpr bool SM = decltype(sm_tag)::value;
-            auto msk = [&](float x, int c) {
-                if constexpr (SM) return x * sm[c];
-                else return x;
-            };
-#pragma unroll
-            for (int k = 0; k < NC; k++) {
-                if (!(fl[k] & 1u)) continue;
-                const int e = t + NT * k;
-                const int c = gc[k];
-                const bool upd = (fl[k] & 2u) && (fl[k] & 16u);
-                const bool own = fl[k] & 4u;
-                if (PSV) {
-                    float sxx = sXX[e], szz = msk(sZZ[e], c), sxz = msk(sXZ[e], c);
-                    if (upd) {
-                        float u_x, u_z, w_x, w_z;
-                        if (FWD) {
-                            u_x = (sU[e + RZ] - sU[e]) * idx;
-                            u_z = (sU[e + 1] - sU[e]) * idz;
-                            w_x = (sW[e + RZ] - sW[e]) * idx;
-                            w_z = (sW[e + 1] - sW[e]) * idz;
-                        } else {
-                            u_x = (sU[e] - sU[e - RZ]) * idx;
-                            u_z = (sU[e] - sU[e - 1]) * idz;
-                            w_x = (sW[e] - sW[e - RZ]) * idx;
-                            w_z = (sW[e] - sW[e - 1]) * idz;
-                        }
-                        const float div_uw = u_x + w_z;
-                        sxx = sxx + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
-                        szz = szz + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
-                        sxz = sxz + (sdt * c4[k]) * (u_z + w_x);
-                    }
-                    sXX[e] = sxx * A[k];
-                    sZZ[e] = szz * A[k];
-                    sXZ[e] = sxz * A[k];
-                    if (own) {
-                        const float u = sU[e] * A[k], w = sW[e] * A[k];
-                        if (snap) {
-                            snap_u[s_off + c] = u;
-                            snap_w[s_off + c] = w;
-                        }
-                        if (img) {
-                            if constexpr (!SH) {
-                                aU[k] = aU[k] + fimu[k] * u;
-                                aW[k] = aW[k] + fimw[k] * w;
-                            } else {
-                                float fu = snap_u[s_off + c];
-                                float fw = snap_w[s_off + c];
-                                if (ssponge) {
-                                    fu = fu * A[k];
-                                    fw = fw * A[k];
+            // ---- injection into the region (one thread per cell, entries in order;
+            // the small shapes inject in the velocity update above)
+            if (ig1 > ig0 && !inj_one_pass) {
+                const gptr<const float> su = G(sh->su), sv = G(sh->sv), sw = G(sh->sw);
+                const size_t ntt = (size_t)nt;
+                {
+                    const gptr<const int> titems = G(sh->inj_titems), gstart = G(sh->start);
+                    const gptr<const int> order = G(sh->order), loc = G(sh->loc);
+                    for (int q = ig0 + t; q < ig1; q += NT) {
+                        const int g = titems[q];
+                        const int e0 = gstart[g], e1 = gstart[g + 1];
+                        const int k0 = order[e0];
+                        const int ci = loc[2 * k0], cj = loc[2 * k0 + 1];
+                        const int e = (ci - ri0) * RZ + (cj - rj0);
+                        const int c = ci * nz + cj;
+                        if (PSV) {
+                            float x = sU[e], zz = sW[e];
+                            for (int q2 = e0; q2 < e1; q2++) {
+                                const size_t o = (size_t)order[q2] * ntt + col;
+                                if (FWD) {
+                                    x = x + (su[o] * m.factor) * G(m.iru)[c];
+                                    zz = zz + (sw[o] * m.factor) * G(m.irw)[c];
+                                } else {
+                                    x = x + su[o];
+                                    zz = zz + sw[o];
                                 }
-                                aU[k] = aU[k] + fu * u;
-                                aW[k] = aW[k] + fw * w;
                             }
+                            sU[e] = x;
+                            sW[e] = zz;
                         }
-                        if (chk >= 0) {
-                            if (not_finite_or_big(u) && code < 1) code = 1;
-                            if (not_finite_or_big(w)) code = 3;
+                        if (SH) {
+                            float y = sV[e];
+                            for (int q2 = e0; q2 < e1; q2++) {
+                                const size_t o = (size_t)order[q2] * ntt + col;
+                                y = FWD ? y + (sv[o] * m.factor) * G(m.ir)[c] : y + sv[o];
+                            }
+                            sV[e] = y;
                         }
                     }
                 }
-                if (SH) {
-                    float syx = sYX[e], syz = msk(sYZ[e], c);
-                    if (upd) {
+                __syncthreads();
+            }
+
+            // ---- stress update + sponge (region), outputs (tile)
+            int code = 0;
+            auto stress_phase = [&](auto sm_tag) {
+                constexpr bool SM = decltype(sm_tag)::value;
+                auto mk = [&](float x, int c) {
+                    if constexpr (SM) return x * sm[c];
+                    else return x;
+                };
+    #pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    if (SM && !(fl[k] & 1u)) continue;
+                    const int e = t + NT * k;
+                    const int c = gc[k];
+                    const bool upd = (fl[k] & 18u) == 18u;
+                    float syx = sYX[e], syz = mk(sYZ[e], c);
+                    if (!SM || upd) {
                         float v_x, v_z;
                         if (FWD) {
                             v_x = (sV[e + RZ] - sV[e]) * idx;
@@ -860,129 +770,218 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                             v_x = (sV[e] - sV[e - RZ]) * idx;
                             v_z = (sV[e] - sV[e - 1]) * idz;
                         }
-                        syx = syx + (sdt * d1[k]) * v_x;
-                        syz = syz + (sdt * d2[k]) * v_z;
+                        const float yxn = syx + (sdt * d1[k]) * v_x;
+                        const float yzn = syz + (sdt * d2[k]) * v_z;
+                        syx = upd ? yxn : syx;
+                        syz = upd ? yzn : syz;
                     }
                     sYX[e] = syx * A[k];
                     sYZ[e] = syz * A[k];
-                    if (own) {
+                    if (fl[k] & 4u) {  // the tile's cell: outputs
                         const float v = sV[e] * A[k];
                         if (snap) snap_v[s_off + c] = v;
-                        if (img) {
-                            float fv = snap_v[s_off + c];
-                            if (ssponge) fv = fv * A[k];
-                            aV[k] = aV[k] + fv * v;
-                        }
+                        if (img) aV[k] = aV[k] + fimg[k] * v;
                         if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
                     }
                 }
-            }
-        };
-        // P-SV, up to four cells per thread, flat surface: branch-free over
-        // every slot; the outputs (snapshots, image, finite check) only for the
-        // tile's cells
-        auto stress_psv_bf = [&]() {
-#pragma unroll
-            for (int k = 0; k < NC; k++) {
-                const int e = t + NT * k;
-                const int c = gc[k];
-                const bool upd = (fl[k] & 18u) == 18u;
-                float sxx = sXX[e], szz = sZZ[e], sxz = sXZ[e];
-                float u_x, u_z, w_x, w_z;
-                if (FWD) {
-                    u_x = (sU[e + RZ] - sU[e]) * idx;
-                    u_z = (sU[e + 1] - sU[e]) * idz;
-                    w_x = (sW[e + RZ] - sW[e]) * idx;
-                    w_z = (sW[e + 1] - sW[e]) * idz;
-                } else {
-                    u_x = (sU[e] - sU[e - RZ]) * idx;
-                    u_z = (sU[e] - sU[e - 1]) * idz;
-                    w_x = (sW[e] - sW[e - RZ]) * idx;
-                    w_z = (sW[e] - sW[e - 1]) * idz;
-                }
-                const float div_uw = u_x + w_z;
-                const float xxn = sxx + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
-                const float zzn = szz + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
-                const float xzn = sxz + (sdt * c4[k]) * (u_z + w_x);
-                sxx = upd ? xxn : sxx;
-                szz = upd ? zzn : szz;
-                sxz = upd ? xzn : sxz;
-                sXX[e] = sxx * A[k];
-                sZZ[e] = szz * A[k];
-                sXZ[e] = sxz * A[k];
-                const float u = sU[e] * A[k], w = sW[e] * A[k];
-                if constexpr (PIPE) {  // the image of a slot outside the tile is never written
-                    if (img) {
-                        aU[k] = aU[k] + fimu[k] * u;
-                        aW[k] = aW[k] + fimw[k] * w;
+            };
+            auto stress_psv = [&](auto sm_tag) {
+                constexpr bool SM = decltype(sm_tag)::value;
+                auto msk = [&](float x, int c) {
+                    if constexpr (SM) return x * sm[c];
+                    else return x;
+                };
+    #pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    if (!(fl[k] & 1u)) continue;
+                    const int e = t + NT * k;
+                    const int c = gc[k];
+                    const bool upd = (fl[k] & 2u) && (fl[k] & 16u);
+                    const bool own = fl[k] & 4u;
+                    if (PSV) {
+                        float sxx = sXX[e], szz = msk(sZZ[e], c), sxz = msk(sXZ[e], c);
+                        if (upd) {
+                            float u_x, u_z, w_x, w_z;
+                            if (FWD) {
+                                u_x = (sU[e + RZ] - sU[e]) * idx;
+                                u_z = (sU[e + 1] - sU[e]) * idz;
+                                w_x = (sW[e + RZ] - sW[e]) * idx;
+                                w_z = (sW[e + 1] - sW[e]) * idz;
+                            } else {
+                                u_x = (sU[e] - sU[e - RZ]) * idx;
+                                u_z = (sU[e] - sU[e - 1]) * idz;
+                                w_x = (sW[e] - sW[e - RZ]) * idx;
+                                w_z = (sW[e] - sW[e - 1]) * idz;
+                            }
+                            const float div_uw = u_x + w_z;
+                            sxx = sxx + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
+                            szz = szz + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
+                            sxz = sxz + (sdt * c4[k]) * (u_z + w_x);
+                        }
+                        sXX[e] = sxx * A[k];
+                        sZZ[e] = szz * A[k];
+                        sXZ[e] = sxz * A[k];
+                        if (own) {
+                            const float u = sU[e] * A[k], w = sW[e] * A[k];
+                            if (snap) {
+                                snap_u[s_off + c] = u;
+                                snap_w[s_off + c] = w;
+                            }
+                            if (img) {
+                                if constexpr (!SH) {
+                                    aU[k] = aU[k] + fimu[k] * u;
+                                    aW[k] = aW[k] + fimw[k] * w;
+                                } else {
+                                    float fu = snap_u[s_off + c];
+                                    float fw = snap_w[s_off + c];
+                                    if (ssponge) {
+                                        fu = fu * A[k];
+                                        fw = fw * A[k];
+                                    }
+                                    aU[k] = aU[k] + fu * u;
+                                    aW[k] = aW[k] + fw * w;
+                                }
+                            }
+                            if (chk >= 0) {
+                                if (not_finite_or_big(u) && code < 1) code = 1;
+                                if (not_finite_or_big(w)) code = 3;
+                            }
+                        }
+                    }
+                    if (SH) {
+                        float syx = sYX[e], syz = msk(sYZ[e], c);
+                        if (upd) {
+                            float v_x, v_z;
+                            if (FWD) {
+                                v_x = (sV[e + RZ] - sV[e]) * idx;
+                                v_z = (sV[e + 1] - sV[e]) * idz;
+                            } else {
+                                v_x = (sV[e] - sV[e - RZ]) * idx;
+                                v_z = (sV[e] - sV[e - 1]) * idz;
+                            }
+                            syx = syx + (sdt * d1[k]) * v_x;
+                            syz = syz + (sdt * d2[k]) * v_z;
+                        }
+                        sYX[e] = syx * A[k];
+                        sYZ[e] = syz * A[k];
+                        if (own) {
+                            const float v = sV[e] * A[k];
+                            if (snap) snap_v[s_off + c] = v;
+                            if (img) {
+                                float fv = snap_v[s_off + c];
+                                if (ssponge) fv = fv * A[k];
+                                aV[k] = aV[k] + fv * v;
+                            }
+                            if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
+                        }
                     }
                 }
-                if (fl[k] & 4u) {
-                    if constexpr (!PIPE) {
+            };
+            // P-SV, up to four cells per thread, flat surface: branch-free over
+            // every slot; the outputs (snapshots, image, finite check) only for the
+            // tile's cells
+            auto stress_psv_bf = [&]() {
+    #pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    const int e = t + NT * k;
+                    const int c = gc[k];
+                    const bool upd = (fl[k] & 18u) == 18u;
+                    float sxx = sXX[e], szz = sZZ[e], sxz = sXZ[e];
+                    float u_x, u_z, w_x, w_z;
+                    if (FWD) {
+                        u_x = (sU[e + RZ] - sU[e]) * idx;
+                        u_z = (sU[e + 1] - sU[e]) * idz;
+                        w_x = (sW[e + RZ] - sW[e]) * idx;
+                        w_z = (sW[e + 1] - sW[e]) * idz;
+                    } else {
+                        u_x = (sU[e] - sU[e - RZ]) * idx;
+                        u_z = (sU[e] - sU[e - 1]) * idz;
+                        w_x = (sW[e] - sW[e - RZ]) * idx;
+                        w_z = (sW[e] - sW[e - 1]) * idz;
+                    }
+                    const float div_uw = u_x + w_z;
+                    const float xxn = sxx + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * u_x);
+                    const float zzn = szz + sdt * (c2[k] * div_uw + (2.0f * c3[k]) * w_z);
+                    const float xzn = sxz + (sdt * c4[k]) * (u_z + w_x);
+                    sxx = upd ? xxn : sxx;
+                    szz = upd ? zzn : szz;
+                    sxz = upd ? xzn : sxz;
+                    sXX[e] = sxx * A[k];
+                    sZZ[e] = szz * A[k];
+                    sXZ[e] = sxz * A[k];
+                    const float u = sU[e] * A[k], w = sW[e] * A[k];
+                    if constexpr (PIPE) {  // the image of a slot outside the tile is never written
                         if (img) {
                             aU[k] = aU[k] + fimu[k] * u;
                             aW[k] = aW[k] + fimw[k] * w;
                         }
                     }
-                    if (snap) {
-                        snap_u[s_off + c] = u;
-                        snap_w[s_off + c] = w;
-                    }
-                    if (chk >= 0) {
-                        if (not_finite_or_big(u) && code < 1) code = 1;
-                        if (not_finite_or_big(w)) code = 3;
+                    if (fl[k] & 4u) {
+                        if constexpr (!PIPE) {
+                            if (img) {
+                                aU[k] = aU[k] + fimu[k] * u;
+                                aW[k] = aW[k] + fimw[k] * w;
+                            }
+                        }
+                        if (snap) {
+                            snap_u[s_off + c] = u;
+                            snap_w[s_off + c] = w;
+                        }
+                        if (chk >= 0) {
+                            if (not_finite_or_big(u) && code < 1) code = 1;
+                            if (not_finite_or_big(w)) code = 3;
+                        }
                     }
                 }
-            }
-        };
-        if constexpr (BFP) {
-            if (has_sm)
-                stress_psv(std::true_type{});
-            else
-                stress_psv_bf();
-        } else if constexpr (PSV) {
-            if (has_sm)
-                stress_psv(std::true_type{});
-            else
-                stress_psv(std::false_type{});
-        } else if (has_sm) {
-            stress_phase(std::true_type{});
-        } else {
-            stress_phase(std::false_type{});
-        }
-        if (code) atomicMax(sh->check_flags + chk, code + (FWD ? 0 : 3));
-
-        // ---- receivers of the tile, after the sponge
-        if (rb1 > rb0) {
-            const gptr<float> ru = G(sh->rec_u), rv = G(sh->rec_v), rw = G(sh->rec_w);
-            const size_t row = (size_t)(FWD ? it : tb) * n_rec;
-            if (rec_one_pass) {
-                if (r_l >= 0) {
-                    if (PSV) {
-                        ru[row + r_l] = sU[r_e] * r_A;
-                        rw[row + r_l] = sW[r_e] * r_A;
-                    }
-                    if (SH) rv[row + r_l] = sV[r_e] * r_A;
-                }
+            };
+            if constexpr (BFP) {
+                if constexpr (SML)
+                    stress_psv(std::true_type{});
+                else
+                    stress_psv_bf();
+            } else if constexpr (PSV) {
+                stress_psv(std::integral_constant<bool, SML>{});
             } else {
-                const int b = rb0, end = rb1;
-                const gptr<const int> ritems = G(sh->rec_titems), rloc = G(sh->rec_loc);
-                for (int q = b + t; q < end; q += NT) {
-                    const int l = ritems[q];
-                    const int ci = rloc[2 * l], cj = rloc[2 * l + 1];
-                    const int e = (ci - ri0) * RZ + (cj - rj0);
-                    const float Ac = sponge_at(ci, cj, nx, nz, m.fw, m.tab);
-                    if (PSV) {
-                        ru[row + l] = sU[e] * Ac;
-                        rw[row + l] = sW[e] * Ac;
+                stress_phase(std::integral_constant<bool, SML>{});
+            }
+            if (code) atomicMax(sh->check_flags + chk, code + (FWD ? 0 : 3));
+
+            // ---- receivers of the tile, after the sponge
+            if (rb1 > rb0) {
+                const gptr<float> ru = G(sh->rec_u), rv = G(sh->rec_v), rw = G(sh->rec_w);
+                const size_t row = (size_t)(FWD ? it : tb) * n_rec;
+                if (rec_one_pass) {
+                    if (r_l >= 0) {
+                        if (PSV) {
+                            ru[row + r_l] = sU[r_e] * r_A;
+                            rw[row + r_l] = sW[r_e] * r_A;
+                        }
+                        if (SH) rv[row + r_l] = sV[r_e] * r_A;
                     }
-                    if (SH) rv[row + l] = sV[e] * Ac;
+                } else {
+                    const int b = rb0, end = rb1;
+                    const gptr<const int> ritems = G(sh->rec_titems), rloc = G(sh->rec_loc);
+                    for (int q = b + t; q < end; q += NT) {
+                        const int l = ritems[q];
+                        const int ci = rloc[2 * l], cj = rloc[2 * l + 1];
+                        const int e = (ci - ri0) * RZ + (cj - rj0);
+                        const float Ac = sponge_at(ci, cj, nx, nz, m.fw, m.tab);
+                        if (PSV) {
+                            ru[row + l] = sU[e] * Ac;
+                            rw[row + l] = sW[e] * Ac;
+                        }
+                        if (SH) rv[row + l] = sV[e] * Ac;
+                    }
                 }
             }
+            started = true;
+            __syncthreads();
         }
-        started = true;
-        __syncthreads();
-    }
+    };
+    if (has_sm)
+        step_loop(std::true_type{});
+    else
+        step_loop(std::false_type{});
 
     // ---- write the tile's state (post-sponge velocities) and image
     const gptr<float> o_u = G(sh->out[0]), o_v = G(sh->out[1]), o_w = G(sh->out[2]);
