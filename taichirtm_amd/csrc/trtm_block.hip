@@ -439,10 +439,13 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
     }
 
     bool started = false;  // a step ran in this launch: LDS velocities await the sponge
-    // the step loop, compiled once per surface-matrix choice so that the
+    // the step loop, compiled once per surface-matrix choice and, for the
+    // small shapes, once with and once without the per-step list walks (taken
+    // when a tile's lists are too long for the per-launch tables), so that the
     // executed loop's code is contiguous
-    auto step_loop = [&](auto sm_loop) {
+    auto step_loop = [&](auto sm_loop, auto gen_loop) {
         constexpr bool SML = decltype(sm_loop)::value;
+        constexpr bool GEN = !HOIST || decltype(gen_loop)::value;
         for (int s = s_lo; s < s_hi; s++) {
             const int it = it_first + s;
             const int tb = nt - 1 - it;  // backward time index
@@ -705,7 +708,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
 
             // ---- injection into the region (one thread per cell, entries in order;
             // the small shapes inject in the velocity update above)
-            if (ig1 > ig0 && !inj_one_pass) {
+            if (GEN && ig1 > ig0 && !inj_one_pass) {
                 const gptr<const float> su = G(sh->su), sv = G(sh->sv), sw = G(sh->sw);
                 const size_t ntt = (size_t)nt;
                 {
@@ -950,7 +953,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
             if (rb1 > rb0) {
                 const gptr<float> ru = G(sh->rec_u), rv = G(sh->rec_v), rw = G(sh->rec_w);
                 const size_t row = (size_t)(FWD ? it : tb) * n_rec;
-                if (rec_one_pass) {
+                if (!GEN || rec_one_pass) {
                     if (r_l >= 0) {
                         if (PSV) {
                             ru[row + r_l] = sU[r_e] * r_A;
@@ -978,10 +981,20 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
             __syncthreads();
         }
     };
+    auto run_loop = [&](auto sm_loop) {
+        if constexpr (HOIST) {
+            if (!inj_one_pass || !rec_one_pass)
+                step_loop(sm_loop, std::true_type{});
+            else
+                step_loop(sm_loop, std::false_type{});
+        } else {
+            step_loop(sm_loop, std::true_type{});
+        }
+    };
     if (has_sm)
-        step_loop(std::true_type{});
+        run_loop(std::true_type{});
     else
-        step_loop(std::false_type{});
+        run_loop(std::false_type{});
 
     // ---- write the tile's state (post-sponge velocities) and image
     const gptr<float> o_u = G(sh->out[0]), o_v = G(sh->out[1]), o_w = G(sh->out[2]);

@@ -412,6 +412,27 @@ def test_irregular_isnaps_bitwise(system, shape):
         assert not same(bo["result"][k], regular["result"][k])
 
 
+@pytest.mark.parametrize("system", ["psv", "sh"])
+def test_small_tiles_long_injection_lists_bitwise(system):
+    """The 16 x 16 tiles, 8 steps, with receivers on every cell of three rows
+    under the surface: a tile's region then holds 96 receiver groups, more
+    than its per-launch table takes, so the backward walks its list every
+    step (the loop variant with the per-step list walks)."""
+    _case("blocked 16x16x8 %s, 168 receivers in three rows" % system)
+    c = small_psv(nt=97) if system == "psv" else small_sh(nt=97)
+    kw = c.kwargs(c.shots[0])
+    nx = kw["mu_field"].shape[0]
+    rec = np.array([[i, j] for j in (1, 2, 3) for i in range(2, nx - 2)], np.int32)
+    kw["recv_loc_field"] = rec
+    kw["num_receivers"] = len(rec)
+    wav = c.wavelets(c.shots[0])
+    opts = dict(tile_x=16, tile_z=16, block_steps=8)
+    h = run_hip(kw, wav, isnap=2, kernel="blocked", fw_opts=opts, bw_opts=opts)
+    assert h["bw"].block_config() == (16, 16, 1024, 8)
+    o = run_oracle(kw, wav, isnap=2)
+    assert_bitwise(h, o)
+
+
 WAVE_CFGS = {"psv": [(16, 4), (8, 4), (16, 2), (8, 2)],
              "sh": [(16, 4), (8, 4), (24, 4), (16, 2)],
              "both": [(8, 4), (8, 2)]}
