@@ -433,6 +433,51 @@ def test_small_tiles_long_injection_lists_bitwise(system):
     assert_bitwise(h, o)
 
 
+TOPO_CFGS = [("psv", (16, 16, 8)), ("psv", (24, 56, 4)), ("psv", (48, 64, 4)),
+             ("psv", (32, 128, 4)), ("sh", (16, 16, 8)), ("sh", (48, 64, 8)),
+             ("sh", (32, 48, 8)), ("both", (16, 32, 4))]
+
+
+@pytest.mark.parametrize("system,cfg,long_lists",
+                         [(s, c, False) for s, c in TOPO_CFGS] +
+                         [("psv", (16, 16, 8), True), ("sh", (16, 16, 8), True)])
+def test_blocked_configurations_topography_bitwise(system, cfg, long_lists):
+    """The step loop compiled for a surface matrix (forward_modeling.py:412-419),
+    for the shape families of the blocked kernel: one or two cells per thread
+    (per-launch tables, and with receivers on three full rows the per-step
+    list walks), branch-free up to four cells per thread, per-cell branches
+    beyond; SH and both systems too (syz is masked as well)."""
+    tx, tz, k = cfg
+    _case("blocked topography %s %s long_lists=%s" % (system, cfg, long_lists))
+    c = small_psv(nt=97) if system != "sh" else small_sh(nt=97)
+    kw = c.kwargs(c.shots[0])
+    nx, nz = kw["mu_field"].shape
+    sm = np.ones((nx, nz), np.float32)
+    for i in range(nx):
+        sm[i, : 2 + (i // 7) % 3] = 0.0
+    rows = (4, 5, 6) if long_lists else (5,) if system == "sh" else ()
+    if system == "sh":
+        # source and receivers below the masked rows: SH rows under a zero
+        # syz on both sides exchange nothing with the rows below
+        kw["src_loc_field"] = np.array([[nx // 2, 8]], np.int32)
+    if rows:
+        rec = np.array([[i, j] for j in rows for i in range(2, nx - 2)], np.int32)
+        kw["recv_loc_field"] = rec
+        kw["num_receivers"] = len(rec)
+    w = configs.ricker(40.0, c.nt, c.dt)
+    z = np.zeros_like(w)
+    comps = {"psv": (w, z, w), "sh": (z, w, z), "both": (w, w, w)}[system]
+    wav = tuple(x[None] for x in comps)
+    opts = dict(tile_x=tx, tile_z=tz, block_steps=k)
+    h = run_hip(kw, wav, isnap=3, kernel="blocked", surface=sm, fw_opts=opts, bw_opts=opts)
+    assert h["bw"].block_config()[0:2] == (tx, tz) and h["bw"].block_config()[3] == k
+    o = run_oracle(kw, wav, isnap=3, surface=sm)
+    assert_bitwise(h, o)
+    comp = 2 if system != "sh" else 1
+    assert np.abs(o["image"][comp]).max() > 0
+    assert not same(o["image"][comp], run_oracle(kw, wav, isnap=3)["image"][comp])
+
+
 WAVE_CFGS = {"psv": [(16, 4), (8, 4), (16, 2), (8, 2)],
              "sh": [(16, 4), (8, 4), (24, 4), (16, 2)],
              "both": [(8, 4), (8, 2)]}
