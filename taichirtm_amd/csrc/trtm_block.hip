@@ -542,10 +542,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
             } else if constexpr (SH && !PSV) {
                 if (img) {
     #pragma unroll
-                    for (int k = 0; k < NC; k++) {
-                        fimg[k] = snap_v[s_off + gc[k]];
-                        if (ssponge) fimg[k] = fimg[k] * A[k];
-                    }
+                    for (int k = 0; k < NC; k++) fimg[k] = snap_v[s_off + gc[k]];
                 }
             }
             if constexpr (PSV && !SH && !PIPE) {
@@ -554,13 +551,17 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                     for (int k = 0; k < NC; k++) {
                         fimu[k] = snap_u[s_off + gc[k]];
                         fimw[k] = snap_w[s_off + gc[k]];
-                        if (ssponge) {
-                            fimu[k] = fimu[k] * A[k];
-                            fimw[k] = fimw[k] * A[k];
-                        }
                     }
                 }
             }
+            // The image's forward wavefield of slot k, times the sponge where
+            // the snapshots hold unsponged velocities.  Loads issued in this
+            // step are sponged at the use, so that they are waited for a phase
+            // after they were issued, not right away.
+            auto fwd_img = [&](float f, int k) {
+                if constexpr (PIPE) return f;
+                else return ssponge ? f * A[k] : f;
+            };
 
             // ---- velocity update (region)
             // SH-only kernels, without a surface matrix branch-free over every
@@ -783,7 +784,7 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                     if (fl[k] & 4u) {  // the tile's cell: outputs
                         const float v = sV[e] * A[k];
                         if (snap) snap_v[s_off + c] = v;
-                        if (img) aV[k] = aV[k] + fimg[k] * v;
+                        if (img) aV[k] = aV[k] + fwd_img(fimg[k], k) * v;
                         if (chk >= 0 && not_finite_or_big(v) && code < 2) code = 2;
                     }
                 }
@@ -832,8 +833,8 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                             }
                             if (img) {
                                 if constexpr (!SH) {
-                                    aU[k] = aU[k] + fimu[k] * u;
-                                    aW[k] = aW[k] + fimw[k] * w;
+                                    aU[k] = aU[k] + fwd_img(fimu[k], k) * u;
+                                    aW[k] = aW[k] + fwd_img(fimw[k], k) * w;
                                 } else {
                                     float fu = snap_u[s_off + c];
                                     float fw = snap_w[s_off + c];
@@ -915,15 +916,15 @@ __global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_block(KArgs m,
                     const float u = sU[e] * A[k], w = sW[e] * A[k];
                     if constexpr (PIPE) {  // the image of a slot outside the tile is never written
                         if (img) {
-                            aU[k] = aU[k] + fimu[k] * u;
-                            aW[k] = aW[k] + fimw[k] * w;
+                            aU[k] = aU[k] + fwd_img(fimu[k], k) * u;
+                            aW[k] = aW[k] + fwd_img(fimw[k], k) * w;
                         }
                     }
                     if (fl[k] & 4u) {
                         if constexpr (!PIPE) {
                             if (img) {
-                                aU[k] = aU[k] + fimu[k] * u;
-                                aW[k] = aW[k] + fimw[k] * w;
+                                aU[k] = aU[k] + fwd_img(fimu[k], k) * u;
+                                aW[k] = aW[k] + fwd_img(fimw[k], k) * w;
                             }
                         }
                         if (snap) {
