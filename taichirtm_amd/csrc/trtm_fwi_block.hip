@@ -1348,10 +1348,13 @@ static bool adj_cfg_fits(int systems, const BlockCfg& c) {
 // compiled shapes (tile rows, tile columns, threads, steps per launch)
 static const BlockCfg kFwiCfg[] = {{24, 64, 768, 4},  {32, 64, 1024, 4}, {16, 64, 512, 4},
                                    {16, 64, 512, 8},  {32, 64, 1024, 2}, {16, 64, 256, 2},
-                                   {16, 48, 512, 4},  {24, 32, 512, 4}};
-// (the 512-thread 16 x 48 and 24 x 32 shapes keep the P-SV adjoint within half
-// the LDS of a CU, so two workgroups share one and each one's barrier waits
-// overlap the other's work.  Measured slower on the 4000 x 2000 grid, 5 shots:
+                                   {16, 48, 512, 4},  {24, 32, 512, 4}, {20, 40, 512, 4}};
+// (the 512-thread 16 x 48, 24 x 32 and 20 x 40 shapes keep the P-SV adjoint
+// within half the LDS of a CU, so two workgroups share one and each one's
+// barrier waits overlap the other's work; 20 x 40 has the fewest halo cells
+// per tile cell of them with 40-cell columns, the fastest adjoint measured on
+// the 4000 x 2000 grid, 5 shots, 4 % ahead of 16 x 48; 22 x 38, 28 x 30 and
+// 26 x 32 fell between 20 x 40 and 16 x 48 or behind it.  Measured slower:
 // 16 x 48 and 24 x 32 with 8 steps per launch and 1024 threads, which read the
 // state, medium and gradients half as often per step but leave one workgroup
 // per CU (+22-25 %), and 16 x 32 with 320 threads, three per CU (+45 %))
@@ -1509,6 +1512,7 @@ static int launch_cfg(const BlockCfg& c, bool adjoint, bool hist, const S& src, 
     TRTM_FB(24, 64, 768, 4)
     TRTM_FB(16, 48, 512, 4)
     TRTM_FB(24, 32, 512, 4)
+    TRTM_FB(20, 40, 512, 4)
 #undef TRTM_FB
     // forward-only shapes
 #define TRTM_FF(TX, TZ, NT, K)                                                                  \
@@ -1691,6 +1695,7 @@ static int launch_scat_cfg(const BlockCfg& c, const ByArray4<FS>& src, int shots
     TRTM_FS(16, 64, 256, 2)
     TRTM_FS(16, 48, 512, 4)
     TRTM_FS(24, 32, 512, 4)
+    TRTM_FS(20, 40, 512, 4)
     TRTM_FS(32, 64, 1024, 8)
     TRTM_FS(24, 64, 768, 8)
     TRTM_FS(48, 64, 1024, 8)

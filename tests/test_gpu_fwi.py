@@ -236,7 +236,7 @@ def test_scattered_residual_same_bits_for_every_shape():
     r32 = [_t(x, torch.float32) for x in mt]
     ref = None
     for tx, tz, k in ((24, 64, 4), (32, 64, 4), (16, 64, 8), (16, 64, 2), (16, 48, 4),
-                      (24, 32, 4), (32, 64, 8), (24, 64, 8)):
+                      (24, 32, 4), (20, 40, 4), (32, 64, 8), (24, 64, 8)):
         p = ElasticFWI(*grid, shots, precision="float32", checkpoint_every=30, history_bytes=0,
                        kernel="blocked", tile_x=tx, tile_z=tz, block_steps=k)
         res = torch.stack(p.residual(*m32, r32))
@@ -361,7 +361,7 @@ def test_fwi_forward_records_equal_forward_modeling():
 
 
 BLOCK_SHAPES = [(24, 64, 4), (32, 64, 4), (16, 64, 4), (16, 64, 8), (32, 64, 2), (16, 64, 2),
-                (16, 48, 4), (24, 32, 4)]
+                (16, 48, 4), (24, 32, 4), (20, 40, 4)]
 
 
 def _uneven(shots, nts, spreads):
@@ -448,7 +448,7 @@ def test_blocked_kernels_bitwise_equal_step_kernels(system, shape):
         assert torch.equal(got[2][key], ref[2][key]), key
 
 
-INNER_SHAPES = [(16, 48, 4), (24, 32, 4), (32, 64, 4), (16, 64, 8)]
+INNER_SHAPES = [(16, 48, 4), (24, 32, 4), (20, 40, 4), (32, 64, 4), (16, 64, 8)]
 
 
 def _inner_tiles(nx, nz, fw, shape, points):
