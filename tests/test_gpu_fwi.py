@@ -385,7 +385,7 @@ def _records_and_gradient(grid, shots, m0, mt, adjoint_shape=None, **kw):
     return recs, g, {k: v.clone() for k, v in p.coefficient_gradient.items()}
 
 
-FORWARD_ONLY_SHAPES = [(32, 64, 8), (24, 64, 8), (48, 64, 8)]
+FORWARD_ONLY_SHAPES = [(32, 64, 8), (24, 64, 8), (48, 64, 8), (48, 48, 8), (40, 56, 8)]
 
 
 @pytest.mark.parametrize("shape", FORWARD_ONLY_SHAPES, ids=lambda s: "%dx%dk%d" % s)
