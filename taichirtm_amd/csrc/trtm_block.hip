@@ -1048,7 +1048,9 @@ static const BlockCfg kCfgBoth[] = {{16, 64, 256, 2}, {16, 64, 256, 4}, {16, 128
 // measured slower on the 200 x 200 grid; 24 x 56 with 512 threads holds four
 // cells per thread, branch-free, two workgroups per CU: the batch and salt
 // jobs' choice; 32 x 64 with 1024 threads and K = 8 (four cells, one
-// workgroup per CU) was slower there)
+// workgroup per CU) was slower there, and the tuner kept 24 x 56 over 28 x 48
+// and 32 x 40 with 512 threads, whose regions hold slightly fewer halo cells
+// per tile cell but shorter columns)
 
 static void cfg_list(int systems, const BlockCfg** list, int* n) {
     if (systems == TRTM_SH) {
