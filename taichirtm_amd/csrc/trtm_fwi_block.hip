@@ -1706,6 +1706,8 @@ static int launch_scat_cfg(const BlockCfg& c, const ByArray4<FS>& src, int shots
     TRTM_FS(32, 64, 1024, 8)
     TRTM_FS(24, 64, 768, 8)
     TRTM_FS(48, 64, 1024, 8)
+    TRTM_FS(48, 48, 1024, 8)
+    TRTM_FS(40, 56, 1024, 8)
 #undef TRTM_FS
     return fail(TRTM_ERR_UNSUPPORTED, "scattered FWI shape %dx%d/%d/%d not compiled", c.tx, c.tz,
                 c.nt, c.k);

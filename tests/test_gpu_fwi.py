@@ -235,8 +235,11 @@ def test_scattered_residual_same_bits_for_every_shape():
     m32 = [_t(x, torch.float32) for x in m0]
     r32 = [_t(x, torch.float32) for x in mt]
     ref = None
-    for tx, tz, k in ((24, 64, 4), (32, 64, 4), (16, 64, 8), (16, 64, 2), (16, 48, 4),
-                      (24, 32, 4), (20, 40, 4), (32, 64, 8), (24, 64, 8)):
+    from taichirtm_amd import _lib, autotune
+    listed = [c[1:] for c in autotune.fwi_candidates(_lib.PSV, "scattered")]
+    # every shape the library lists for the pass (the tuner tries them all)
+    assert (48, 48, 8) in listed and (24, 64, 4) in listed
+    for tx, tz, k in listed:
         p = ElasticFWI(*grid, shots, precision="float32", checkpoint_every=30, history_bytes=0,
                        kernel="blocked", tile_x=tx, tile_z=tz, block_steps=k)
         res = torch.stack(p.residual(*m32, r32))
