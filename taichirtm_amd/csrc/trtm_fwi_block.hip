@@ -1356,7 +1356,8 @@ static const BlockCfg kFwiCfg[] = {{24, 64, 768, 4},  {32, 64, 1024, 4}, {16, 64
 // barrier waits overlap the other's work; 20 x 40 has the fewest halo cells
 // per tile cell of them with 40-cell columns, the fastest adjoint measured on
 // the 4000 x 2000 grid, 5 shots, 4 % ahead of 16 x 48; 22 x 38, 28 x 30 and
-// 26 x 32 fell between 20 x 40 and 16 x 48 or behind it.  Measured slower:
+// 26 x 32 fell between 20 x 40 and 16 x 48 or behind it; 20 x 40 with 448
+// threads, three cells for every thread, was 13 % slower.  Measured slower:
 // 16 x 48 and 24 x 32 with 8 steps per launch and 1024 threads, which read the
 // state, medium and gradients half as often per step but leave one workgroup
 // per CU (+22-25 %), and 16 x 32 with 320 threads, three per CU (+45 %))
