@@ -1369,7 +1369,8 @@ static const BlockCfg kFwiDefaultFwd = {24, 64, 768, 4}, kFwiDefaultAdj = {16, 4
 // halve the state and coefficient traffic per step of the checkpointing and
 // history passes; the 1024-thread 48 x 48 and 40 x 56 tiles fill half the LDS
 // of a CU with fewer halo cells per tile cell than 32 x 64 (the salt job's
-// checkpointing pass 5 %, its history pass 2 % faster per launch)
+// checkpointing pass 5 %, its history pass 2 % faster per launch; the tuner
+// kept them over 34 x 64 and 36 x 60)
 static const BlockCfg kFwiFwdOnly[] = {{32, 64, 1024, 8}, {24, 64, 768, 8}, {48, 64, 1024, 8},
                                        {48, 48, 1024, 8}, {40, 56, 1024, 8}};
 
