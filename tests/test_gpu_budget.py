@@ -99,3 +99,34 @@ def test_salt_full_grid_within_budget():
         assert bool(torch.isfinite(x).all()) and float(x.abs().max()) > 0
     for c in "uw":
         assert bool(torch.isfinite(img[c]).all()) and float(img[c].abs().max()) > 0
+
+
+def test_shared_forward_image_branch_free_shape(monkeypatch):
+    """The RTM backward of the shared-forward job in the salt job's shape (24 x 56
+    tiles, 512 threads, 4 steps: four cells per thread, branch-free, its forward
+    wavefield the history's velocities times the sponge, sponged where the
+    image uses them) gives the image of the separate checkpointed RTM bit for
+    bit."""
+    _case("budget shared forward, RTM backward 24x56x4, 160x100x300, 3 shots")
+    true, start, shots, obs, with_obs = _survey(160, 100, 300, 3, (40, 120))
+    grid = (start.nx, start.nz, start.dx, start.dz, start.fs, start.fw)
+    m = (start.vp, start.vs, start.rho)
+    made = []
+
+    def pinned(*a, **k):
+        made.append(BatchRTM(*a, tile_x=24, tile_z=56, block_steps=4, **k))
+        return made[-1]
+    monkeypatch.setattr(budget, "BatchRTM", pinned)
+    job = budget.GradientAndImage(*grid, with_obs, 1 << 40, kernel="blocked", share_forward=True)
+    _, _, img = job.run(*m)
+    monkeypatch.undo()
+    assert made and made[0].bw[0].block_config() == (24, 56, 512, 4)
+    mu, lam = start.moduli()
+    p = budget.plan(start.nx, start.nz, with_obs, 1 << 40)
+    rtm = BatchRTM(start.nx, start.nz, start.dx, start.dz, start.fs, mu, lam, start.rho, start.fw,
+                   with_obs, imaging="checkpoint", checkpoint_every=p.L_rtm, kernel="blocked")
+    assert rtm.run() == [0] * 3
+    st = rtm.stacked_image()
+    for c in "uw":
+        assert torch.equal(st[c], img[c])
+        assert float(img[c].abs().max()) > 0
