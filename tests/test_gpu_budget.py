@@ -101,26 +101,31 @@ def test_salt_full_grid_within_budget():
         assert bool(torch.isfinite(img[c]).all()) and float(img[c].abs().max()) > 0
 
 
-def test_shared_forward_image_branch_free_shape(monkeypatch):
+@pytest.mark.parametrize("shape", [(24, 56, 512, 4), (16, 16, 1024, 8)],
+                         ids=lambda s: "%dx%dk%d" % (s[0], s[1], s[3]))
+def test_shared_forward_image_branch_free_shape(monkeypatch, shape):
     """The RTM backward of the shared-forward job in the salt job's shape (24 x 56
     tiles, 512 threads, 4 steps: four cells per thread, branch-free, its forward
     wavefield the history's velocities times the sponge, sponged where the
-    image uses them) gives the image of the separate checkpointed RTM bit for
-    bit."""
-    _case("budget shared forward, RTM backward 24x56x4, 160x100x300, 3 shots")
+    image uses them) and in the small-grid shape (16 x 16, 8 steps: the
+    forward wavefield loaded a step ahead) gives the image of the separate
+    checkpointed RTM bit for bit."""
+    tx, tz, nt_, k = shape
+    _case("budget shared forward, RTM backward %s, 160x100x300, 3 shots" % (shape,))
     true, start, shots, obs, with_obs = _survey(160, 100, 300, 3, (40, 120))
     grid = (start.nx, start.nz, start.dx, start.dz, start.fs, start.fw)
     m = (start.vp, start.vs, start.rho)
     made = []
+    k_ = k
 
     def pinned(*a, **k):
-        made.append(BatchRTM(*a, tile_x=24, tile_z=56, block_steps=4, **k))
+        made.append(BatchRTM(*a, tile_x=tx, tile_z=tz, block_steps=k_, **k))
         return made[-1]
     monkeypatch.setattr(budget, "BatchRTM", pinned)
     job = budget.GradientAndImage(*grid, with_obs, 1 << 40, kernel="blocked", share_forward=True)
     _, _, img = job.run(*m)
     monkeypatch.undo()
-    assert made and made[0].bw[0].block_config() == (24, 56, 512, 4)
+    assert made and made[0].bw[0].block_config() == shape
     mu, lam = start.moduli()
     p = budget.plan(start.nx, start.nz, with_obs, 1 << 40)
     rtm = BatchRTM(start.nx, start.nz, start.dx, start.dz, start.fs, mu, lam, start.rho, start.fw,
